@@ -1,0 +1,165 @@
+"""Benchmark of the per-cell HDG hot path on one MI355X.
+
+Workload (the configuration the metric is quoted on): 2D silicon p-n diode,
+256 x 256 quadrilateral cells, HDG degree 2, V/n/p, FP64 (81 cell dofs,
+36 trace dofs per cell), state = thermal-equilibrium first guess under a
+0.3 V bias boundary condition.  One step = pass 1 of a Newton step over all
+65,536 cells: assembly, static condensation and the deterministic scatter
+into the global trace system (hdg_condense), inputs resident in HBM.
+
+Prints one JSON line.  ``value`` = cells assembled+condensed per second;
+``roofline`` = the condense kernel against MI355X's FP64 peak with the
+dense algorithmic FLOP count per cell; ``cpu_baseline`` = the oracle (the
+reference-faithful C++ restatement: Gauss-Jordan inverse + mmult, WorkStream
+threading, serial in-order scatter) timed on the host on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def flops_per_cell(N, T):
+    """dense algorithmic count for pass 1: LU of the N x N cell block,
+    solves for the T trace columns and the rhs, Schur product."""
+    return 2 * N ** 3 / 3 + 2 * N ** 2 * (T + 1) + 2 * T * N * (T + 1)
+
+
+def bytes_per_cell(N, T):
+    """pass 1: read the cell state (N) and the trace state (T), write the
+    condensed block (T*T) and right-hand side (T)."""
+    return 8 * (N + T + T * T + T)
+
+
+def cpu_baseline(n_sample, degree, threads, bias):
+    """oracle pass 1 on an n_sample^2 diode of the same per-cell shape."""
+    import numpy as np
+
+    from ddhdg_amd import configs
+    from oracle import oracle as O
+    from tests import helpers as H
+
+    prob, params, adim = configs.diode(2, n_sample, degree, bias=bias)
+    o = H.oracle_for(prob, params, adim)
+    o.set_state(*H.equilibrium_guess(prob, adim, degree))
+    o.pass1(False, threads, 64)  # warm-up on a few cells
+    times = [o.pass1(False, threads) for _ in range(3)]
+    t = float(np.median(times))
+    return o.ncells / t, o.ncells, t
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=256)
+    ap.add_argument("--degree", type=int, default=2)
+    ap.add_argument("--bias", type=float, default=0.3)
+    ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the per-cell path has no multi-GPU form in this "
+              "build (see DESIGN.md for how it would shard); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from ddhdg_amd import configs
+    from ddhdg_amd.np_solver import NPSolver
+
+    prob, params, adim = configs.diode(2, args.cells, args.degree, bias=args.bias)
+    s = NPSolver(prob, params, adim, device="cuda")
+    s.compute_local_charge_neutrality()
+    s.project_densities_from_potential()
+    ctx = s.ctx
+    N, T = ctx.NC, ctx.T
+    vals = torch.empty(ctx.nnz, dtype=torch.float64, device="cuda")
+    rhs = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
+    dirv = torch.zeros(ctx.ntrace, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def step():
+        ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps       # two color launches per step
+    ms_per_step = wall * 1e3 / args.steps
+    cells = ctx.ncells
+    value = cells * args.steps / wall
+    # reconstruct pass (reported beside the metric)
+    dxt = torch.zeros_like(s.x_trace)
+    dxc = torch.empty_like(s.x_cell)
+    for _ in range(2):
+        s.ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc)
+    torch.cuda.synchronize()
+    r0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc)
+    torch.cuda.synchronize()
+    recon_ms = (time.perf_counter() - r0) * 1e3 / args.steps
+
+    launch_ms = kernel_ms / 2.0
+    cells_per_launch = cells / 2.0
+    achieved_tflops = flops_per_cell(N, T) * cells_per_launch / (launch_ms * 1e-3) / 1e12
+    achieved_gbs = bytes_per_cell(N, T) * cells_per_launch / (launch_ms * 1e-3) / 1e9
+    line = {
+        "metric": "cells assembled+condensed/s",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"2D p-n diode {args.cells}x{args.cells} quads, HDG degree "
+                               f"{args.degree}, V/n/p, FP64, pass 1 (assembly+condensation+scatter)",
+                   "cells": cells, "cell_dofs": N, "trace_dofs_per_cell": T,
+                   "global_trace_dofs": ctx.ntrace, "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+                     "traffic": None},
+        "hbm": {"achieved_GBs": achieved_gbs, "peak_GBs": HBM_PEAK_GBS,
+                "frac": achieved_gbs / HBM_PEAK_GBS, "bytes_per_cell": bytes_per_cell(N, T)},
+        "kernel_ms_per_launch": launch_ms,
+        "reconstruct_ms": recon_ms,
+    }
+    if not args.no_cpu_baseline:
+        threads = min(os.cpu_count() or 1, 16)
+        v, nc, t = cpu_baseline(args.cpu_sample, args.degree, threads, args.bias)
+        line["cpu_baseline"] = {
+            "value": v, "unit": "cells/s", "cores": threads, "kind": "port",
+            "sample": f"oracle pass 1 over a {args.cpu_sample}x{args.cpu_sample} diode of the same "
+                      f"per-cell shape ({nc} cells, median of 3 runs, {t:.2f} s each)"}
+        line["speedup_vs_cpu"] = value / v
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,102 @@
+"""The silicon diode workloads the path is measured on.
+
+Shared settings (silicon, 300 K): eps_r = 11.7, mu_n = 0.14, mu_p = 0.045
+m^2/Vs, Nc = Nv = n_i = 1.45e16 m^-3 with Ec = Ev = 0, SRH with
+n_i = 1.45e16 m^-3 and tau_n = tau_p = 1e-6 s, HDG tau = 1 in rescaled units,
+use_cell flux, Adimensionalizer(1e-6 m, q/k_B, 1e23 m^-3, 1).  Mesh
+coordinates are in micrometres (scale_length units).  Ohmic contacts:
+Dirichlet on boundary ids 0 (x = 0) and 1 (x = 1 um), V = charge-neutral
+potential (+ bias on the x = 1 contact), n and p the equilibrium densities;
+no condition on the other ids (zero normal flux).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .fe import BoxMesh
+from .problem import (DIRICHLET, EPSILON0, Adimensionalizer, BoundaryConditionHandler,
+                      FixedTauNPSolverParameters, HomogeneousMobility, HomogeneousPermittivity,
+                      HomogeneousProblem, NonlinearSolverParameters,
+                      ShockleyReadHallFixedTemperature, compute_density,
+                      local_charge_neutrality_potential, n, p, V)
+
+NI = 1.45e16
+T_K = 300.0
+
+
+def step_doping(nd=1e23, na=1e23, junction=0.5):
+    def f(x):
+        return np.where(x[..., 0] < junction, nd, -na)
+    return f
+
+
+def gaussian_doping(peak=1e24, sigma=0.1, background=1e22, center=(0.5, 0.5, 0.5)):
+    c = np.asarray(center)
+
+    def f(x):
+        r2 = np.sum((x - c[: x.shape[-1]]) ** 2, axis=-1)
+        return background - peak * np.exp(-r2 / (2 * sigma ** 2))
+    return f
+
+
+def scaled_unit_tau(adim):
+    """physical tau values whose rescaled values are 1 (the diffusion scale
+    D/h of the meshes used here).  The reference default tau = 1 leaves the
+    rescaled n, p stabilisation at 1e-12, where the local problems are
+    nearly singular (condition ~1e11)."""
+    return tuple(adim.get_tau_rescaling_factor(c) for c in (V, n, p))
+
+
+def diode(dim, n_cells, degree, doping=None, bias=0.0, tau=None, nonlinear=None):
+    """(problem, parameters, adimensionalizer) of a box diode."""
+    mesh = BoxMesh(dim, [n_cells] * dim, [0.0] * dim, [1.0] * dim)
+    doping = doping or step_doping()
+    bh = BoundaryConditionHandler()
+    prob = HomogeneousProblem(
+        mesh=mesh,
+        permittivity=HomogeneousPermittivity(11.7 * EPSILON0),
+        n_mobility=HomogeneousMobility(0.14, n),
+        p_mobility=HomogeneousMobility(0.045, p),
+        recombination_term=ShockleyReadHallFixedTemperature(NI, 1e-6, 1e-6),
+        temperature=T_K,
+        doping=doping,
+        boundary_handler=bh,
+        conduction_band_density=NI,
+        valence_band_density=NI,
+    )
+    set_contacts(prob, bias)
+    adim = Adimensionalizer(1e-6, 1.602176634e-19 / 1.380649e-23, 1e23, 1.0)
+    tau = tau or scaled_unit_tau(adim)
+    params = FixedTauNPSolverParameters(
+        degree, degree, degree,
+        nonlinear or NonlinearSolverParameters(1e-10, 1e-10, 100, 1.0),
+        tau[0], tau[1], tau[2])
+    return prob, params, adim
+
+
+def contact_potential(prob, x):
+    return local_charge_neutrality_potential(prob, prob.doping(x), prob.temperature)
+
+
+def set_contacts(prob, bias=0.0):
+    """ohmic contacts on ids 0 and 1; the bias raises the x = 1 contact."""
+    bh = prob.boundary_handler
+
+    def Vc(b):
+        return lambda x: contact_potential(prob, x) + b
+
+    def dens(c):
+        return lambda x: compute_density(prob, c, 0.0, contact_potential(prob, x), prob.temperature)
+
+    for bid, b in ((0, 0.0), (1, bias)):
+        bh.add_boundary_condition(bid, DIRICHLET, V, Vc(b))
+        bh.add_boundary_condition(bid, DIRICHLET, n, dens(n))
+        bh.add_boundary_condition(bid, DIRICHLET, p, dens(p))
+
+
+def set_bias(solver, bias):
+    """bias continuation step: same-type replacement of the x = 1 contact
+    potential (NPSolver::replace_boundary_condition)."""
+    prob = solver.problem
+    solver.replace_boundary_condition(
+        1, DIRICHLET, V, lambda x: contact_potential(prob, x) + bias)

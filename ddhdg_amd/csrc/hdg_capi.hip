@@ -1,0 +1,556 @@
+// hdg_capi.hip -- C ABI of the gfx950 per-cell HDG path (include/ddhdg_hip.h).
+//
+// Host side of a context: structured-mesh topology, the restricted trace
+// matrix pattern and its per-cell scatter map, the reference-cell tables and
+// the launches of the kernels in hdg_kernels.hip.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ddhdg_hip.h"
+#include "hdg_kernels.h"
+#include "hdg_tables.h"
+
+namespace
+{
+  thread_local std::string g_err;
+
+  int fail(const std::string &m, int code = -1)
+  {
+    g_err = m;
+    return code;
+  }
+
+  int hip_fail(hipError_t e, const char *what)
+  {
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return -3;
+  }
+} // namespace
+
+struct hdg_ctx
+{
+  hdg_mesh_desc mesh;
+  hdg_phys      phys;
+  int           d, k, nb, nf, qc, qf, nfc, NC;
+  long long     ncells, nfaces;
+  int           n[3];
+  double        h[3], lo[3];
+  long long     face_off[4], face_dims[3][3];
+  int           en[3], ne, encomp[3];
+  hdg::HostTables ht;
+  // block pattern: per face, sorted faces it couples with
+  std::vector<long long>     blk_cols;  // concatenated
+  std::vector<long long>     blkptr;    // nfaces+1
+  std::vector<int>           nbk;       // nfaces
+  std::vector<unsigned char> cslot;     // ncells*nfc*nfc
+  std::vector<int>           color_cells[2];
+  // device
+  double         *d_tab = nullptr;
+  hdg::DevTables  tb{};
+  double         *d_doping = nullptr, *d_dirv = nullptr, *d_neuv = nullptr;
+  long long      *d_blkptr = nullptr;
+  int            *d_nbk    = nullptr;
+  unsigned char  *d_cslot  = nullptr;
+  int            *d_cells[2] = {nullptr, nullptr};
+};
+
+namespace
+{
+  long long face_index(const hdg_ctx *C, int a, const int pos[3])
+  {
+    const long long *m = C->face_dims[a];
+    return C->face_off[a] + pos[0] + m[0] * (pos[1] + m[1] * (long long)pos[2]);
+  }
+
+  void cell_coords(const hdg_ctx *C, long long c, int ic[3])
+  {
+    ic[0]       = (int)(c % C->n[0]);
+    long long r = c / C->n[0];
+    ic[1]       = (int)(r % C->n[1]);
+    ic[2]       = (int)(r / C->n[1]);
+  }
+
+  long long cell_face(const hdg_ctx *C, long long c, int lf)
+  {
+    int ic[3];
+    cell_coords(C, c, ic);
+    ic[lf / 2] += lf % 2;
+    return face_index(C, lf / 2, ic);
+  }
+
+  int boundary_id(const hdg_ctx *C, long long c, int lf)
+  {
+    int ic[3];
+    cell_coords(C, c, ic);
+    const int a = lf / 2, s = lf % 2;
+    if ((s == 0 && ic[a] == 0) || (s == 1 && ic[a] == C->n[a] - 1))
+      return lf;
+    return -1;
+  }
+
+  void set_enabled(hdg_ctx *C, int V, int n, int p)
+  {
+    C->en[0] = V != 0;
+    C->en[1] = n != 0;
+    C->en[2] = p != 0;
+    C->ne    = 0;
+    for (int c = 0; c < 3; ++c)
+      if (C->en[c])
+        C->encomp[C->ne++] = c;
+    for (int c = C->ne; c < 3; ++c)
+      C->encomp[c] = 0;
+  }
+
+  template <class T>
+  int upload(T **dst, const std::vector<T> &src)
+  {
+    if (*dst)
+      {
+        hipFree(*dst);
+        *dst = nullptr;
+      }
+    if (src.empty())
+      return 0;
+    hipError_t e = hipMalloc((void **)dst, src.size() * sizeof(T));
+    if (e != hipSuccess)
+      return hip_fail(e, "hipMalloc");
+    e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+      return hip_fail(e, "hipMemcpy");
+    return 0;
+  }
+
+  hdg::KParams params(const hdg_ctx *C, int thermodyn)
+  {
+    hdg::KParams p{};
+    for (int a = 0; a < 3; ++a)
+      p.n[a] = C->n[a];
+    for (int a = 0; a < 4; ++a)
+      p.face_off[a] = C->face_off[a];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        p.face_dims[a][b] = C->face_dims[a][b];
+    for (int c = 0; c < 3; ++c)
+      {
+        p.en[c]     = C->en[c];
+        p.encomp[c] = C->encomp[c];
+        p.tau[c]    = C->phys.tau[c];
+      }
+    p.ne     = C->ne;
+    p.thermo = thermodyn ? 1 : 0;
+    p.eps    = C->phys.eps;
+    p.mun    = C->phys.mu_n;
+    p.mup    = C->phys.mu_p;
+    p.UT     = C->phys.U_T;
+    p.rec    = C->phys.recombination;
+    p.rec0   = C->phys.rec[0];
+    p.rec1   = C->phys.rec[1];
+    p.rec2   = C->phys.rec[2];
+    p.n_rf   = C->phys.n_rescale;
+    p.rec_rf = C->phys.rec_rescale;
+    p.kTq    = C->phys.kT_over_q;
+    p.ec_r   = C->phys.ec_r;
+    p.ev_r   = C->phys.ev_r;
+    p.nc     = C->phys.nc;
+    p.nv     = C->phys.nv;
+    for (int b = 0; b < 6; ++b)
+      for (int c = 0; c < 3; ++c)
+        p.bc[b][c] = C->phys.bc_type[b][c];
+    p.tb     = C->tb;
+    p.doping = C->d_doping;
+    p.dirv   = C->d_dirv;
+    p.neuv   = C->d_neuv;
+    p.blkptr = C->d_blkptr;
+    p.nbk    = C->d_nbk;
+    p.cslot  = C->d_cslot;
+    return p;
+  }
+} // namespace
+
+extern "C" {
+
+const char *hdg_last_error(void)
+{
+  return g_err.c_str();
+}
+
+int hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys, hdg_ctx **out)
+{
+  if (!mesh || !phys || !out)
+    return fail("hdg_ctx_create: null argument");
+  if (!hdg::supported(mesh->dim, mesh->degree))
+    return fail("hdg_ctx_create: (dim, degree) not supported: 2D k=1..3, 3D k=1", -2);
+  if (phys->flux_type != 0)
+    return fail("hdg_ctx_create: only the use_cell flux is supported", -2);
+  for (int a = 0; a < mesh->dim; ++a)
+    if (mesh->n_cells[a] < 1 || !(mesh->upper[a] > mesh->lower[a]))
+      return fail("hdg_ctx_create: invalid mesh extent", -2);
+  hdg_ctx *C = new hdg_ctx();
+  C->mesh    = *mesh;
+  C->phys    = *phys;
+  C->d       = mesh->dim;
+  C->k       = mesh->degree;
+  C->ncells  = 1;
+  for (int a = 0; a < 3; ++a)
+    {
+      C->n[a]  = a < C->d ? mesh->n_cells[a] : 1;
+      C->lo[a] = a < C->d ? mesh->lower[a] : 0;
+      C->h[a]  = a < C->d ? (mesh->upper[a] - mesh->lower[a]) / mesh->n_cells[a] : 1;
+      C->ncells *= C->n[a];
+    }
+  C->face_off[0] = 0;
+  for (int a = 0; a < 3; ++a)
+    {
+      for (int b = 0; b < 3; ++b)
+        C->face_dims[a][b] = C->n[b] + (a == b ? 1 : 0);
+      const long long cnt =
+        a < C->d ? C->face_dims[a][0] * C->face_dims[a][1] * C->face_dims[a][2] : 0;
+      C->face_off[a + 1] = C->face_off[a] + cnt;
+    }
+  C->nfaces = C->face_off[C->d];
+  if (!hdg::build_tables(C->d, C->k, C->h, C->ht))
+    {
+      delete C;
+      return fail("hdg_ctx_create: singular reference-cell mass matrix");
+    }
+  const hdg::HostTables &T = C->ht;
+  C->nb  = T.nb;
+  C->nf  = T.nf;
+  C->qc  = T.qc;
+  C->qf  = T.qf;
+  C->nfc = T.nfc;
+  C->NC  = 3 * (C->d + 1) * C->nb;
+  set_enabled(C, 1, 1, 1);
+
+  // block pattern and per-cell scatter slots
+  std::vector<std::vector<long long>> fcells(C->nfaces);
+  for (long long c = 0; c < C->ncells; ++c)
+    for (int lf = 0; lf < C->nfc; ++lf)
+      fcells[cell_face(C, c, lf)].push_back(c);
+  C->blkptr.assign(C->nfaces + 1, 0);
+  C->nbk.assign(C->nfaces, 0);
+  std::vector<std::vector<long long>> cols(C->nfaces);
+  for (long long f = 0; f < C->nfaces; ++f)
+    {
+      for (long long c : fcells[f])
+        for (int lf = 0; lf < C->nfc; ++lf)
+          cols[f].push_back(cell_face(C, c, lf));
+      std::sort(cols[f].begin(), cols[f].end());
+      cols[f].erase(std::unique(cols[f].begin(), cols[f].end()), cols[f].end());
+      C->nbk[f]         = (int)cols[f].size();
+      C->blkptr[f + 1] = C->blkptr[f] + C->nbk[f];
+    }
+  C->blk_cols.reserve(C->blkptr[C->nfaces]);
+  for (long long f = 0; f < C->nfaces; ++f)
+    C->blk_cols.insert(C->blk_cols.end(), cols[f].begin(), cols[f].end());
+  C->cslot.assign(C->ncells * C->nfc * C->nfc, 0);
+  for (long long c = 0; c < C->ncells; ++c)
+    for (int a = 0; a < C->nfc; ++a)
+      {
+        const long long fa = cell_face(C, c, a);
+        for (int b = 0; b < C->nfc; ++b)
+          {
+            const long long fb = cell_face(C, c, b);
+            const auto it = std::lower_bound(cols[fa].begin(), cols[fa].end(), fb);
+            C->cslot[(c * C->nfc + a) * C->nfc + b] = (unsigned char)(it - cols[fa].begin());
+          }
+      }
+  for (long long c = 0; c < C->ncells; ++c)
+    {
+      int ic[3];
+      cell_coords(C, c, ic);
+      C->color_cells[(ic[0] + ic[1] + ic[2]) & 1].push_back((int)c);
+    }
+
+  // device tables, one allocation
+  std::vector<const std::vector<double> *> parts = {
+    &T.JxWc, &T.phic, &T.gradc, &T.phif, &T.JxWf, &T.xi, &T.Bz, &T.Bt, &T.LDc,
+    &T.LDf,  &T.LCc,  &T.LCf,   &T.PhiMinvC, &T.PhiMinvF, &T.Minv, &T.MinvD, &T.MinvC,
+    &T.Mfinv};
+  std::vector<double> all;
+  std::vector<size_t> off;
+  for (auto *v : parts)
+    {
+      off.push_back(all.size());
+      all.insert(all.end(), v->begin(), v->end());
+      all.resize((all.size() + 31) / 32 * 32, 0.0); // 256-byte alignment
+    }
+  int rc = upload(&C->d_tab, all);
+  if (rc == 0)
+    {
+      const double *b    = C->d_tab;
+      hdg::DevTables &tb = C->tb;
+      tb.JxWc            = b + off[0];
+      tb.phic            = b + off[1];
+      tb.gradc           = b + off[2];
+      tb.phif            = b + off[3];
+      tb.JxWf            = b + off[4];
+      tb.xi              = b + off[5];
+      tb.Bz              = b + off[6];
+      tb.Bt              = b + off[7];
+      tb.LDc             = b + off[8];
+      tb.LDf             = b + off[9];
+      tb.LCc             = b + off[10];
+      tb.LCf             = b + off[11];
+      tb.PhiMinvC        = b + off[12];
+      tb.PhiMinvF        = b + off[13];
+      tb.Minv            = b + off[14];
+      tb.MinvD           = b + off[15];
+      tb.MinvC           = b + off[16];
+      tb.Mfinv           = b + off[17];
+    }
+  std::vector<long long> bp(C->blkptr.begin(), C->blkptr.end() - 1);
+  if (rc == 0)
+    rc = upload(&C->d_blkptr, bp);
+  if (rc == 0)
+    rc = upload(&C->d_nbk, C->nbk);
+  if (rc == 0)
+    rc = upload(&C->d_cslot, C->cslot);
+  for (int col = 0; col < 2 && rc == 0; ++col)
+    rc = upload(&C->d_cells[col], C->color_cells[col]);
+  if (rc == 0)
+    rc = upload(&C->d_doping, std::vector<double>(C->ncells * C->qc, 0.0));
+  if (rc == 0)
+    rc = upload(&C->d_dirv, std::vector<double>(C->nfaces * 3 * C->qf, 0.0));
+  if (rc == 0)
+    rc = upload(&C->d_neuv, std::vector<double>(C->nfaces * 3 * C->qf, 0.0));
+  if (rc != 0)
+    {
+      std::string m = g_err;
+      hdg_ctx_destroy(C);
+      return fail(m, rc);
+    }
+  *out = C;
+  return 0;
+}
+
+void hdg_ctx_destroy(hdg_ctx *C)
+{
+  if (!C)
+    return;
+  for (void *p : {(void *)C->d_tab, (void *)C->d_doping, (void *)C->d_dirv,
+                  (void *)C->d_neuv, (void *)C->d_blkptr, (void *)C->d_nbk,
+                  (void *)C->d_cslot, (void *)C->d_cells[0], (void *)C->d_cells[1]})
+    if (p)
+      hipFree(p);
+  delete C;
+}
+
+int hdg_sizes(const hdg_ctx *C, long long out[HDG_NSIZES])
+{
+  if (!C || !out)
+    return fail("hdg_sizes: null argument");
+  const long long tdf = (long long)C->ne * C->nf;
+  out[HDG_NCELLS] = C->ncells;
+  out[HDG_NFACES] = C->nfaces;
+  out[HDG_NC]     = C->NC;
+  out[HDG_QC]     = C->qc;
+  out[HDG_QF]     = C->qf;
+  out[HDG_NB]     = C->nb;
+  out[HDG_NF]     = C->nf;
+  out[HDG_NE]     = C->ne;
+  out[HDG_NS]     = (long long)C->ne * C->nb;
+  out[HDG_T]      = C->nfc * tdf;
+  out[HDG_NTRACE] = C->nfaces * tdf;
+  out[HDG_NNZ]    = C->blkptr[C->nfaces] * tdf * tdf;
+  return 0;
+}
+
+int hdg_set_enabled(hdg_ctx *C, int V, int n, int p)
+{
+  if (!C)
+    return fail("hdg_set_enabled: null context");
+  if (!(V || n || p))
+    return fail("hdg_set_enabled: no component enabled", -2);
+  set_enabled(C, V, n, p);
+  return 0;
+}
+
+int hdg_csr_pattern(const hdg_ctx *C, long long *rowptr, int *colind)
+{
+  if (!C || !rowptr || !colind)
+    return fail("hdg_csr_pattern: null argument");
+  const long long tdf = (long long)C->ne * C->nf;
+  if (C->nfaces * tdf > 2147483647LL)
+    return fail("hdg_csr_pattern: trace system too large for int32 columns", -2);
+  long long pos = 0;
+  for (long long f = 0; f < C->nfaces; ++f)
+    for (long long r = 0; r < tdf; ++r)
+      {
+        rowptr[f * tdf + r] = pos;
+        for (long long b = C->blkptr[f]; b < C->blkptr[f + 1]; ++b)
+          for (long long c = 0; c < tdf; ++c)
+            colind[pos++] = (int)(C->blk_cols[b] * tdf + c);
+      }
+  rowptr[C->nfaces * tdf] = pos;
+  return 0;
+}
+
+int hdg_dirichlet_dofs(const hdg_ctx *C, long long *idx)
+{
+  if (!C)
+    return fail("hdg_dirichlet_dofs: null context");
+  const long long tdf = (long long)C->ne * C->nf;
+  int             cnt = 0;
+  for (long long c = 0; c < C->ncells; ++c)
+    for (int lf = 0; lf < C->nfc; ++lf)
+      {
+        const int bid = boundary_id(C, c, lf);
+        if (bid < 0)
+          continue;
+        const long long f = cell_face(C, c, lf);
+        for (int ci = 0; ci < C->ne; ++ci)
+          if (C->phys.bc_type[bid][C->encomp[ci]] == 1)
+            for (int m = 0; m < C->nf; ++m)
+              {
+                if (idx)
+                  idx[cnt] = f * tdf + ci * C->nf + m;
+                ++cnt;
+              }
+      }
+  return cnt;
+}
+
+int hdg_cell_qpoints(const hdg_ctx *C, double *xyz)
+{
+  if (!C || !xyz)
+    return fail("hdg_cell_qpoints: null argument");
+  const int d = C->d, k = C->k;
+  for (long long c = 0; c < C->ncells; ++c)
+    {
+      int ic[3];
+      cell_coords(C, c, ic);
+      for (int q = 0; q < C->qc; ++q)
+        {
+          int r = q;
+          for (int a = 0; a < d; ++a)
+            {
+              const int qa = r % (k + 1);
+              r /= (k + 1);
+              xyz[(c * C->qc + q) * d + a] = C->lo[a] + (ic[a] + C->ht.qx1[qa]) * C->h[a];
+            }
+        }
+    }
+  return 0;
+}
+
+int hdg_face_qpoints(const hdg_ctx *C, double *xyz)
+{
+  if (!C || !xyz)
+    return fail("hdg_face_qpoints: null argument");
+  const int d = C->d, k = C->k;
+  for (int a = 0; a < d; ++a)
+    {
+      const long long *m = C->face_dims[a];
+      for (long long fi = 0; fi < C->face_off[a + 1] - C->face_off[a]; ++fi)
+        {
+          const long long pos[3] = {fi % m[0], (fi / m[0]) % m[1], fi / (m[0] * m[1])};
+          const long long f      = C->face_off[a] + fi;
+          for (int s = 0; s < C->qf; ++s)
+            {
+              double pt[3];
+              for (int b = 0; b < d; ++b)
+                pt[b] = C->lo[b] + pos[b] * C->h[b];
+              int r = s;
+              for (int mm = 0; mm < d - 1; ++mm)
+                {
+                  const int b = C->ht.tang[a][mm];
+                  pt[b] += C->ht.qx1[r % (k + 1)] * C->h[b];
+                  r /= (k + 1);
+                }
+              for (int b = 0; b < d; ++b)
+                xyz[(f * C->qf + s) * d + b] = pt[b];
+            }
+        }
+    }
+  return 0;
+}
+
+int hdg_set_doping(hdg_ctx *C, const double *doping_qp)
+{
+  if (!C || !doping_qp)
+    return fail("hdg_set_doping: null argument");
+  hipError_t e = hipMemcpy(C->d_doping, doping_qp, C->ncells * C->qc * sizeof(double),
+                           hipMemcpyHostToDevice);
+  return e == hipSuccess ? 0 : hip_fail(e, "hdg_set_doping");
+}
+
+int hdg_set_bc_values(hdg_ctx *C, const double *dir, const double *neu)
+{
+  if (!C)
+    return fail("hdg_set_bc_values: null context");
+  const size_t n = C->nfaces * 3 * C->qf * sizeof(double);
+  if (dir)
+    {
+      hipError_t e = hipMemcpy(C->d_dirv, dir, n, hipMemcpyHostToDevice);
+      if (e != hipSuccess)
+        return hip_fail(e, "hdg_set_bc_values");
+    }
+  if (neu)
+    {
+      hipError_t e = hipMemcpy(C->d_neuv, neu, n, hipMemcpyHostToDevice);
+      if (e != hipSuccess)
+        return hip_fail(e, "hdg_set_bc_values");
+    }
+  return 0;
+}
+
+int hdg_condense(hdg_ctx *C, const double *x_cell, const double *x_trace, int thermodyn,
+                 double *csr_vals, double *rhs, double *dir_vals, void *stream)
+{
+  if (!C || !x_cell || !x_trace || !csr_vals || !rhs || !dir_vals)
+    return fail("hdg_condense: null argument");
+  if (thermodyn && (C->en[1] || C->en[2] || !C->en[0]))
+    return fail("hdg_condense: thermodynamic equilibrium runs with V only", -2);
+  hdg::KParams p = params(C, thermodyn);
+  p.xc     = x_cell;
+  p.xt     = x_trace;
+  p.vals   = csr_vals;
+  p.rhs    = rhs;
+  p.dirout = dir_vals;
+  hipStream_t s = (hipStream_t)stream;
+  for (int col = 0; col < 2; ++col)
+    {
+      hipError_t e = hdg::launch_condense(C->d, C->k, p, C->d_cells[col],
+                                          (int)C->color_cells[col].size(), col, 0, s);
+      if (e != hipSuccess)
+        return hip_fail(e, "hdg_condense launch");
+    }
+  return 0;
+}
+
+int hdg_reconstruct(hdg_ctx *C, const double *x_cell, const double *x_trace,
+                    const double *dx_trace, int thermodyn, double *dx_cell, void *stream)
+{
+  if (!C || !x_cell || !x_trace || !dx_trace || !dx_cell)
+    return fail("hdg_reconstruct: null argument");
+  if (thermodyn && (C->en[1] || C->en[2] || !C->en[0]))
+    return fail("hdg_reconstruct: thermodynamic equilibrium runs with V only", -2);
+  hdg::KParams p = params(C, thermodyn);
+  p.xc  = x_cell;
+  p.xt  = x_trace;
+  p.dxt = dx_trace;
+  p.dxc = dx_cell;
+  hipError_t e = hdg::launch_reconstruct(C->d, C->k, p, (int)C->ncells, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "hdg_reconstruct launch");
+}
+
+int hdg_cell_systems(hdg_ctx *C, const double *x_cell, const double *x_trace, int thermodyn,
+                     const int *cells, int ncell, double *S, double *g, void *stream)
+{
+  if (!C || !x_cell || !x_trace || !cells || !S || !g)
+    return fail("hdg_cell_systems: null argument");
+  if (thermodyn && (C->en[1] || C->en[2] || !C->en[0]))
+    return fail("hdg_cell_systems: thermodynamic equilibrium runs with V only", -2);
+  hdg::KParams p = params(C, thermodyn);
+  p.xc   = x_cell;
+  p.xt   = x_trace;
+  p.Sout = S;
+  p.gout = g;
+  hipError_t e = hdg::launch_condense(C->d, C->k, p, cells, ncell, 0, 1, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "hdg_cell_systems launch");
+}
+
+} // extern "C"

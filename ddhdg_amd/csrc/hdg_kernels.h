@@ -1,0 +1,53 @@
+// hdg_kernels.h -- kernel parameter block shared by the host layer and the
+// gfx950 kernels of the per-cell HDG path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hdg
+{
+  struct DevTables
+  {
+    const double *JxWc, *phic, *gradc, *phif, *JxWf, *xi, *Bz, *Bt;
+    const double *LDc, *LDf, *LCc, *LCf, *PhiMinvC, *PhiMinvF;
+    const double *Minv, *MinvD, *MinvC, *Mfinv;
+  };
+
+  struct KParams
+  {
+    // mesh
+    int       n[3];
+    long long face_off[4];
+    long long face_dims[3][3];
+    // enabled components
+    int en[3];
+    int ne;
+    int encomp[3];
+    int thermo;
+    // physics (dimensionless)
+    double eps, mun, mup, UT, tau[3];
+    int    rec;
+    double rec0, rec1, rec2, n_rf, rec_rf;
+    double kTq, ec_r, ev_r, nc, nv;
+    int    bc[6][3];
+    DevTables tb;
+    // per-mesh data
+    const double *doping;   // ncells*QC
+    const double *dirv;     // nfaces*3*QF
+    const double *neuv;     // nfaces*3*QF
+    // scatter map
+    const long long     *blkptr;   // nfaces
+    const int           *nbk;      // nfaces
+    const unsigned char *cslot;    // ncells*NFC*NFC
+    // state
+    const double *xc, *xt, *dxt;
+    // outputs
+    double *vals, *rhs, *dirout, *dxc, *Sout, *gout;
+  };
+
+  // launchers (hdg_kernels.hip)
+  hipError_t launch_condense(int dim, int deg, const KParams &p, const int *cells,
+                             int ncell, int color, int mode, hipStream_t s);
+  hipError_t launch_reconstruct(int dim, int deg, const KParams &p, int ncells,
+                                hipStream_t s);
+  bool supported(int dim, int deg);
+} // namespace hdg

@@ -1,0 +1,806 @@
+// hdg_kernels.hip -- gfx950 kernels of the per-cell HDG path (generic form).
+//
+// One workgroup (256 threads = 4 wave64) per cell.  The cell's reduced
+// bordered system
+//        [ A'  CT' | c' ]      rows: scalar test functions (ne*nb)
+//        [ TC' TT' | t' ]            trace test functions (2d*ne*nf)
+// is built in LDS after the flux unknowns have been eliminated with the
+// precomputed lifting operators (hdg_tables.h), then ne*nb steps of
+// Gaussian elimination with row-scaled partial pivoting reduce the trace
+// rows to [ S | g ] = [TT - TC A^-1 CT | t - TC A^-1 c], the same blocks the
+// reference builds with gauss_jordan + mmult
+// (include/assembling_methods.template.h:2654-2668).
+//
+// The reference's terms map onto three per-quadrature-point forms applied
+// to a column's "bundle" (its scalar values u, lifted fluxes W and trace
+// values lambda at the quadrature points):
+//   z rows, cell qp:  phi_i * alpha_c + grad phi_i . beta_c
+//        add_cell_products_to_cc_matrix/rhs (template.h:269-760)
+//   z rows, face qp:  phi_i * F_c      add_border_products_to_cc_matrix/rhs,
+//        assemble_ct_matrix, add_ct_matrix_terms_to_cc_rhs (:885-1141,
+//        :1737-2118)
+//   trace rows:       xi_i * F_c       assemble_tc_matrix, assemble_tt_matrix,
+//        add_tc/tt_matrix_terms_to_tt_rhs (:1145-1588), apply_nbc_on_face
+// where F_c is the HDG numerical flux of component c (use_cell).
+#include "hdg_kernels.h"
+
+namespace hdg
+{
+  namespace
+  {
+    constexpr int NT = 256;
+
+    template <int DIM, int DEG>
+    struct Sh
+    {
+      static constexpr int K1    = DEG + 1;
+      static constexpr int NB    = DIM == 2 ? K1 * K1 : K1 * K1 * K1;
+      static constexpr int NF    = DIM == 2 ? K1 : K1 * K1;
+      static constexpr int QC    = NB;
+      static constexpr int QF    = NF;
+      static constexpr int NFC   = 2 * DIM;
+      static constexpr int NCD   = 3 * (DIM + 1) * NB;
+      static constexpr int MAXNS = 3 * NB;
+      static constexpr int MAXT  = NFC * 3 * NF;
+      static constexpr int MAXR  = MAXNS + MAXT;
+      static constexpr int LD    = MAXR + 2;
+      static constexpr int SZ    = QC * (1 + DIM) + NFC * QF;
+      // LDS layout (doubles)
+      static constexpr int oR     = 0;
+      static constexpr int oG     = oR + MAXR * LD;
+      static constexpr int oXS    = oG + SZ * LD;
+      static constexpr int oXT    = oXS + NCD;
+      static constexpr int oU0    = oXT + NFC * 3 * NF;
+      static constexpr int oW0    = oU0 + 3 * QC;
+      static constexpr int oU0F   = oW0 + 3 * DIM * QC;
+      static constexpr int oW0F   = oU0F + 3 * NFC * QF;
+      static constexpr int oTR0   = oW0F + 3 * NFC * QF;
+      static constexpr int oQD    = oTR0 + 3 * NFC * QF; // 6*QC point data
+      static constexpr int oRHSQ  = oQD + 6 * QC;
+      static constexpr int oWRC   = oRHSQ + 3 * DIM * NB;
+      static constexpr int oWRF   = oWRC + 3 * DIM * QC;
+      static constexpr int oLAMF  = oWRF + 3 * NFC * QF;
+      static constexpr int oDEL   = oLAMF + 3 * NFC * QF;
+      static constexpr int oLUPD  = oDEL + NFC * 3 * NF;
+      static constexpr int oRS    = oLUPD + NFC * 3 * NF;
+      static constexpr int oMUL   = oRS + MAXR;
+      static constexpr int oSOL   = oMUL + MAXR;
+      static constexpr int NDBL   = oSOL + MAXNS;
+      // ints after the doubles
+      static constexpr int iUSED  = 0;
+      static constexpr int iPIV   = iUSED + MAXR;
+      static constexpr int iDIR   = iPIV + MAXNS;
+      static constexpr int iNEU   = iDIR + NFC * 3;
+      static constexpr int iFACE  = iNEU + NFC * 3;
+      static constexpr int iBID   = iFACE + 2 * NFC; // face index as 2 ints
+      static constexpr int NINT   = iBID + NFC;
+      static constexpr size_t bytes = NDBL * sizeof(double) + NINT * sizeof(int);
+    };
+
+    __device__ inline long long face_of(const KParams &p, int ix, int iy, int iz,
+                                        int lf)
+    {
+      int       pos[3] = {ix, iy, iz};
+      const int a      = lf >> 1;
+      pos[a] += lf & 1;
+      const long long *m = p.face_dims[a];
+      return p.face_off[a] + pos[0] + m[0] * (pos[1] + m[1] * (long long)pos[2]);
+    }
+
+    // SRH / linear recombination with the reference's scaling
+    // (src/recombination_term.cpp:433-480, src/adimensionalizer.cpp:476-497)
+    __device__ inline void recomb(const KParams &p, double n, double pp, double &r,
+                                  double &drn, double &drp)
+    {
+      if (p.rec == 1)
+        {
+          const double ni  = p.rec0 / p.n_rf;
+          const double num = n * pp - ni * ni;
+          const double den = p.rec2 * (n + ni) + p.rec1 * (pp + ni);
+          r                = num * p.n_rf / den;
+          drn              = pp / den - num * p.rec2 / (den * den);
+          drp              = n / den - num * p.rec1 / (den * den);
+        }
+      else if (p.rec == 2)
+        {
+          r   = p.rec0 + (p.rec1 * n + p.rec2 * pp) * p.n_rf;
+          drn = p.rec1;
+          drp = p.rec2;
+        }
+      else
+        {
+          r = drn = drp = 0;
+        }
+      r /= p.rec_rf;
+      drn /= p.rec_rf;
+      drp /= p.rec_rf;
+    }
+
+    // ------------------------------------------------------------------
+    // Shared first half of both passes: load, state at quadrature points,
+    // flux-row residual and its lifting, Dirichlet projection (pass 1) or
+    // lifted trace update (pass 2), and the reduced system in LDS.
+    // pass2: rows = z rows only, columns = scalar dofs + one rhs column.
+    template <int DIM, int DEG>
+    __device__ void build_reduced(const KParams &p, long long cell, bool pass2,
+                                  double *sm, int *si, int &ns_out, int &T_out)
+    {
+      using S           = Sh<DIM, DEG>;
+      constexpr int NB  = S::NB, NF = S::NF, QC = S::QC, QF = S::QF;
+      constexpr int NFC = S::NFC, LD = S::LD, SZ = S::SZ;
+      const int     tid = threadIdx.x;
+      double *R = sm + S::oR, *G = sm + S::oG, *xs = sm + S::oXS, *xt = sm + S::oXT;
+      double *u0 = sm + S::oU0, *W0 = sm + S::oW0, *u0f = sm + S::oU0F;
+      double *W0f = sm + S::oW0F, *tr0 = sm + S::oTR0, *qd = sm + S::oQD;
+      double *rhsq = sm + S::oRHSQ, *Wrc = sm + S::oWRC, *Wrf = sm + S::oWRF;
+      double *lamf = sm + S::oLAMF, *del = sm + S::oDEL, *lupd = sm + S::oLUPD;
+      int    *isdir = si + S::iDIR, *isneu = si + S::iNEU, *bidv = si + S::iBID;
+      long long *facev = reinterpret_cast<long long *>(si + S::iFACE);
+      // point data: r, drn, drp, phi'_V, phi_rhs, doping
+      double *qr = qd, *qdrn = qd + QC, *qdrp = qd + 2 * QC, *qphd = qd + 3 * QC;
+      double *qphr = qd + 4 * QC, *qdop = qd + 5 * QC;
+      const DevTables &tb = p.tb;
+
+      const int ne   = p.ne;
+      const int ns   = ne * NB;
+      const int tdf  = ne * NF;
+      const int T    = NFC * tdf;
+      const int nrow = pass2 ? ns : ns + T;
+      const int ncol = pass2 ? ns + 1 : ns + T + 1;
+      ns_out         = ns;
+      T_out          = T;
+
+      int ic[3];
+      ic[0] = (int)(cell % p.n[0]);
+      ic[1] = (int)((cell / p.n[0]) % p.n[1]);
+      ic[2] = (int)(cell / ((long long)p.n[0] * p.n[1]));
+
+      // ---- A: load state and boundary flags
+      if (tid < NFC)
+        {
+          const int lf = tid, a = lf >> 1, sd = lf & 1;
+          facev[lf]    = face_of(p, ic[0], ic[1], ic[2], lf);
+          const bool bnd = sd ? (ic[a] == p.n[a] - 1) : (ic[a] == 0);
+          const int  bid = bnd ? lf : -1;
+          bidv[lf]       = bid;
+          for (int c = 0; c < 3; ++c)
+            {
+              const int bt         = (bid >= 0 && p.en[c]) ? p.bc[bid][c] : 0;
+              isdir[lf * 3 + c]    = bt == 1;
+              isneu[lf * 3 + c]    = bt == 2;
+            }
+        }
+      for (int i = tid; i < S::NCD; i += NT)
+        xs[i] = p.xc[cell * S::NCD + i];
+      for (int i = tid; i < QC; i += NT)
+        qdop[i] = p.doping[cell * QC + i];
+      __syncthreads();
+      for (int i = tid; i < NFC * 3 * NF; i += NT)
+        {
+          const int lf = i / (3 * NF), r = i % (3 * NF);
+          xt[i]        = p.xt[facev[lf] * 3 * NF + r];
+          if (pass2)
+            lupd[i] = p.dxt[facev[lf] * 3 * NF + r];
+        }
+      __syncthreads();
+
+      // ---- B: state at quadrature points
+      // cell: u0[c][q], W0[c][a][q]
+      for (int e = tid; e < 3 * (DIM + 1) * QC; e += NT)
+        {
+          const int c = e / ((DIM + 1) * QC), r = e % ((DIM + 1) * QC);
+          const int blk = r / QC, q = r % QC; // blk < DIM: flux axis, == DIM scalar
+          const double *x = xs + c * (DIM + 1) * NB + blk * NB;
+          double        v = 0;
+          for (int j = 0; j < NB; ++j)
+            v += x[j] * tb.phic[q * NB + j];
+          if (blk == DIM)
+            u0[c * QC + q] = v;
+          else
+            W0[(c * DIM + blk) * QC + q] = v;
+        }
+      // faces: u0f, W0f (normal comp), tr0; [c][g][s]
+      for (int e = tid; e < 3 * NFC * QF; e += NT)
+        {
+          const int c = e / (NFC * QF), g = (e / QF) % NFC, s = e % QF;
+          const int a = g >> 1;
+          const double nu = (g & 1) ? 1.0 : -1.0;
+          const double *xsc = xs + c * (DIM + 1) * NB;
+          const double *ph  = tb.phif + (g * QF + s) * NB;
+          double        u = 0, w = 0, t = 0;
+          for (int j = 0; j < NB; ++j)
+            {
+              u += xsc[DIM * NB + j] * ph[j];
+              w += xsc[a * NB + j] * ph[j];
+            }
+          for (int m = 0; m < NF; ++m)
+            t += xt[(g * 3 + c) * NF + m] * tb.xi[s * NF + m];
+          u0f[e] = u;
+          W0f[e] = nu * w;
+          tr0[e] = t;
+          if (pass2)
+            {
+              double l = 0;
+              for (int m = 0; m < NF; ++m)
+                l += lupd[(g * 3 + c) * NF + m] * tb.xi[s * NF + m];
+              lamf[e] = l;
+            }
+          else
+            lamf[e] = 0;
+        }
+      __syncthreads();
+
+      // ---- C: point data and flux-row residual
+      for (int q = tid; q < QC; q += NT)
+        {
+          double r = 0, a = 0, b = 0;
+          if (p.en[1] || p.en[2])
+            recomb(p, u0[QC + q], u0[2 * QC + q], r, a, b);
+          qr[q]   = r;
+          qdrn[q] = a;
+          qdrp[q] = b;
+          if (p.thermo)
+            {
+              // template.h:442-459 and 677-693 with phi_n = phi_p = 0
+              const double V0 = u0[q], kT = p.kTq;
+              const double e1 = exp((p.ev_r - V0) / kT);
+              const double e2 = exp((V0 - p.ec_r) / kT);
+              qphd[q] = -1 / p.n_rf * (p.nv / kT * e1 + p.nc / kT * e2);
+              qphr[q] = 1 / p.n_rf * (p.nv * e1 - p.nc * e2);
+            }
+          else
+            qphd[q] = qphr[q] = 0;
+        }
+      // rhs_q[c][a][i] = sum_q (u0 d_a phi_i - W0_a phi_i) JxW
+      //                 - sum_{faces of axis a} tr0 phi_i nu JxW
+      for (int e = tid; e < 3 * DIM * NB; e += NT)
+        {
+          const int c = e / (DIM * NB), a = (e / NB) % DIM, i = e % NB;
+          double    v = 0;
+          if (p.en[c])
+            {
+              for (int q = 0; q < QC; ++q)
+                v += (u0[c * QC + q] * tb.gradc[(q * NB + i) * DIM + a] -
+                      W0[(c * DIM + a) * QC + q] * tb.phic[q * NB + i]) *
+                     tb.JxWc[q];
+              for (int sd = 0; sd < 2; ++sd)
+                {
+                  const int    g  = 2 * a + sd;
+                  const double nu = sd ? 1.0 : -1.0;
+                  for (int s = 0; s < QF; ++s)
+                    v -= tr0[(c * NFC + g) * QF + s] * tb.phif[(g * QF + s) * NB + i] *
+                         nu * tb.JxWf[g * QF + s];
+                }
+            }
+          rhsq[e] = v;
+        }
+      // Dirichlet update: L2 projection of g/rf - lambda0 on the face
+      // (apply_dbc_on_face, template.h:1592-1691); zero in thermodynamic
+      // equilibrium (template.h:1662-1665)
+      if (!pass2)
+        for (int e = tid; e < NFC * 3 * NF; e += NT)
+          {
+            const int g = e / (3 * NF), c = (e / NF) % 3, m = e % NF;
+            double    v = 0;
+            if (isdir[g * 3 + c] && !p.thermo)
+              {
+                const int a = g >> 1;
+                for (int n2 = 0; n2 < NF; ++n2)
+                  {
+                    double b = 0;
+                    for (int s = 0; s < QF; ++s)
+                      b += tb.xi[s * NF + n2] * tb.JxWf[g * QF + s] *
+                           (p.dirv[(facev[g] * 3 + c) * QF + s] -
+                            tr0[(c * NFC + g) * QF + s]);
+                    v += tb.Mfinv[(a * NF + m) * NF + n2] * b;
+                  }
+              }
+            del[e] = v;
+          }
+      __syncthreads();
+
+      // ---- D: lifted residual flux W^r (plus lifted trace update, pass 2)
+      for (int e = tid; e < 3 * DIM * QC; e += NT)
+        {
+          const int c = e / (DIM * QC), a = (e / QC) % DIM, q = e % QC;
+          double    v = 0;
+          if (p.en[c])
+            {
+              for (int i = 0; i < NB; ++i)
+                v += tb.PhiMinvC[q * NB + i] * rhsq[(c * DIM + a) * NB + i];
+              if (pass2)
+                for (int sd = 0; sd < 2; ++sd)
+                  {
+                    const int f = 2 * a + sd;
+                    for (int m = 0; m < NF; ++m)
+                      v += tb.LCc[(f * QC + q) * NF + m] * lupd[(f * 3 + c) * NF + m];
+                  }
+            }
+          Wrc[e] = v;
+        }
+      for (int e = tid; e < 3 * NFC * QF; e += NT)
+        {
+          const int c = e / (NFC * QF), g = (e / QF) % NFC, s = e % QF;
+          const int a = g >> 1;
+          double    v = 0;
+          if (p.en[c])
+            {
+              for (int i = 0; i < NB; ++i)
+                v += tb.PhiMinvF[(g * QF + s) * NB + i] * rhsq[(c * DIM + a) * NB + i];
+              if (pass2)
+                for (int sd = 0; sd < 2; ++sd)
+                  {
+                    const int f = 2 * a + sd;
+                    for (int m = 0; m < NF; ++m)
+                      v += tb.LCf[((f * NFC + g) * QF + s) * NF + m] *
+                           lupd[(f * 3 + c) * NF + m];
+                  }
+            }
+          Wrf[e] = v;
+        }
+      __syncthreads();
+
+      // ---- E: reduced system, one row block (component) at a time
+      const double eps = p.eps, mun = p.mun, mup = p.mup, UT = p.UT;
+      const double tV = p.tau[0], tN = p.tau[1], tP = p.tau[2];
+      for (int ci = 0; ci < ne; ++ci)
+        {
+          const int c = p.encomp[ci];
+          for (int e = tid; e < SZ * ncol; e += NT)
+            {
+              const int slot = e / ncol, m = e % ncol;
+              // decode column m
+              int kind, cc = 0, j = 0, f = 0;
+              if (m < ns)
+                {
+                  kind = 0;
+                  cc   = p.encomp[m / NB];
+                  j    = m % NB;
+                }
+              else if (!pass2 && m < ns + T)
+                {
+                  kind         = 1;
+                  const int lt = m - ns;
+                  f            = lt / tdf;
+                  cc           = p.encomp[(lt % tdf) / NF];
+                  j            = lt % NF;
+                }
+              else
+                kind = 2;
+              double val = 0;
+              if (slot < QC * (1 + DIM))
+                {
+                  const int q = slot / (1 + DIM), cs = slot % (1 + DIM);
+                  double    uu[3] = {0, 0, 0};
+                  if (kind == 0)
+                    uu[cc] = tb.phic[q * NB + j];
+                  if (cs == 0)
+                    {
+                      if (c == 0)
+                        val = p.thermo ? -qphd[q] * uu[0] : uu[1] - uu[2];
+                      else
+                        val = -(qdrn[q] * uu[1] + qdrp[q] * uu[2]);
+                      if (kind == 2)
+                        {
+                          if (c == 0)
+                            val += qdop[q] +
+                                   (p.thermo ? qphr[q] : u0[2 * QC + q] - u0[QC + q]);
+                          else
+                            val += qr[q];
+                        }
+                    }
+                  else
+                    {
+                      const int a  = cs - 1;
+                      double    WW[3] = {0, 0, 0};
+                      if (kind == 0)
+                        WW[cc] = tb.LDc[(q * DIM + a) * NB + j];
+                      else if (kind == 1)
+                        WW[cc] = ((f >> 1) == a) ? tb.LCc[(f * QC + q) * NF + j] : 0.0;
+                      else
+                        for (int c2 = 0; c2 < 3; ++c2)
+                          WW[c2] = -Wrc[(c2 * DIM + a) * QC + q];
+                      const double E0a = W0[(0 * DIM + a) * QC + q];
+                      if (c == 0)
+                        {
+                          val = -eps * WW[0];
+                          if (kind == 2)
+                            val += eps * E0a;
+                        }
+                      else if (c == 1)
+                        {
+                          const double n0 = u0[QC + q];
+                          val = -mun * E0a * uu[1] + n0 * mun * WW[0] + UT * mun * WW[1];
+                          if (kind == 2)
+                            val += n0 * mun * E0a - UT * mun * W0[(1 * DIM + a) * QC + q];
+                        }
+                      else
+                        {
+                          const double p0 = u0[2 * QC + q];
+                          val = mup * E0a * uu[2] - p0 * mup * WW[0] + UT * mup * WW[2];
+                          if (kind == 2)
+                            val += -p0 * mup * E0a - UT * mup * W0[(2 * DIM + a) * QC + q];
+                        }
+                    }
+                }
+              else
+                {
+                  const int fs = slot - QC * (1 + DIM);
+                  const int g = fs / QF, s = fs % QF;
+                  double    uu[3] = {0, 0, 0}, WN[3] = {0, 0, 0}, LL[3] = {0, 0, 0};
+                  if (kind == 0)
+                    {
+                      uu[cc] = tb.phif[(g * QF + s) * NB + j];
+                      WN[cc] = tb.LDf[(g * QF + s) * NB + j];
+                    }
+                  else if (kind == 1)
+                    {
+                      WN[cc] = tb.LCf[((f * NFC + g) * QF + s) * NF + j];
+                      if (f == g)
+                        LL[cc] = tb.xi[s * NF + j];
+                    }
+                  else
+                    for (int c2 = 0; c2 < 3; ++c2)
+                      {
+                        WN[c2] = -Wrf[(c2 * NFC + g) * QF + s];
+                        LL[c2] = -lamf[(c2 * NFC + g) * QF + s];
+                      }
+                  const int    fi  = g * QF + s;
+                  const double E0n = W0f[(0 * NFC) * QF + fi];
+                  if (c == 0)
+                    {
+                      val = eps * WN[0] + tV * uu[0] - tV * LL[0];
+                      if (kind == 2)
+                        val += -eps * E0n - tV * u0f[fi] + tV * tr0[fi];
+                    }
+                  else if (c == 1)
+                    {
+                      const double n0 = u0f[NFC * QF + fi];
+                      val = uu[1] * mun * E0n + n0 * mun * WN[0] - UT * mun * WN[1] -
+                            tN * uu[1] + tN * LL[1];
+                      if (kind == 2)
+                        val += -(n0 * mun * E0n - UT * mun * W0f[NFC * QF + fi]) +
+                               tN * n0 - tN * tr0[NFC * QF + fi];
+                    }
+                  else
+                    {
+                      const double p0 = u0f[2 * NFC * QF + fi];
+                      val = -(uu[2] * mup * E0n + p0 * mup * WN[0]) - UT * mup * WN[2] -
+                            tP * uu[2] + tP * LL[2];
+                      if (kind == 2)
+                        val += -(-p0 * mup * E0n - UT * mup * W0f[2 * NFC * QF + fi]) +
+                               tP * p0 - tP * tr0[2 * NFC * QF + fi];
+                    }
+                }
+              G[slot * LD + m] = val;
+            }
+          __syncthreads();
+          // rows of this component: z rows (NB) then trace rows (NFC*NF)
+          const int nzr = NB + (pass2 ? 0 : NFC * NF);
+          for (int e = tid; e < nzr * ncol; e += NT)
+            {
+              const int rr = e / ncol, m = e % ncol;
+              double    v  = 0;
+              int       row;
+              if (rr < NB)
+                {
+                  row = ci * NB + rr;
+                  for (int sl = 0; sl < SZ; ++sl)
+                    v += tb.Bz[sl * NB + rr] * G[sl * LD + m];
+                }
+              else
+                {
+                  const int tr = rr - NB, g = tr / NF, i = tr % NF;
+                  row = ns + g * tdf + ci * NF + i;
+                  if (!isdir[g * 3 + c])
+                    {
+                      for (int s = 0; s < QF; ++s)
+                        v += tb.Bt[(g * QF + s) * NF + i] *
+                             G[(QC * (1 + DIM) + g * QF + s) * LD + m];
+                      if (m == ncol - 1 && isneu[g * 3 + c])
+                        for (int s = 0; s < QF; ++s)
+                          v += tb.Bt[(g * QF + s) * NF + i] *
+                               p.neuv[(facev[g] * 3 + c) * QF + s];
+                    }
+                  else
+                    v = 0;
+                }
+              R[row * LD + m] = v;
+            }
+          __syncthreads();
+        }
+      (void)nrow;
+    }
+
+    // Gaussian elimination of the first ns columns with row-scaled partial
+    // pivoting (rows stay in place; used[] marks pivot rows)
+    template <int DIM, int DEG>
+    __device__ void eliminate(double *sm, int *si, int ns, int nrow, int ncol)
+    {
+      using S          = Sh<DIM, DEG>;
+      constexpr int LD = S::LD;
+      const int     tid = threadIdx.x;
+      double *R = sm + S::oR, *rs = sm + S::oRS, *mul = sm + S::oMUL;
+      int    *used = si + S::iUSED, *piv = si + S::iPIV;
+      for (int i = tid; i < nrow; i += NT)
+        {
+          double mx = 0;
+          if (i < ns)
+            for (int j = 0; j < ns; ++j)
+              mx = fmax(mx, fabs(R[i * LD + j]));
+          rs[i]   = mx > 0 ? 1.0 / mx : 1.0;
+          used[i] = i < ns ? 0 : 1; // trace rows never pivot
+        }
+      __syncthreads();
+      for (int k = 0; k < ns; ++k)
+        {
+          if (tid < 64)
+            {
+              double best = -1;
+              int    bi   = -1;
+              for (int i = tid; i < ns; i += 64)
+                if (!used[i])
+                  {
+                    const double sc = fabs(R[i * LD + k]) * rs[i];
+                    if (sc > best)
+                      {
+                        best = sc;
+                        bi   = i;
+                      }
+                  }
+              for (int off = 32; off > 0; off >>= 1)
+                {
+                  const double ob = __shfl_xor(best, off);
+                  const int    oi = __shfl_xor(bi, off);
+                  if (ob > best || (ob == best && oi >= 0 && (bi < 0 || oi < bi)))
+                    {
+                      best = ob;
+                      bi   = oi;
+                    }
+                }
+              if (tid == 0)
+                {
+                  piv[k]   = bi;
+                  used[bi] = 2; // current pivot
+                }
+            }
+          __syncthreads();
+          const int    pr = piv[k];
+          const double pv = R[pr * LD + k];
+          for (int i = tid; i < nrow; i += NT)
+            mul[i] = (i < ns ? used[i] == 0 : true) && i != pr ? R[i * LD + k] / pv : 0.0;
+          __syncthreads();
+          const int w = ncol - k - 1;
+          for (int e = tid; e < nrow * w; e += NT)
+            {
+              const int i = e / w, j = k + 1 + e % w;
+              const double m = mul[i];
+              if (m != 0.0)
+                R[i * LD + j] -= m * R[pr * LD + j];
+            }
+          __syncthreads();
+          if (tid == 0)
+            used[pr] = 1;
+          __syncthreads();
+        }
+    }
+
+    template <int DIM, int DEG>
+    __global__ void __launch_bounds__(NT)
+      condense_kernel(KParams p, const int *cells, int ncell, int color, int mode)
+    {
+      using S = Sh<DIM, DEG>;
+      extern __shared__ double smem[];
+      double *sm = smem;
+      int    *si = reinterpret_cast<int *>(smem + S::NDBL);
+      const int b = blockIdx.x;
+      if (b >= ncell)
+        return;
+      const long long cell = cells[b];
+      int             ns, T;
+      build_reduced<DIM, DEG>(p, cell, false, sm, si, ns, T);
+      constexpr int NF = S::NF, NFC = S::NFC, LD = S::LD;
+      double       *R   = sm + S::oR;
+      double       *del = sm + S::oDEL;
+      int          *isdir = si + S::iDIR, *bidv = si + S::iBID;
+      long long    *facev = reinterpret_cast<long long *>(si + S::iFACE);
+      const int     tid   = threadIdx.x;
+      const int     tdf   = p.ne * NF;
+      const int     nrow = ns + T, ncol = ns + T + 1;
+      // LocalCondenser::condense_ct_matrix (src/local_condenser.cpp:141-160)
+      // on the reduced columns: rhs -= delta_j * col_j, col_j = 0
+      for (int r = tid; r < nrow; r += NT)
+        {
+          double acc = 0;
+          for (int lt = 0; lt < T; ++lt)
+            {
+              const int g = lt / tdf, c = p.encomp[(lt % tdf) / NF], m = lt % NF;
+              if (isdir[g * 3 + c])
+                {
+                  acc += del[(g * 3 + c) * NF + m] * R[r * LD + ns + lt];
+                  R[r * LD + ns + lt] = 0;
+                }
+            }
+          R[r * LD + ncol - 1] -= acc;
+        }
+      __syncthreads();
+      eliminate<DIM, DEG>(sm, si, ns, nrow, ncol);
+      // output
+      if (mode == 1)
+        {
+          for (int e = tid; e < T * T; e += NT)
+            {
+              const int i = e / T, j = e % T;
+              p.Sout[(long long)b * T * T + e] = R[(ns + i) * LD + ns + j];
+            }
+          for (int i = tid; i < T; i += NT)
+            p.gout[(long long)b * T + i] = R[(ns + i) * LD + ncol - 1];
+          return;
+        }
+      const unsigned char *slot = p.cslot + cell * NFC * NFC;
+      for (int e = tid; e < T * T; e += NT)
+        {
+          const int i = e / T, j = e % T;
+          const int gi = i / tdf, gj = j / tdf, ri = i % tdf, rj = j % tdf;
+          const int ci = p.encomp[ri / NF];
+          const long long F = facev[gi];
+          const bool dir = isdir[gi * 3 + ci];
+          const double v = dir ? (i == j ? 1.0 : 0.0) : R[(ns + i) * LD + ns + j];
+          const long long pos = p.blkptr[F] * tdf * tdf + (long long)ri * p.nbk[F] * tdf +
+                                (long long)slot[gi * NFC + gj] * tdf + rj;
+          const bool add = color == 1 && gi == gj && bidv[gi] < 0;
+          if (add)
+            p.vals[pos] += v;
+          else
+            p.vals[pos] = v;
+        }
+      for (int i = tid; i < T; i += NT)
+        {
+          const int gi = i / tdf, ri = i % tdf;
+          const int ci = p.encomp[ri / NF];
+          const long long F   = facev[gi];
+          const long long row = F * tdf + ri;
+          const bool dir = isdir[gi * 3 + ci];
+          const double v = dir ? 0.0 : R[(ns + i) * LD + ncol - 1];
+          if (color == 1 && bidv[gi] < 0)
+            p.rhs[row] += v;
+          else
+            p.rhs[row] = v;
+          if (dir)
+            p.dirout[row] = del[(gi * 3 + ci) * NF + (ri % NF)];
+        }
+    }
+
+    template <int DIM, int DEG>
+    __global__ void __launch_bounds__(NT) reconstruct_kernel(KParams p, int ncells)
+    {
+      using S = Sh<DIM, DEG>;
+      extern __shared__ double smem[];
+      double *sm = smem;
+      int    *si = reinterpret_cast<int *>(smem + S::NDBL);
+      const long long cell = blockIdx.x;
+      if (cell >= ncells)
+        return;
+      int ns, T;
+      build_reduced<DIM, DEG>(p, cell, true, sm, si, ns, T);
+      constexpr int NB = S::NB, NF = S::NF, LD = S::LD, DIMc = DIM;
+      double       *R = sm + S::oR, *sol = sm + S::oSOL, *rhsq = sm + S::oRHSQ;
+      double       *lupd = sm + S::oLUPD;
+      int          *piv  = si + S::iPIV;
+      const int     tid  = threadIdx.x;
+      eliminate<DIM, DEG>(sm, si, ns, ns, ns + 1);
+      if (tid == 0)
+        for (int k = ns - 1; k >= 0; --k)
+          {
+            const int pr = piv[k];
+            double    v  = R[pr * LD + ns];
+            for (int j = k + 1; j < ns; ++j)
+              v -= R[pr * LD + j] * sol[j];
+            sol[k] = v / R[pr * LD + k];
+          }
+      __syncthreads();
+      // W = M^-1 (rhs_q + D_a u - sum_f C_{a,f} lambda_f), written with u
+      double *out = p.dxc + cell * S::NCD;
+      for (int e = tid; e < S::NCD; e += NT)
+        {
+          const int c = e / ((DIMc + 1) * NB), r = e % ((DIMc + 1) * NB);
+          const int blk = r / NB, i = r % NB;
+          double    v   = 0;
+          if (p.en[c])
+            {
+              int ci = 0;
+              for (int c2 = 0; c2 < c; ++c2)
+                ci += p.en[c2];
+              const double *u = sol + ci * NB;
+              if (blk == DIMc)
+                v = u[i];
+              else
+                {
+                  const int a = blk;
+                  for (int l = 0; l < NB; ++l)
+                    v += p.tb.Minv[i * NB + l] * rhsq[(c * DIMc + a) * NB + l] +
+                         p.tb.MinvD[(a * NB + i) * NB + l] * u[l];
+                  for (int sd = 0; sd < 2; ++sd)
+                    {
+                      const int f = 2 * a + sd;
+                      for (int m = 0; m < NF; ++m)
+                        v -= p.tb.MinvC[(f * NB + i) * NF + m] * lupd[(f * 3 + c) * NF + m];
+                    }
+                }
+            }
+          out[e] = v;
+        }
+    }
+
+    template <int DIM, int DEG>
+    hipError_t launch_c(const KParams &p, const int *cells, int ncell, int color, int mode,
+                        hipStream_t s)
+    {
+      using S = Sh<DIM, DEG>;
+      static bool attr = false;
+      if (!attr)
+        {
+          hipError_t e = hipFuncSetAttribute((const void *)condense_kernel<DIM, DEG>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)S::bytes);
+          if (e != hipSuccess)
+            return e;
+          e = hipFuncSetAttribute((const void *)reconstruct_kernel<DIM, DEG>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)S::bytes);
+          if (e != hipSuccess)
+            return e;
+          attr = true;
+        }
+      if (ncell <= 0)
+        return hipSuccess;
+      hipLaunchKernelGGL((condense_kernel<DIM, DEG>), dim3(ncell), dim3(NT), S::bytes, s, p,
+                         cells, ncell, color, mode);
+      return hipGetLastError();
+    }
+
+    template <int DIM, int DEG>
+    hipError_t launch_r(const KParams &p, int ncells, hipStream_t s)
+    {
+      using S = Sh<DIM, DEG>;
+      hipError_t e = launch_c<DIM, DEG>(p, nullptr, 0, 0, 0, s); // sets attributes
+      if (e != hipSuccess)
+        return e;
+      hipLaunchKernelGGL((reconstruct_kernel<DIM, DEG>), dim3(ncells), dim3(NT), S::bytes, s,
+                         p, ncells);
+      return hipGetLastError();
+    }
+  } // namespace
+
+  bool supported(int dim, int deg)
+  {
+    return (dim == 2 && deg >= 1 && deg <= 3) || (dim == 3 && deg == 1);
+  }
+
+  hipError_t launch_condense(int dim, int deg, const KParams &p, const int *cells, int ncell,
+                             int color, int mode, hipStream_t s)
+  {
+    if (dim == 2 && deg == 1)
+      return launch_c<2, 1>(p, cells, ncell, color, mode, s);
+    if (dim == 2 && deg == 2)
+      return launch_c<2, 2>(p, cells, ncell, color, mode, s);
+    if (dim == 2 && deg == 3)
+      return launch_c<2, 3>(p, cells, ncell, color, mode, s);
+    if (dim == 3 && deg == 1)
+      return launch_c<3, 1>(p, cells, ncell, color, mode, s);
+    return hipErrorInvalidValue;
+  }
+
+  hipError_t launch_reconstruct(int dim, int deg, const KParams &p, int ncells, hipStream_t s)
+  {
+    if (dim == 2 && deg == 1)
+      return launch_r<2, 1>(p, ncells, s);
+    if (dim == 2 && deg == 2)
+      return launch_r<2, 2>(p, ncells, s);
+    if (dim == 2 && deg == 3)
+      return launch_r<2, 3>(p, ncells, s);
+    if (dim == 3 && deg == 1)
+      return launch_r<3, 1>(p, ncells, s);
+    return hipErrorInvalidValue;
+  }
+} // namespace hdg

@@ -1,0 +1,183 @@
+"""ctypes binding of ``libddhdg_hip.so`` (C ABI in include/ddhdg_hip.h).
+
+The native library is required: importing the context raises if it is not
+built, and every call that returns a negative status raises ``HdgError`` with
+the library's message.  Device buffers are torch tensors on the current HIP
+device; their raw pointers cross the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libddhdg_hip.so")
+_LIB = None
+
+(HDG_NCELLS, HDG_NFACES, HDG_NC, HDG_QC, HDG_QF, HDG_NB, HDG_NF, HDG_NE, HDG_NS, HDG_T,
+ HDG_NTRACE, HDG_NNZ, HDG_NSIZES) = range(13)
+
+# every symbol include/ddhdg_hip.h declares
+EXPORTS = [
+    "hdg_last_error", "hdg_ctx_create", "hdg_ctx_destroy", "hdg_sizes", "hdg_set_enabled",
+    "hdg_csr_pattern", "hdg_dirichlet_dofs", "hdg_cell_qpoints", "hdg_face_qpoints",
+    "hdg_set_doping", "hdg_set_bc_values", "hdg_condense", "hdg_reconstruct",
+    "hdg_cell_systems",
+]
+
+
+class HdgError(RuntimeError):
+    pass
+
+
+class MeshDesc(ct.Structure):
+    _fields_ = [("dim", ct.c_int), ("degree", ct.c_int), ("n_cells", ct.c_int * 3),
+                ("lower", ct.c_double * 3), ("upper", ct.c_double * 3)]
+
+
+class Phys(ct.Structure):
+    _fields_ = [("eps", ct.c_double), ("mu_n", ct.c_double), ("mu_p", ct.c_double),
+                ("U_T", ct.c_double), ("tau", ct.c_double * 3), ("recombination", ct.c_int),
+                ("rec", ct.c_double * 3), ("n_rescale", ct.c_double),
+                ("rec_rescale", ct.c_double), ("kT_over_q", ct.c_double),
+                ("ec_r", ct.c_double), ("ev_r", ct.c_double), ("nc", ct.c_double),
+                ("nv", ct.c_double), ("flux_type", ct.c_int),
+                ("bc_type", (ct.c_int * 3) * 6)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise HdgError(f"{LIB_PATH} is not built (python -m ddhdg_amd.build)")
+        L = ct.CDLL(LIB_PATH)
+        P = ct.POINTER
+        i, v, dp, lp, ip = ct.c_int, ct.c_void_p, P(ct.c_double), P(ct.c_longlong), P(ct.c_int)
+        sigs = {
+            "hdg_last_error": (ct.c_char_p, []),
+            "hdg_ctx_create": (i, [P(MeshDesc), P(Phys), P(v)]),
+            "hdg_ctx_destroy": (None, [v]),
+            "hdg_sizes": (i, [v, lp]),
+            "hdg_set_enabled": (i, [v, i, i, i]),
+            "hdg_csr_pattern": (i, [v, lp, ip]),
+            "hdg_dirichlet_dofs": (i, [v, lp]),
+            "hdg_cell_qpoints": (i, [v, dp]),
+            "hdg_face_qpoints": (i, [v, dp]),
+            "hdg_set_doping": (i, [v, dp]),
+            "hdg_set_bc_values": (i, [v, dp, dp]),
+            "hdg_condense": (i, [v, v, v, i, v, v, v, v]),
+            "hdg_reconstruct": (i, [v, v, v, v, i, v, v]),
+            "hdg_cell_systems": (i, [v, v, v, i, v, i, v, v, v]),
+        }
+        for name, (res, args) in sigs.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc < 0:
+        raise HdgError(lib().hdg_last_error().decode())
+    return rc
+
+
+def _dp(a):
+    return a.ctypes.data_as(ct.POINTER(ct.c_double))
+
+
+def _ptr(t):
+    return ct.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class HdgContext:
+    """One mesh on the GPU: owns the device tables of the C ABI context."""
+
+    def __init__(self, mesh: MeshDesc, phys: Phys):
+        self.L = lib()
+        self.ctx = ct.c_void_p()
+        _check(self.L.hdg_ctx_create(ct.byref(mesh), ct.byref(phys), ct.byref(self.ctx)))
+        self.mesh, self.phys = mesh, phys
+        self.dim = mesh.dim
+        self._refresh()
+
+    def close(self):
+        if self.ctx:
+            self.L.hdg_ctx_destroy(self.ctx)
+            self.ctx = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _refresh(self):
+        s = np.zeros(HDG_NSIZES, dtype=np.int64)
+        _check(self.L.hdg_sizes(self.ctx, s.ctypes.data_as(ct.POINTER(ct.c_longlong))))
+        self.sizes = s
+        (self.ncells, self.nfaces, self.NC, self.Qc, self.Qf, self.nb, self.nf, self.ne,
+         self.Ns, self.T, self.ntrace, self.nnz) = (int(x) for x in s)
+
+    def set_enabled(self, V, n, p):
+        _check(self.L.hdg_set_enabled(self.ctx, int(V), int(n), int(p)))
+        self._refresh()
+
+    def csr_pattern(self):
+        rp = np.zeros(self.ntrace + 1, dtype=np.int64)
+        ci = np.zeros(self.nnz, dtype=np.int32)
+        _check(self.L.hdg_csr_pattern(self.ctx, rp.ctypes.data_as(ct.POINTER(ct.c_longlong)),
+                                      ci.ctypes.data_as(ct.POINTER(ct.c_int))))
+        return rp, ci
+
+    def dirichlet_dofs(self):
+        n = _check(self.L.hdg_dirichlet_dofs(self.ctx, None))
+        idx = np.zeros(max(n, 1), dtype=np.int64)
+        _check(self.L.hdg_dirichlet_dofs(self.ctx, idx.ctypes.data_as(ct.POINTER(ct.c_longlong))))
+        return idx[:n]
+
+    def cell_qpoints(self):
+        x = np.zeros((self.ncells, self.Qc, self.dim))
+        _check(self.L.hdg_cell_qpoints(self.ctx, _dp(x)))
+        return x
+
+    def face_qpoints(self):
+        x = np.zeros((self.nfaces, self.Qf, self.dim))
+        _check(self.L.hdg_face_qpoints(self.ctx, _dp(x)))
+        return x
+
+    def set_doping(self, rescaled_doping_qp):
+        a = np.ascontiguousarray(rescaled_doping_qp, dtype=np.float64)
+        assert a.size == self.ncells * self.Qc
+        _check(self.L.hdg_set_doping(self.ctx, _dp(a)))
+
+    def set_bc_values(self, dirichlet=None, neumann=None):
+        n = self.nfaces * 3 * self.Qf
+        d = None if dirichlet is None else np.ascontiguousarray(dirichlet, dtype=np.float64)
+        m = None if neumann is None else np.ascontiguousarray(neumann, dtype=np.float64)
+        for a in (d, m):
+            assert a is None or a.size == n
+        _check(self.L.hdg_set_bc_values(self.ctx, None if d is None else _dp(d),
+                                        None if m is None else _dp(m)))
+
+    # -- device-pointer calls (torch tensors on the current device)
+    def condense(self, x_cell, x_trace, thermodyn, vals, rhs, dir_vals):
+        _check(self.L.hdg_condense(self.ctx, _ptr(x_cell), _ptr(x_trace), int(thermodyn),
+                                   _ptr(vals), _ptr(rhs), _ptr(dir_vals), _stream()))
+
+    def reconstruct(self, x_cell, x_trace, dx_trace, thermodyn, dx_cell):
+        _check(self.L.hdg_reconstruct(self.ctx, _ptr(x_cell), _ptr(x_trace), _ptr(dx_trace),
+                                      int(thermodyn), _ptr(dx_cell), _stream()))
+
+    def cell_systems(self, x_cell, x_trace, thermodyn, cells, S, g):
+        _check(self.L.hdg_cell_systems(self.ctx, _ptr(x_cell), _ptr(x_trace), int(thermodyn),
+                                       _ptr(cells), int(cells.numel()), _ptr(S), _ptr(g),
+                                       _stream()))

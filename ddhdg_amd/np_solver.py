@@ -1,0 +1,328 @@
+"""NPSolver: the Newton driver around the gfx950 hot path.
+
+Mirrors the reference ``NPSolver<dim, Problem>`` host API
+(include/np_solver.h:44-914): ``run``, ``compute_thermodynamic_equilibrium``,
+``set_current_solution``, ``project_component``, ``set_enabled_components``,
+``replace_boundary_condition``, ``assemble_system``.
+
+Per Newton step (``private_run``, src/np_solver.cpp:1452-1561):
+  pass 1   hdg_condense      GPU: assembly + condensation + scatter
+  solve    global trace system, host sparse direct LU (SciPy SuperLU, the
+           stand-in for the reference's SparseDirectUMFPACK; the global
+           solve is outside the per-cell hot path)
+  pass 2   hdg_reconstruct   GPU: cell update from the trace update
+  update   L-infinity norms and the alpha-damped update on the device
+The Newton state stays in device memory between steps and between
+continuation solves.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import fe
+from .hip_lib import HdgContext, MeshDesc, Phys
+from .problem import (DIRICHLET, EV, KB, NEUMANN, Q, Adimensionalizer,
+                      FixedTauNPSolverParameters, LinearRecombinationTerm,
+                      ShockleyReadHallFixedTemperature, compute_density, evaluate,
+                      local_charge_neutrality_potential, n, p, V)
+
+
+@dataclass
+class NonlinearIterationResults:
+    """include/nonlinear_iteration_results.h:5-14"""
+    converged: bool
+    iterations: int
+    last_update_norm: float
+
+
+def fold_physics(problem, parameters, adim) -> Phys:
+    """Dimensionless coefficients as the reference ScratchData computes them
+    (include/permittivity.h:67-83, include/mobility.h:66-82,
+    src/np_solver_parameters.cpp:13-28, include/adimensionalizer.h)."""
+    ph = Phys()
+    ph.eps = problem.permittivity.epsilon / adim.get_permittivity_rescaling_factor()
+    ph.mu_n = problem.n_mobility.mu / adim.get_mobility_rescaling_factor()
+    ph.mu_p = problem.p_mobility.mu / adim.get_mobility_rescaling_factor()
+    T = float(problem.temperature)
+    ph.U_T = (KB * T / Q) / adim.get_thermal_voltage_rescaling_factor()
+    taus = (parameters.V_tau, parameters.n_tau, parameters.p_tau)
+    for c in range(3):
+        ph.tau[c] = taus[c] / adim.get_tau_rescaling_factor(c)
+    r = problem.recombination_term
+    if isinstance(r, ShockleyReadHallFixedTemperature):
+        ph.recombination = 1
+        ph.rec[0] = r.intrinsic_carrier_concentration
+        ph.rec[1] = r.electron_life_time
+        ph.rec[2] = r.hole_life_time
+    elif isinstance(r, LinearRecombinationTerm):
+        ph.recombination = 2
+        ph.rec[0], ph.rec[1], ph.rec[2] = (r.constant_term, r.n_linear_coefficient,
+                                           r.p_linear_coefficient)
+    elif r is None:
+        ph.recombination = 0
+    else:
+        raise NotImplementedError(f"recombination model {type(r).__name__}")
+    ph.n_rescale = adim.get_component_rescaling_factor(n)
+    ph.rec_rescale = adim.get_recombination_rescaling_factor()
+    V_rf = adim.get_component_rescaling_factor(V)
+    ph.kT_over_q = KB * T / (Q * V_rf)
+    ph.ec_r = problem.conduction_band_edge_energy * EV / (Q * V_rf)
+    ph.ev_r = problem.valence_band_edge_energy * EV / (Q * V_rf)
+    ph.nc = problem.conduction_band_density
+    ph.nv = problem.valence_band_density
+    if parameters.dd_flux_type != "use_cell":
+        raise NotImplementedError("only the use_cell drift-diffusion flux is implemented")
+    if parameters.phi_linearize:
+        raise NotImplementedError("phi_linearize is not implemented")
+    ph.flux_type = 0
+    tab = problem.boundary_handler.bc_type_table()
+    for b in range(6):
+        for c in range(3):
+            ph.bc_type[b][c] = tab[b][c]
+    return ph
+
+
+class NPSolver:
+    def __init__(self, problem, parameters=None, adimensionalizer=None, device="cuda"):
+        import torch
+        self.torch = torch
+        self.problem = problem
+        self.parameters = parameters or FixedTauNPSolverParameters()
+        self.adim = adimensionalizer or Adimensionalizer()
+        degs = {self.parameters.V_degree, self.parameters.n_degree, self.parameters.p_degree}
+        if len(degs) != 1:
+            raise NotImplementedError("V, n and p must share one degree")
+        self.degree = degs.pop()
+        mesh = problem.mesh
+        self.mesh = mesh
+        self.ref = fe.RefCell(mesh, self.degree)
+        md = MeshDesc()
+        md.dim, md.degree = mesh.dim, self.degree
+        for a in range(3):
+            md.n_cells[a] = mesh.n[a]
+            md.lower[a] = mesh.lower[a]
+            md.upper[a] = mesh.upper[a]
+        self.ctx = HdgContext(md, fold_physics(problem, self.parameters, self.adim))
+        self.device = torch.device(device)
+        self.cell_qp = self.ctx.cell_qpoints()
+        self.face_qp = self.ctx.face_qpoints()
+        self.enabled = (True, True, True)
+        self._upload_problem_data()
+        self.x_cell = torch.zeros(self.ctx.ncells * self.ctx.NC, dtype=torch.float64,
+                                  device=self.device)
+        self.x_trace = torch.zeros(self.ctx.nfaces * 3 * self.ctx.nf, dtype=torch.float64,
+                                   device=self.device)
+        self._pattern_cache = {}
+
+    # ------------------------------------------------------------ set-up
+    def _upload_problem_data(self):
+        ctx = self.ctx
+        dop = evaluate(self.problem.doping, self.cell_qp) / self.adim.get_component_rescaling_factor(n)
+        ctx.set_doping(dop)
+        self._upload_bc()
+
+    def _upload_bc(self):
+        ctx, bh = self.ctx, self.problem.boundary_handler
+        dirv = np.zeros((ctx.nfaces, 3, ctx.Qf))
+        neuv = np.zeros((ctx.nfaces, 3, ctx.Qf))
+        mesh = self.mesh
+        for lf in range(2 * mesh.dim):
+            faces = mesh.cell_faces[mesh.cell_bid[:, lf] == lf, lf]
+            for c in range(3):
+                bc = bh.get(lf, c)
+                if bc is None:
+                    continue
+                kind, f = bc
+                vals = evaluate(f, self.face_qp[faces])
+                if kind == DIRICHLET:
+                    dirv[faces, c, :] = vals / self.adim.get_component_rescaling_factor(c)
+                else:
+                    neuv[faces, c, :] = vals / self.adim.get_neumann_boundary_condition_rescaling_factor(c)
+        ctx.set_bc_values(dirv.ravel(), neuv.ravel())
+
+    def replace_boundary_condition(self, boundary_id, bc_type, component, f):
+        """NPSolver::replace_boundary_condition (src/np_solver.cpp:2160-2193)."""
+        old = self.problem.boundary_handler.get(boundary_id, component)
+        if old is None or old[0] != bc_type:
+            raise NotImplementedError("only same-type replacement is supported")
+        self.problem.boundary_handler.replace_boundary_condition(boundary_id, bc_type,
+                                                                 component, f)
+        self._upload_bc()
+
+    def set_enabled_components(self, V_enabled, n_enabled, p_enabled):
+        self.enabled = (bool(V_enabled), bool(n_enabled), bool(p_enabled))
+        self.ctx.set_enabled(*self.enabled)
+
+    def is_enabled(self, c):
+        return self.enabled[c]
+
+    # ----------------------------------------------------------- state
+    @property
+    def NC(self):
+        return self.ctx.NC
+
+    def get_state(self):
+        return (self.x_cell.cpu().numpy().copy(), self.x_trace.cpu().numpy().copy())
+
+    def set_state(self, cell, trace):
+        t = self.torch
+        self.x_cell.copy_(t.as_tensor(np.asarray(cell, dtype=np.float64).ravel()))
+        self.x_trace.copy_(t.as_tensor(np.asarray(trace, dtype=np.float64).ravel()))
+
+    def _write_component(self, c, u, W, trace):
+        cell, tr = self.get_state()
+        d, nb, nf = self.mesh.dim, self.ref.nb, self.ref.nf
+        cell = cell.reshape(self.ctx.ncells, 3, d + 1, nb)
+        cell[:, c, d, :] = u
+        cell[:, c, :d, :] = W
+        tr = tr.reshape(self.ctx.nfaces, 3, nf)
+        tr[:, c, :] = trace
+        self.set_state(cell, tr)
+
+    def project_component(self, c, f):
+        """NPSolver::project_component for V, n or p (src/project_component.cpp:
+        478-574): cells by L2 projection, trace by l2_average."""
+        rf = self.adim.get_component_rescaling_factor(c)
+        fc = evaluate(f, self.cell_qp) / rf
+        ff = evaluate(f, fe.cell_face_points(self.mesh, self.face_qp)) / rf
+        u, W = fe.project_cells(self.mesh, self.ref, fc, ff)
+        tr = fe.trace_l2_average(self.mesh, self.ref, fe.face_values(self.mesh, self.ref, u))
+        self._write_component(c, u, W, tr)
+
+    set_component = project_component
+
+    def set_current_solution(self, V_function, n_function, p_function, use_projection=True):
+        for c, f in ((V, V_function), (n, n_function), (p, p_function)):
+            self.project_component(c, f)
+
+    def _V_at(self, u_V):
+        rf = self.adim.get_component_rescaling_factor(V)
+        return (np.einsum("qj,cj->cq", self.ref.phic, u_V) * rf,
+                fe.face_values(self.mesh, self.ref, u_V) * rf)
+
+    def project_densities_from_potential(self, phi_n=0.0, phi_p=0.0):
+        """project_component(phi_n / phi_p, f) (src/project_component.cpp:
+        282-452): n, p from the current V and given quasi-Fermi potentials."""
+        cell, _ = self.get_state()
+        d, nb = self.mesh.dim, self.ref.nb
+        u_V = cell.reshape(self.ctx.ncells, 3, d + 1, nb)[:, V, d, :]
+        Vc, Vf = self._V_at(u_V)
+        T = float(self.problem.temperature)
+        for c, phi in ((n, phi_n), (p, phi_p)):
+            rf = self.adim.get_component_rescaling_factor(c)
+            pc = evaluate(phi, self.cell_qp)
+            pf = evaluate(phi, fe.cell_face_points(self.mesh, self.face_qp))
+            fc = compute_density(self.problem, c, pc, Vc, T) / rf
+            ff = compute_density(self.problem, c, pf, Vf, T) / rf
+            u, W = fe.project_cells(self.mesh, self.ref, fc, ff)
+            tr = fe.trace_l2_average(self.mesh, self.ref, fe.face_values(self.mesh, self.ref, u))
+            self._write_component(c, u, W, tr)
+
+    def compute_local_charge_neutrality(self):
+        """src/thermodynamic_equilibrium.cpp:47-351, 351-359."""
+        T = float(self.problem.temperature)
+        rf = self.adim.get_component_rescaling_factor(V)
+
+        def phi(x):
+            return local_charge_neutrality_potential(
+                self.problem, evaluate(self.problem.doping, x), T) / rf
+
+        fc = phi(self.cell_qp)
+        ff = phi(fe.cell_face_points(self.mesh, self.face_qp))
+        u, W = fe.project_cells(self.mesh, self.ref, fc, ff)
+        tr = fe.trace_project(self.mesh, self.ref, phi(self.face_qp))
+        self._write_component(V, u, W, tr)
+
+    # ---------------------------------------------------------- Newton
+    def _pattern(self):
+        key = self.enabled
+        if key not in self._pattern_cache:
+            rp, ci = self.ctx.csr_pattern()
+            nf = self.ctx.nf
+            comps = [c for c in range(3) if self.enabled[c]]
+            ne = len(comps)
+            f = np.repeat(np.arange(self.ctx.nfaces), ne * nf)
+            cidx = np.tile(np.repeat(np.asarray(comps), nf), self.ctx.nfaces)
+            j = np.tile(np.arange(nf), self.ctx.nfaces * ne)
+            r2f = f * 3 * nf + cidx * nf + j
+            self._pattern_cache[key] = (rp, ci, self.ctx.dirichlet_dofs(),
+                                        self.torch.as_tensor(r2f, device=self.device))
+        return self._pattern_cache[key]
+
+    def assemble_system(self, thermodyn=False):
+        """pass 1 only; returns the condensed trace matrix (SciPy CSR), the
+        right-hand side (= minus the residual) and the Dirichlet values, as
+        pyddhdg's assemble_system/get_residual/get_jacobian expose them."""
+        import scipy.sparse as sp
+        t = self.torch
+        rp, ci, dirs, _ = self._pattern()
+        vals = t.empty(self.ctx.nnz, dtype=t.float64, device=self.device)
+        rhs = t.empty(self.ctx.ntrace, dtype=t.float64, device=self.device)
+        dirv = t.zeros(self.ctx.ntrace, dtype=t.float64, device=self.device)
+        self.ctx.condense(self.x_cell, self.x_trace, thermodyn, vals, rhs, dirv)
+        A = sp.csr_matrix((vals.cpu().numpy(), ci, rp), shape=(self.ctx.ntrace,) * 2)
+        return A, rhs.cpu().numpy(), dirs, dirv.cpu().numpy()[dirs]
+
+    def _private_run(self, abs_tol, rel_tol, max_it, thermodyn, log=None):
+        import scipy.sparse.linalg as spla
+        t = self.torch
+        rp, ci, dirs, r2f = self._pattern()
+        ctx = self.ctx
+        vals = t.empty(ctx.nnz, dtype=t.float64, device=self.device)
+        rhs = t.empty(ctx.ntrace, dtype=t.float64, device=self.device)
+        dirv = t.zeros(ctx.ntrace, dtype=t.float64, device=self.device)
+        upd_trace = t.zeros_like(self.x_trace)
+        upd_cell = t.zeros_like(self.x_cell)
+        dirs_t = t.as_tensor(dirs, device=self.device)
+        alpha = self.parameters.nonlinear_parameters.alpha
+        converged, step, upd_cell_norm = False, 0, 0.0
+        import scipy.sparse as sp
+        for step in range(1, max_it + 1 if max_it >= 0 else 1 << 30):
+            ctx.condense(self.x_cell, self.x_trace, thermodyn, vals, rhs, dirv)
+            A = sp.csc_matrix(sp.csr_matrix((vals.cpu().numpy(), ci, rp),
+                                            shape=(ctx.ntrace, ctx.ntrace)))
+            sol = spla.splu(A).solve(rhs.cpu().numpy())
+            sol_t = t.as_tensor(sol, device=self.device)
+            sol_t[dirs_t] = dirv[dirs_t]
+            upd_trace.zero_()
+            upd_trace[r2f] = sol_t
+            ctx.reconstruct(self.x_cell, self.x_trace, upd_trace, thermodyn, upd_cell)
+            upd_cell_norm = float(upd_cell.abs().max())
+            upd_norm = max(upd_cell_norm, float(upd_trace.abs().max()))
+            cur_norm = max(float(self.x_cell.abs().max()), float(self.x_trace.abs().max()))
+            self.x_cell.add_(upd_cell, alpha=alpha)
+            self.x_trace.add_(upd_trace, alpha=alpha)
+            if log is not None:
+                log(step, upd_norm, cur_norm)
+            if upd_norm < abs_tol or upd_norm < rel_tol * cur_norm:
+                converged = True
+                break
+        return NonlinearIterationResults(converged, step, upd_cell_norm)
+
+    def run(self, absolute_tol=None, relative_tol=None, max_number_of_iterations=None, log=None):
+        nl = self.parameters.nonlinear_parameters
+        return self._private_run(
+            nl.absolute_tolerance if absolute_tol is None else absolute_tol,
+            nl.relative_tolerance if relative_tol is None else relative_tol,
+            nl.max_number_of_iterations if max_number_of_iterations is None
+            else max_number_of_iterations, False, log)
+
+    def compute_thermodynamic_equilibrium(self, generate_first_guess=True, absolute_tol=None,
+                                          relative_tol=None, max_number_of_iterations=None,
+                                          log=None):
+        """src/thermodynamic_equilibrium.cpp:365-405."""
+        nl = self.parameters.nonlinear_parameters
+        saved = self.enabled
+        self.set_enabled_components(True, False, False)
+        if generate_first_guess:
+            self.compute_local_charge_neutrality()
+        res = self._private_run(
+            nl.absolute_tolerance if absolute_tol is None else absolute_tol,
+            nl.relative_tolerance if relative_tol is None else relative_tol,
+            nl.max_number_of_iterations if max_number_of_iterations is None
+            else max_number_of_iterations, True, log)
+        self.project_densities_from_potential(0.0, 0.0)
+        self.set_enabled_components(*saved)
+        return res

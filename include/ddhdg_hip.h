@@ -1,0 +1,166 @@
+/*
+ * ddhdg_hip.h -- C ABI of the MI355X (gfx950) per-cell HDG hot path.
+ *
+ * Replaces, at whole-mesh granularity, the reference operator
+ *   NPSolver<dim,Problem>::assemble_system(trace_reconstruct,
+ *                                          thermodyn_eq, multithreading)
+ *   (reference include/np_solver.h:627-630,
+ *    include/assembling_methods.template.h:10-89)
+ * together with the per-cell operator it dispatches
+ *   assemble_system_one_cell_internal<prm, hasBC, Tau, schur>
+ *   (include/assembling_methods.template.h:2290-2698)
+ * and its copier copy_local_to_global (src/np_solver.cpp:1293-1322).
+ *
+ * Plain C: pointers and sizes only.  Every function returns 0 on success or
+ * a negative code; hdg_last_error() (thread-local) holds the message.  A
+ * context owns its device tables; it is used from one host thread and one
+ * HIP stream at a time.
+ *
+ * State vectors live in device memory (HBM) and are passed as device
+ * pointers; the numbering is
+ *   cell c = ix + nx*(iy + ny*iz)
+ *   cell vector (all three components, dimensionless), per cell NC doubles:
+ *     comp*(dim+1)*nb + a*nb + j   flux (E, Wn, Wp) axis a, basis j
+ *     comp*(dim+1)*nb + dim*nb + j scalar (V, n, p), basis j
+ *   faces: axis-major, lexicographic on each face grid
+ *   full trace vector:       f*3*nf + comp*nf + j
+ *   restricted trace vector: f*ne*nf + ci*nf + j  (enabled components only)
+ * as the reference's deal.II FESystem/FE_FaceQ ordering within a cell.
+ */
+#ifndef DDHDG_HIP_H
+#define DDHDG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hdg_ctx hdg_ctx;
+
+/* Structured box mesh with colorized boundary ids (id 2a+s for the faces
+ * with normal axis a on side s), as GridGenerator::subdivided_hyper_
+ * rectangle(..., colorize=true) builds it.  Coordinates in scale_length
+ * units, as the reference's triangulation. */
+typedef struct
+{
+  int    dim;        /* 2 or 3 */
+  int    degree;     /* k: 2D 1..3, 3D 1 */
+  int    n_cells[3];
+  double lower[3];
+  double upper[3];
+} hdg_mesh_desc;
+
+/* Dimensionless coefficients, folded by the host layer from Problem and
+ * Adimensionalizer exactly as the reference ScratchData computes them
+ * (include/permittivity.h:67-83, include/mobility.h:66-82,
+ *  src/np_solver_parameters.cpp:13-28, include/adimensionalizer.h). */
+typedef struct
+{
+  double eps;            /* epsilon / permittivity rescaling factor */
+  double mu_n, mu_p;     /* mobility / mobility rescaling factor */
+  double U_T;            /* k_B T / q / thermal voltage rescaling */
+  double tau[3];         /* FixedTauComputer, rescaled, V n p */
+  int    recombination;  /* 0 none, 1 SRH fixed T, 2 linear */
+  double rec[3];         /* SRH: n_i (physical), tau_n, tau_p;
+                            linear: a, b, c */
+  double n_rescale;      /* doping magnitude (n, p rescaling factor) */
+  double rec_rescale;    /* recombination rescaling factor */
+  /* Boltzmann terms of the thermodynamic-equilibrium operator
+   * (include/assembling_methods.template.h:442-459, 677-693) */
+  double kT_over_q;      /* k_B T / (q V_rescale) */
+  double ec_r, ev_r;     /* band edges / (q V_rescale) */
+  double nc, nv;         /* band densities, physical */
+  int    flux_type;      /* 0 use_cell (only one supported) */
+  int    bc_type[6][3];  /* per boundary id and component:
+                            0 none, 1 Dirichlet, 2 Neumann */
+} hdg_phys;
+
+/* sizes returned by hdg_sizes */
+enum
+{
+  HDG_NCELLS = 0,
+  HDG_NFACES,
+  HDG_NC,        /* cell dofs per cell, all components */
+  HDG_QC,        /* cell quadrature points */
+  HDG_QF,        /* face quadrature points */
+  HDG_NB,        /* scalar basis per cell */
+  HDG_NF,        /* trace basis per face */
+  HDG_NE,        /* enabled components */
+  HDG_NS,        /* enabled scalar dofs per cell (reduced cell block) */
+  HDG_T,         /* enabled trace dofs per cell */
+  HDG_NTRACE,    /* restricted trace dofs (global) */
+  HDG_NNZ,       /* restricted trace matrix nonzeros */
+  HDG_NSIZES
+};
+
+const char *hdg_last_error(void);
+
+/* Context over one mesh on the current HIP device.  Allocates the device
+ * tables (geometry, lifting operators, scatter map).  All components
+ * enabled on creation. */
+int  hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys,
+                    hdg_ctx **out);
+void hdg_ctx_destroy(hdg_ctx *ctx);
+int  hdg_sizes(const hdg_ctx *ctx, long long out[HDG_NSIZES]);
+
+/* NPSolver::set_enabled_components (reference include/np_solver.h:172-175);
+ * rebuilds the restricted trace pattern. */
+int hdg_set_enabled(hdg_ctx *ctx, int V, int n, int p);
+
+/* Restricted trace matrix pattern, host arrays: rowptr (NTRACE+1),
+ * colind (NNZ); rows are sorted. */
+int hdg_csr_pattern(const hdg_ctx *ctx, long long *rowptr, int *colind);
+/* Restricted indices of the Dirichlet-constrained trace dofs (host array,
+ * returns the count when idx is NULL). */
+int hdg_dirichlet_dofs(const hdg_ctx *ctx, long long *idx);
+/* Quadrature points (host arrays): cells ncells*QC*dim, faces
+ * nfaces*QF*dim (face-local order, first tangent axis fastest). */
+int hdg_cell_qpoints(const hdg_ctx *ctx, double *xyz);
+int hdg_face_qpoints(const hdg_ctx *ctx, double *xyz);
+
+/* Rescaled doping at cell quadrature points (host array, ncells*QC),
+ * i.e. the reference's rescaled_doping evaluated at the quadrature points
+ * (src/np_solver.cpp:341-343). */
+int hdg_set_doping(hdg_ctx *ctx, const double *doping_qp);
+/* Boundary data at face quadrature points (host arrays, nfaces*3*QF):
+ * Dirichlet values divided by the component rescaling factor and Neumann
+ * values divided by the Neumann rescaling factor; entries of interior
+ * faces are ignored.  NULL leaves the current values. */
+int hdg_set_bc_values(hdg_ctx *ctx, const double *dirichlet,
+                      const double *neumann);
+
+/* Pass 1 of a Newton step: assembly + LocalCondenser + static condensation
+ * + deterministic scatter.  Replaces assemble_system(false, thermodyn, *)
+ * (template.h:10-89 with :2290-2687) and copy_local_to_global.
+ *   x_cell, x_trace   device, current state (ncells*NC, nfaces*3*nf)
+ *   csr_vals          device, NNZ values in the hdg_csr_pattern order
+ *   rhs               device, NTRACE (= system_rhs, minus the residual)
+ *   dir_vals          device, NTRACE; on Dirichlet dofs receives the
+ *                     projected Dirichlet update (template.h:1592-1691),
+ *                     other entries untouched
+ * Dirichlet rows carry a unit diagonal and a zero right-hand side.
+ * stream: hipStream_t (NULL for the default stream). */
+int hdg_condense(hdg_ctx *ctx, const double *x_cell, const double *x_trace,
+                 int thermodyn, double *csr_vals, double *rhs,
+                 double *dir_vals, void *stream);
+
+/* Pass 2: reconstruction of the cell update from the full trace update
+ * (nfaces*3*nf, Dirichlet values included).  Replaces
+ * assemble_system(true, thermodyn, *) (template.h:2357-2366, 2642-2643,
+ * 2688-2697).  dx_cell: device, ncells*NC, disabled components zero. */
+int hdg_reconstruct(hdg_ctx *ctx, const double *x_cell,
+                    const double *x_trace, const double *dx_trace,
+                    int thermodyn, double *dx_cell, void *stream);
+
+/* Condensed blocks of selected cells, the condensed counterpart of the
+ * reference's get_local_cell_system (src/np_solver.cpp:2337-2445):
+ * cells (device, ncell ints); S (device, ncell*T*T, row-major, local trace
+ * order face-major then component then basis) and g (device, ncell*T).
+ * Dirichlet rows and columns are zero, as in the reference. */
+int hdg_cell_systems(hdg_ctx *ctx, const double *x_cell,
+                     const double *x_trace, int thermodyn, const int *cells,
+                     int ncell, double *S, double *g, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+"""The C-ABI library loads and exports what include/ddhdg_hip.h declares;
+host-side set-up code against closed forms (no GPU calls)."""
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+import pytest
+
+from ddhdg_amd import fe, hip_lib
+from ddhdg_amd.problem import Adimensionalizer, EPSILON0, KB, Q, V, n
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "ddhdg_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(hdg_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(hip_lib.LIB_PATH), "run __graft_entry__.build() first"
+    L = ct.CDLL(hip_lib.LIB_PATH)
+    decl = declared_symbols()
+    assert decl, "no declarations parsed"
+    for name in decl:
+        assert hasattr(L, name), name
+    assert sorted(hip_lib.EXPORTS) == decl
+
+
+def test_adimensionalizer_closed_forms():
+    # include/adimensionalizer.h:28-262 evaluated by hand for (1e-6, q/kB, 1e23, 1)
+    a = Adimensionalizer(1e-6, Q / KB, 1e23, 1.0)
+    assert a.get_component_rescaling_factor(V) == pytest.approx(1.0, rel=1e-15)
+    assert a.get_component_rescaling_factor(n) == 1e23
+    assert a.get_permittivity_rescaling_factor() == pytest.approx(1e-12 * Q * 1e23, rel=1e-14)
+    assert a.get_recombination_rescaling_factor() == pytest.approx(1e35, rel=1e-14)
+    assert a.get_tau_rescaling_factor(V) == pytest.approx(Q * 1e23, rel=1e-14)
+    assert a.get_tau_rescaling_factor(n) == pytest.approx(1e12, rel=1e-14)
+
+
+@pytest.mark.parametrize("dim,k", [(2, 1), (2, 2), (2, 3), (3, 1)])
+def test_projection_reproduces_polynomials(dim, k):
+    """the cell L2 projection is exact on Q_k, and the flux projection of a
+    linear function is minus its gradient (integration by parts)."""
+    mesh = fe.BoxMesh(dim, [3] * dim, [0.0] * dim, [1.5] * dim)
+    ref = fe.RefCell(mesh, k)
+    from tests.helpers import _cell_qpoints, _face_qpoints
+    cq = _cell_qpoints(mesh, ref)
+    cfq = _face_qpoints(mesh, ref)[mesh.cell_faces]
+
+    def f(x):
+        return 0.3 + 1.7 * x[..., 0] - 0.4 * x[..., 1] + 0.2 * x[..., 0] ** k * x[..., 1] ** min(k, 1)
+    u, W = fe.project_cells(mesh, ref, f(cq), f(cfq))
+    vals = np.einsum("qj,cj->cq", ref.phic, u)
+    assert np.abs(vals - f(cq)).max() < 1e-12
+
+    def lin(x):
+        return 2.0 * x[..., 0] - 3.0 * x[..., dim - 1]
+    _, W = fe.project_cells(mesh, ref, lin(cq), lin(cfq))
+    assert np.abs(W[:, 0, :] + 2.0).max() < 1e-11
+    assert np.abs(W[:, dim - 1, :] - 3.0).max() < 1e-11
+
+
+def test_charge_neutrality_closed_form():
+    # src/thermodynamic_equilibrium.cpp:31-42 with Nc = Nv, Ec = Ev = 0
+    from ddhdg_amd import configs
+    prob, _, _ = configs.diode(2, 2, 1)
+    from ddhdg_amd.problem import local_charge_neutrality_potential
+    ut = 300 * KB / Q
+    v = local_charge_neutrality_potential(prob, np.array([1e23, -1e23, 0.0]), 300.0)
+    assert v[0] == pytest.approx(ut * np.arcsinh(1e23 / (2 * 1.45e16)), rel=1e-14)
+    assert v[1] == pytest.approx(-v[0], rel=1e-14) and v[2] == 0.0

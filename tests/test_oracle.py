@@ -1,0 +1,177 @@
+"""Pins of the CPU oracle against what the reference's own tests and the
+mathematics fix (no GPU needed)."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from ddhdg_amd import configs, fe
+from ddhdg_amd.problem import (DIRICHLET, NEUMANN, Q, KB, Adimensionalizer, BoundaryConditionHandler,
+                               FixedTauNPSolverParameters, HomogeneousMobility,
+                               HomogeneousPermittivity, HomogeneousProblem,
+                               LinearRecombinationTerm, NonlinearSolverParameters, V, n, p)
+from oracle import oracle as O
+from tests import helpers as H
+
+
+def srh_problem(ni, tn, tp):
+    return O.make_problem(2, 1, [1, 1, 1], [0, 0, 0], [1, 1, 1], epsilon=1, mu_n=1, mu_p=1,
+                          temperature=300, nc=1, nv=1, ec=0, ev=0, recombination=1,
+                          rec_params=(ni, tn, tp))
+
+
+def test_srh_known_answer():
+    # reference tests/recombination_terms.cpp:17-38
+    nn, pp, ni, tn, tp = 0.123, 0.42, 1.0, 2.0, 3.0
+    expected = (nn * pp - ni) / (tp * (nn + ni) + tn * (pp + ni))
+    r = O.recombination(srh_problem(ni, tn, tp), nn, pp, 1.0)
+    assert r[0] == pytest.approx(expected, rel=4e-16, abs=0)
+
+
+@pytest.mark.parametrize("rf", [1.0, 1e23])
+def test_srh_derivative_scaling(rf):
+    """dR/dn, dR/dp against a central difference of R in the rescaled
+    densities: the reference returns them without the x rf factor
+    (src/recombination_term.cpp:453-480), a quirk kept on purpose."""
+    P = srh_problem(0.5 * rf, 1e-6, 2e-6)
+    nn, pp = 0.7, 0.3
+    h = 1e-6
+    r = O.recombination(P, nn, pp, rf)
+    dn = (O.recombination(P, nn + h, pp, rf)[0] - O.recombination(P, nn - h, pp, rf)[0]) / (2 * h)
+    dp = (O.recombination(P, nn, pp + h, rf)[0] - O.recombination(P, nn, pp - h, rf)[0]) / (2 * h)
+    assert r[1] * rf == pytest.approx(dn, rel=1e-7)
+    assert r[2] * rf == pytest.approx(dp, rel=1e-7)
+
+
+def linear_problem(dim, n_cells, degree, eps, tauV, neumann=False):
+    """reference tests/linear_potential.cpp:12-184 (V = x, n = p = 0), with
+    the permittivity and tau_V chosen so that the rescaled coefficients are
+    O(1): with the reference's eps = 1 F/m and the default Adimensionalizer
+    the V rows are 6e18 times the flux rows and unscaled partial pivoting
+    (the reference's gauss_jordan) loses every digit."""
+    mesh = fe.BoxMesh(dim, [n_cells] * dim, [-1.0] * dim, [1.0] * dim)
+    bh = BoundaryConditionHandler()
+    for bid in range(2 * dim):
+        if neumann and bid >= 2:
+            # zero normal field on the y/z faces (tests/neumann_boundary_conditions.cpp)
+            bh.add_boundary_condition(bid, NEUMANN, V, 0.0)
+            bh.add_boundary_condition(bid, NEUMANN, n, 0.0)
+            bh.add_boundary_condition(bid, NEUMANN, p, 0.0)
+            continue
+        bh.add_boundary_condition(bid, DIRICHLET, V, lambda x: x[..., 0])
+        bh.add_boundary_condition(bid, DIRICHLET, n, 0.0)
+        bh.add_boundary_condition(bid, DIRICHLET, p, 0.0)
+    prob = HomogeneousProblem(mesh, HomogeneousPermittivity(eps), HomogeneousMobility(1.0, n),
+                              HomogeneousMobility(1.0, p), LinearRecombinationTerm(0, 0, 0),
+                              Q / KB, 0.0, bh, 1.0, 1.0, 0.0, 0.0)
+    params = FixedTauNPSolverParameters(degree, degree, degree,
+                                        NonlinearSolverParameters(1e-10, 1e-10, 1), tauV, 1, 1)
+    return prob, params, Adimensionalizer()
+
+
+@pytest.mark.parametrize("dim,n_cells,degree", [(2, 2, 1), (2, 2, 2), (2, 2, 3), (3, 1, 1), (3, 1, 2)])
+@pytest.mark.parametrize("neumann", [False, True])
+def test_linear_potential_exact(dim, n_cells, degree, neumann):
+    prob, params, adim = linear_problem(dim, n_cells, degree, Q, Q, neumann)
+    o = H.oracle_for(prob, params, adim)
+    res = o.run_newton(max_it=1)
+    assert res.iterations == 1
+    cell, tr = o.get_state()
+    mesh = prob.mesh
+    ref = fe.RefCell(mesh, degree)
+    nb, d = ref.nb, dim
+    cell = cell.reshape(mesh.ncells, 3, d + 1, nb)
+    # scalar nodes are equidistant: V dofs equal x at the nodes
+    idx = np.array(np.unravel_index(np.arange(nb), [degree + 1] * d, order="F")).T
+    xn = mesh.lower[0] + (mesh.ic[:, 0][:, None] + idx[None, :, 0] / degree) * mesh.h[0]
+    assert np.abs(cell[:, V, d, :] - xn).max() < 1e-10
+    assert np.abs(cell[:, V, 0, :] + 1.0).max() < 1e-10       # E = -grad V
+    assert np.abs(cell[:, V, 1:d, :]).max() < 1e-10
+    assert np.abs(cell[:, n]).max() < 1e-10 and np.abs(cell[:, p]).max() < 1e-10
+
+
+def test_qpoints_layout_matches_host_tables():
+    prob, params, adim = configs.diode(2, 4, 2)
+    o = H.oracle_for(prob, params, adim)
+    ref = fe.RefCell(prob.mesh, 2)
+    assert np.allclose(o.cell_qpoints(), H._cell_qpoints(prob.mesh, ref), atol=1e-15)
+    assert np.allclose(o.face_qpoints(), H._face_qpoints(prob.mesh, ref), atol=1e-15)
+    prob3, params3, adim3 = configs.diode(3, 2, 1)
+    o3 = H.oracle_for(prob3, params3, adim3)
+    ref3 = fe.RefCell(prob3.mesh, 1)
+    assert np.allclose(o3.face_qpoints(), H._face_qpoints(prob3.mesh, ref3), atol=1e-15)
+
+
+def _full_system(o, thermodyn=False):
+    """uncondensed global system from the oracle's local cell systems."""
+    N, T, nc, ntr = o.N, o.T, o.ncells, o.ntr
+    rows, cols, vals = [], [], []
+    rhs = np.zeros(nc * N + ntr)
+    dirs = {}
+    for c in range(nc):
+        A, CT, TC, TT, cc, t = o.local_system(c, thermodyn)
+        S, g, dofs, di, dv = o.condensed_cell(c, thermodyn)
+        for i, v in zip(di, dv):
+            dirs[int(i)] = v
+        r0 = c * N
+        ii, jj = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+        rows += list((r0 + ii).ravel()); cols += list((r0 + jj).ravel()); vals += list(A.ravel())
+        ii, jj = np.meshgrid(np.arange(N), np.arange(T), indexing="ij")
+        rows += list((r0 + ii).ravel()); cols += list((nc * N + dofs[jj]).ravel()); vals += list(CT.ravel())
+        ii, jj = np.meshgrid(np.arange(T), np.arange(N), indexing="ij")
+        rows += list((nc * N + dofs[ii]).ravel()); cols += list((r0 + jj).ravel()); vals += list(-TC.ravel())
+        ii, jj = np.meshgrid(np.arange(T), np.arange(T), indexing="ij")
+        rows += list((nc * N + dofs[ii]).ravel()); cols += list((nc * N + dofs[jj]).ravel()); vals += list(TT.ravel())
+        rhs[r0:r0 + N] = cc
+        np.add.at(rhs, nc * N + dofs, t)
+    M = sp.coo_matrix((vals, (rows, cols)), shape=(nc * N + ntr,) * 2).tolil()
+    for i, v in dirs.items():
+        M[nc * N + i, :] = 0
+        M[nc * N + i, nc * N + i] = 1
+        rhs[nc * N + i] = v
+    return M.tocsr(), rhs, dirs
+
+
+@pytest.mark.parametrize("dim,n_cells,degree", [(2, 2, 1), (2, 3, 2), (3, 2, 1)])
+def test_schur_identity(dim, n_cells, degree):
+    """condense -> trace solve -> reconstruct equals a direct solve of the
+    uncondensed system (the Schur-complement identity), on a perturbed diode
+    state."""
+    prob, params, adim = configs.diode(dim, n_cells, degree)
+    o = H.oracle_for(prob, params, adim)
+    cell, tr = H.perturbed(*H.equilibrium_guess(prob, adim, degree), seed=3)
+    o.set_state(cell, tr)
+    M, rhs, dirs = _full_system(o)
+    x = sp.linalg.spsolve(M.tocsc(), rhs)
+    o.pass1()
+    A, b, di, dv = o.system()
+    lam = sp.linalg.spsolve(A.tocsc(), b)
+    lam[di] = dv
+    r2f = o.restricted_to_full((1, 1, 1))
+    ut = np.zeros(o.nfaces * 3 * o.nf)
+    ut[r2f] = lam
+    uc = o.pass2(ut)
+    N = o.N
+    ucr = uc.reshape(o.ncells, o.NC)
+    xc = x[: o.ncells * N].reshape(o.ncells, N)
+    assert H.rel_fro(lam, x[o.ncells * N:]) < 1e-11
+    assert H.rel_fro(ucr, xc) < 1e-10
+
+
+def test_workstream_threads_bitwise():
+    """threaded pass 1 (in-order copier) is bitwise identical to serial."""
+    prob, params, adim = configs.diode(2, 8, 2)
+    o = H.oracle_for(prob, params, adim)
+    o.set_state(*H.perturbed(*H.equilibrium_guess(prob, adim, 2), seed=5))
+    o.pass1(nthreads=1)
+    A1, b1, _, _ = o.system()
+    o.pass1(nthreads=4)
+    A4, b4, _, _ = o.system()
+    assert np.array_equal(A1.data, A4.data) and np.array_equal(b1, b4)
+
+
+def test_oracle_equilibrium_newton_converges():
+    prob, params, adim = configs.diode(2, 8, 1)
+    o = H.oracle_for(prob, params, adim)
+    o.set_state(*H.equilibrium_guess(prob, adim, 1))
+    res = o.run_newton(enabled=(True, False, False), thermodyn=True, max_it=30)
+    assert res.converged and res.iterations < 15
