@@ -21,7 +21,7 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I" + os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function"]
-SOURCES = ["hdg_tables.cpp", "hdg_kernels.hip", "hdg_capi.hip"]
+SOURCES = ["hdg_tables.cpp", "hdg_kernels.hip", "hdg_fast2d.hip", "hdg_capi.hip"]
 
 
 def _run(cmd):

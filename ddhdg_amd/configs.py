@@ -47,8 +47,14 @@ def scaled_unit_tau(adim):
     return tuple(adim.get_tau_rescaling_factor(c) for c in (V, n, p))
 
 
-def diode(dim, n_cells, degree, doping=None, bias=0.0, tau=None, nonlinear=None):
-    """(problem, parameters, adimensionalizer) of a box diode."""
+def diode(dim, n_cells, degree, doping=None, bias=0.0, tau=None, nonlinear=None,
+          consistent_drift_jacobian=True):
+    """(problem, parameters, adimensionalizer) of a box diode.
+
+    The diode runs use the consistent linearisation of the volume drift term
+    (the reference's own converges at best linearly and diverges on these
+    1e23 m^-3 junctions); pass consistent_drift_jacobian=False for the
+    reference's."""
     mesh = BoxMesh(dim, [n_cells] * dim, [0.0] * dim, [1.0] * dim)
     doping = doping or step_doping()
     bh = BoundaryConditionHandler()
@@ -71,6 +77,7 @@ def diode(dim, n_cells, degree, doping=None, bias=0.0, tau=None, nonlinear=None)
         degree, degree, degree,
         nonlinear or NonlinearSolverParameters(1e-10, 1e-10, 100, 1.0),
         tau[0], tau[1], tau[2])
+    params.consistent_drift_jacobian = consistent_drift_jacobian
     return prob, params, adim
 
 
@@ -100,3 +107,20 @@ def set_bias(solver, bias):
     prob = solver.problem
     solver.replace_boundary_condition(
         1, DIRICHLET, V, lambda x: contact_potential(prob, x) + bias)
+
+
+def solve_equilibrium_and_bias(solver, biases=(), log=None):
+    """The reference's diode workflow (examples/residual.ipynb cells 4-17):
+    thermodynamic equilibrium (V only, Boltzmann carriers), densities from
+    the potential, n and p with V frozen, then the coupled system; then one
+    coupled solve per bias point, the Newton state kept between them.
+    Returns the list of NonlinearIterationResults (one per stage)."""
+    results = [solver.compute_thermodynamic_equilibrium(log=log)]
+    solver.set_enabled_components(False, True, True)
+    results.append(solver.run(log=log))
+    solver.set_enabled_components(True, True, True)
+    results.append(solver.run(log=log))
+    for b in biases:
+        set_bias(solver, b)
+        results.append(solver.run(log=log))
+    return results

@@ -39,6 +39,7 @@ struct hdg_ctx
   double        h[3], lo[3];
   long long     face_off[4], face_dims[3][3];
   int           en[3], ne, encomp[3];
+  int           variant = 0; // 0 auto, 1 generic, 2 register-resident
   hdg::HostTables ht;
   // block pattern: per face, sorted faces it couples with
   std::vector<long long>     blk_cols;  // concatenated
@@ -122,6 +123,13 @@ namespace
     return 0;
   }
 
+  bool use_fast(const hdg_ctx *C, int thermodyn)
+  {
+    if (C->variant == 1)
+      return false;
+    return hdg::fast_supported(C->d, C->k, C->ne, thermodyn, C->en);
+  }
+
   hdg::KParams params(const hdg_ctx *C, int thermodyn)
   {
     hdg::KParams p{};
@@ -158,6 +166,7 @@ namespace
     for (int b = 0; b < 6; ++b)
       for (int c = 0; c < 3; ++c)
         p.bc[b][c] = C->phys.bc_type[b][c];
+    p.sdrift = C->phys.drift_jacobian ? -1.0 : 1.0;
     p.tb     = C->tb;
     p.doping = C->d_doping;
     p.dirv   = C->d_dirv;
@@ -368,6 +377,19 @@ int hdg_set_enabled(hdg_ctx *C, int V, int n, int p)
   return 0;
 }
 
+int hdg_set_kernel_variant(hdg_ctx *C, int variant)
+{
+  if (!C)
+    return fail("hdg_set_kernel_variant: null context");
+  if (variant < 0 || variant > 2)
+    return fail("hdg_set_kernel_variant: variant must be 0, 1 or 2", -2);
+  if (variant == 2 && !hdg::fast_supported(C->d, C->k, C->ne, 0, C->en) &&
+      !hdg::fast_supported(C->d, C->k, C->ne, 1, C->en))
+    return fail("hdg_set_kernel_variant: no register-resident kernel for this shape", -2);
+  C->variant = variant;
+  return 0;
+}
+
 int hdg_csr_pattern(const hdg_ctx *C, long long *rowptr, int *colind)
 {
   if (!C || !rowptr || !colind)
@@ -510,11 +532,13 @@ int hdg_condense(hdg_ctx *C, const double *x_cell, const double *x_trace, int th
   p.vals   = csr_vals;
   p.rhs    = rhs;
   p.dirout = dir_vals;
-  hipStream_t s = (hipStream_t)stream;
+  hipStream_t s    = (hipStream_t)stream;
+  const bool  fast = use_fast(C, thermodyn);
   for (int col = 0; col < 2; ++col)
     {
-      hipError_t e = hdg::launch_condense(C->d, C->k, p, C->d_cells[col],
-                                          (int)C->color_cells[col].size(), col, 0, s);
+      const int  nc = (int)C->color_cells[col].size();
+      hipError_t e  = fast ? hdg::launch_fast_condense(C->k, p, C->d_cells[col], nc, col, 0, s)
+                           : hdg::launch_condense(C->d, C->k, p, C->d_cells[col], nc, col, 0, s);
       if (e != hipSuccess)
         return hip_fail(e, "hdg_condense launch");
     }
@@ -549,7 +573,9 @@ int hdg_cell_systems(hdg_ctx *C, const double *x_cell, const double *x_trace, in
   p.xt   = x_trace;
   p.Sout = S;
   p.gout = g;
-  hipError_t e = hdg::launch_condense(C->d, C->k, p, cells, ncell, 0, 1, (hipStream_t)stream);
+  hipError_t e = use_fast(C, thermodyn)
+                   ? hdg::launch_fast_condense(C->k, p, cells, ncell, 0, 1, (hipStream_t)stream)
+                   : hdg::launch_condense(C->d, C->k, p, cells, ncell, 0, 1, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_cell_systems launch");
 }
 

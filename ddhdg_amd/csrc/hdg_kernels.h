@@ -29,6 +29,7 @@ namespace hdg
     double rec0, rec1, rec2, n_rf, rec_rf;
     double kTq, ec_r, ev_r, nc, nv;
     int    bc[6][3];
+    double sdrift; // +1 reference linearisation, -1 consistent
     DevTables tb;
     // per-mesh data
     const double *doping;   // ncells*QC
@@ -50,4 +51,9 @@ namespace hdg
   hipError_t launch_reconstruct(int dim, int deg, const KParams &p, int ncells,
                                 hipStream_t s);
   bool supported(int dim, int deg);
+  // register-resident pass 1 (hdg_fast2d.hip): 2D, degree 1-2, all three
+  // components or the V-only thermodynamic-equilibrium operator
+  bool       fast_supported(int dim, int deg, int ne, int thermo, const int en[3]);
+  hipError_t launch_fast_condense(int deg, const KParams &p, const int *cells, int ncell,
+                                  int color, int mode, hipStream_t s);
 } // namespace hdg

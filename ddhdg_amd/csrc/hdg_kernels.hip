@@ -22,6 +22,7 @@
 //   trace rows:       xi_i * F_c       assemble_tc_matrix, assemble_tt_matrix,
 //        add_tc/tt_matrix_terms_to_tt_rhs (:1145-1588), apply_nbc_on_face
 // where F_c is the HDG numerical flux of component c (use_cell).
+#include "hdg_device.h"
 #include "hdg_kernels.h"
 
 namespace hdg
@@ -76,45 +77,6 @@ namespace hdg
       static constexpr int NINT   = iBID + NFC;
       static constexpr size_t bytes = NDBL * sizeof(double) + NINT * sizeof(int);
     };
-
-    __device__ inline long long face_of(const KParams &p, int ix, int iy, int iz,
-                                        int lf)
-    {
-      int       pos[3] = {ix, iy, iz};
-      const int a      = lf >> 1;
-      pos[a] += lf & 1;
-      const long long *m = p.face_dims[a];
-      return p.face_off[a] + pos[0] + m[0] * (pos[1] + m[1] * (long long)pos[2]);
-    }
-
-    // SRH / linear recombination with the reference's scaling
-    // (src/recombination_term.cpp:433-480, src/adimensionalizer.cpp:476-497)
-    __device__ inline void recomb(const KParams &p, double n, double pp, double &r,
-                                  double &drn, double &drp)
-    {
-      if (p.rec == 1)
-        {
-          const double ni  = p.rec0 / p.n_rf;
-          const double num = n * pp - ni * ni;
-          const double den = p.rec2 * (n + ni) + p.rec1 * (pp + ni);
-          r                = num * p.n_rf / den;
-          drn              = pp / den - num * p.rec2 / (den * den);
-          drp              = n / den - num * p.rec1 / (den * den);
-        }
-      else if (p.rec == 2)
-        {
-          r   = p.rec0 + (p.rec1 * n + p.rec2 * pp) * p.n_rf;
-          drn = p.rec1;
-          drp = p.rec2;
-        }
-      else
-        {
-          r = drn = drp = 0;
-        }
-      r /= p.rec_rf;
-      drn /= p.rec_rf;
-      drp /= p.rec_rf;
-    }
 
     // ------------------------------------------------------------------
     // Shared first half of both passes: load, state at quadrature points,
@@ -410,14 +372,14 @@ namespace hdg
                       else if (c == 1)
                         {
                           const double n0 = u0[QC + q];
-                          val = -mun * E0a * uu[1] + n0 * mun * WW[0] + UT * mun * WW[1];
+                          val = -mun * E0a * uu[1] + p.sdrift * n0 * mun * WW[0] + UT * mun * WW[1];
                           if (kind == 2)
                             val += n0 * mun * E0a - UT * mun * W0[(1 * DIM + a) * QC + q];
                         }
                       else
                         {
                           const double p0 = u0[2 * QC + q];
-                          val = mup * E0a * uu[2] - p0 * mup * WW[0] + UT * mup * WW[2];
+                          val = mup * E0a * uu[2] - p.sdrift * p0 * mup * WW[0] + UT * mup * WW[2];
                           if (kind == 2)
                             val += -p0 * mup * E0a - UT * mup * W0[(2 * DIM + a) * QC + q];
                         }

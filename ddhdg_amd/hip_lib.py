@@ -24,7 +24,7 @@ EXPORTS = [
     "hdg_last_error", "hdg_ctx_create", "hdg_ctx_destroy", "hdg_sizes", "hdg_set_enabled",
     "hdg_csr_pattern", "hdg_dirichlet_dofs", "hdg_cell_qpoints", "hdg_face_qpoints",
     "hdg_set_doping", "hdg_set_bc_values", "hdg_condense", "hdg_reconstruct",
-    "hdg_cell_systems",
+    "hdg_cell_systems", "hdg_set_kernel_variant",
 ]
 
 
@@ -44,7 +44,7 @@ class Phys(ct.Structure):
                 ("rec_rescale", ct.c_double), ("kT_over_q", ct.c_double),
                 ("ec_r", ct.c_double), ("ev_r", ct.c_double), ("nc", ct.c_double),
                 ("nv", ct.c_double), ("flux_type", ct.c_int),
-                ("bc_type", (ct.c_int * 3) * 6)]
+                ("bc_type", (ct.c_int * 3) * 6), ("drift_jacobian", ct.c_int)]
 
 
 def lib():
@@ -61,6 +61,7 @@ def lib():
             "hdg_ctx_destroy": (None, [v]),
             "hdg_sizes": (i, [v, lp]),
             "hdg_set_enabled": (i, [v, i, i, i]),
+            "hdg_set_kernel_variant": (i, [v, i]),
             "hdg_csr_pattern": (i, [v, lp, ip]),
             "hdg_dirichlet_dofs": (i, [v, lp]),
             "hdg_cell_qpoints": (i, [v, dp]),
@@ -130,6 +131,10 @@ class HdgContext:
     def set_enabled(self, V, n, p):
         _check(self.L.hdg_set_enabled(self.ctx, int(V), int(n), int(p)))
         self._refresh()
+
+    def set_kernel_variant(self, variant):
+        """0 automatic, 1 LDS workgroup-per-cell kernel, 2 register-resident."""
+        _check(self.L.hdg_set_kernel_variant(self.ctx, int(variant)))
 
     def csr_pattern(self):
         rp = np.zeros(self.ntrace + 1, dtype=np.int64)

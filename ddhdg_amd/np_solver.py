@@ -77,6 +77,7 @@ def fold_physics(problem, parameters, adim) -> Phys:
     if parameters.phi_linearize:
         raise NotImplementedError("phi_linearize is not implemented")
     ph.flux_type = 0
+    ph.drift_jacobian = 1 if parameters.consistent_drift_jacobian else 0
     tab = problem.boundary_handler.bc_type_table()
     for b in range(6):
         for c in range(3):

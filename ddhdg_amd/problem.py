@@ -176,6 +176,11 @@ class FixedTauNPSolverParameters:
     multithreading: bool = True
     dd_flux_type: str = "use_cell"
     phi_linearize: bool = False
+    # False: the reference's linearisation of the volume drift term
+    # (include/assembling_methods.template.h:500, 521-522), whose sign is
+    # opposite to the derivative of its own residual (:656, :673), so that
+    # Newton is inexact; True: the consistent linearisation
+    consistent_drift_jacobian: bool = False
 
 
 def compute_density(problem, c, qf_potential, potential, temperature):

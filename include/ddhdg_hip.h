@@ -72,6 +72,10 @@ typedef struct
   int    flux_type;      /* 0 use_cell (only one supported) */
   int    bc_type[6][3];  /* per boundary id and component:
                             0 none, 1 Dirichlet, 2 Neumann */
+  int    drift_jacobian; /* 0: the reference's linearisation of the volume
+                            drift term in E (template.h:500, 521-522), whose
+                            sign is opposite to the derivative of its residual
+                            (:656, :673); 1: the consistent (exact) one */
 } hdg_phys;
 
 /* sizes returned by hdg_sizes */
@@ -105,6 +109,13 @@ int  hdg_sizes(const hdg_ctx *ctx, long long out[HDG_NSIZES]);
 /* NPSolver::set_enabled_components (reference include/np_solver.h:172-175);
  * rebuilds the restricted trace pattern. */
 int hdg_set_enabled(hdg_ctx *ctx, int V, int n, int p);
+
+/* Kernel choice for pass 1: 0 = automatic (register-resident wave-per-cell
+ * kernel where it applies: 2D, degree 1-2, all components or the V-only
+ * equilibrium operator; otherwise the LDS workgroup-per-cell kernel),
+ * 1 = always the LDS kernel, 2 = register-resident (error if it does not
+ * apply).  No counterpart in the reference (one CPU code path). */
+int hdg_set_kernel_variant(hdg_ctx *ctx, int variant);
 
 /* Restricted trace matrix pattern, host arrays: rowptr (NTRACE+1),
  * colind (NNZ); rows are sorted. */

@@ -686,6 +686,11 @@ namespace
       const bool eV = C->en[CV], eN = C->en[CN], eP = C->en[CP];
       const bool phimode = thermo || C->P.phi_linearize;
       const double eps = C->eps_r, mun = C->mun_r, mup = C->mup_r;
+      // volume drift coupling to the E trial function: the reference adds
+      // +n0 (mu_n E . grad z2) and -p0 (mu_p E . grad z3) (template.h:500,
+      // 521-522); the derivative of its residual J . grad z (:656, :673) has
+      // the opposite signs
+      const double sdrift = C->P.drift_jacobian ? -1.0 : 1.0;
       for (int q = 0; q < C->Qc; ++q)
         {
           const double JW = JxW[q];
@@ -717,7 +722,7 @@ namespace
                       const Vec3 Dn   = scale(scale(q2[jj], mun), UT[q]);
                       a += (-z2[jj] * q2d[ii] + dot(q2[jj], q2[ii], d) -
                             z2[jj] * dot(munE0, g2[ii], d) +
-                            n0[q] * dot(munE, g2[ii], d) + dot(Dn, g2[ii], d) -
+                            sdrift * n0[q] * dot(munE, g2[ii], d) + dot(Dn, g2[ii], d) -
                             (drn[q] * z2[jj] + drp[q] * z3[jj]) * z2[ii]) *
                            JW;
                     }
@@ -727,7 +732,7 @@ namespace
                       const Vec3 Dp   = scale(scale(q3[jj], mup), UT[q]);
                       a += (-z3[jj] * q3d[ii] + dot(q3[jj], q3[ii], d) +
                             z3[jj] * dot(mupE0, g3[ii], d) -
-                            p0[q] * dot(mupE, g3[ii], d) + dot(Dp, g3[ii], d) -
+                            sdrift * p0[q] * dot(mupE, g3[ii], d) + dot(Dp, g3[ii], d) -
                             (drn[q] * z2[jj] + drp[q] * z3[jj]) * z3[ii]) *
                            JW;
                     }

@@ -57,6 +57,10 @@ typedef struct
                             doping_magnitude, mobility_magnitude */
   int    bc_type[6][3];  /* per boundary id, per component:
                             0 none, 1 Dirichlet, 2 Neumann */
+  int    drift_jacobian; /* 0: the reference's linearisation of the volume
+                            drift term (template.h:500, 521-522, sign opposite
+                            to the residual's J at :656, :673); 1: the
+                            consistent one (Newton Jacobian exact) */
 } orc_problem;
 
 int  orc_create(const orc_problem *prob, orc_ctx **out);

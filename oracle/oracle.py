@@ -41,6 +41,7 @@ class OrcProblem(ct.Structure):
         ("phi_linearize", ct.c_int),
         ("adim", ct.c_double * 4),
         ("bc_type", (ct.c_int * 3) * 6),
+        ("drift_jacobian", ct.c_int),
     ]
 
 
@@ -275,7 +276,7 @@ def recombination(problem: OrcProblem, n, p, rf):
 def make_problem(dim, degree, n_cells, lower, upper, *, epsilon, mu_n, mu_p, temperature,
                  nc, nv, ec, ev, recombination=0, rec_params=(0, 0, 0), tau=(1, 1, 1),
                  flux_type=0, phi_linearize=False, adim=(1.0, 1.602176634e-19 / 1.380649e-23, 1.0, 1.0),
-                 bc_type=None) -> OrcProblem:
+                 bc_type=None, drift_jacobian=0) -> OrcProblem:
     P = OrcProblem()
     P.dim, P.degree = dim, degree
     for a in range(3):
@@ -291,6 +292,7 @@ def make_problem(dim, degree, n_cells, lower, upper, *, epsilon, mu_n, mu_p, tem
     P.flux_type, P.phi_linearize = flux_type, int(phi_linearize)
     for i in range(4):
         P.adim[i] = adim[i]
+    P.drift_jacobian = int(drift_jacobian)
     if bc_type is not None:
         for bid in range(6):
             for c in range(3):

@@ -32,7 +32,8 @@ def oracle_for(problem, params, adim, enabled=(True, True, True)):
         recombination=rtype, rec_params=rp, tau=(params.V_tau, params.n_tau, params.p_tau),
         adim=(adim.scale_length, adim.temperature_magnitude, adim.doping_magnitude,
               adim.mobility_magnitude),
-        bc_type=problem.boundary_handler.bc_type_table())
+        bc_type=problem.boundary_handler.bc_type_table(),
+        drift_jacobian=int(params.consistent_drift_jacobian))
     o = O.Oracle(P)
     o.set_enabled(*enabled)
     o.set_doping(evaluate(problem.doping, o.cell_qpoints()))

@@ -21,10 +21,12 @@ def _torch():
     return torch
 
 
-def _solver(dim, n_cells, k, **kw):
+def _solver(dim, n_cells, k, variant=0, **kw):
     from ddhdg_amd.np_solver import NPSolver
     prob, params, adim = configs.diode(dim, n_cells, k, **kw)
-    return NPSolver(prob, params, adim), prob, params, adim
+    s = NPSolver(prob, params, adim)
+    s.ctx.set_kernel_variant(variant)
+    return s, prob, params, adim
 
 
 def _state(prob, adim, k, seed):
@@ -33,10 +35,14 @@ def _state(prob, adim, k, seed):
 
 @pytest.mark.parametrize("dim,n_cells,k", CASES)
 @pytest.mark.parametrize("thermodyn", [False, True])
-def test_cell_systems_match_oracle(dim, n_cells, k, thermodyn):
+@pytest.mark.parametrize("variant", [1, 0])
+@pytest.mark.parametrize("consistent", [False, True])
+def test_cell_systems_match_oracle(dim, n_cells, k, thermodyn, variant, consistent):
     t = _torch()
-    print(f"case cell_systems dim={dim} n={n_cells} k={k} thermo={thermodyn}", flush=True)
-    s, prob, params, adim = _solver(dim, n_cells, k)
+    print(f"case cell_systems dim={dim} n={n_cells} k={k} thermo={thermodyn} "
+          f"variant={variant} consistent={consistent}", flush=True)
+    s, prob, params, adim = _solver(dim, n_cells, k, variant,
+                                    consistent_drift_jacobian=consistent)
     enabled = (True, False, False) if thermodyn else (True, True, True)
     s.set_enabled_components(*enabled)
     o = H.oracle_for(prob, params, adim, enabled)
@@ -61,10 +67,11 @@ def test_cell_systems_match_oracle(dim, n_cells, k, thermodyn):
 
 
 @pytest.mark.parametrize("dim,n_cells,k", CASES)
-def test_global_system_matches_oracle(dim, n_cells, k):
+@pytest.mark.parametrize("variant", [1, 0])
+def test_global_system_matches_oracle(dim, n_cells, k, variant):
     t = _torch()
-    print(f"case global_system dim={dim} n={n_cells} k={k}", flush=True)
-    s, prob, params, adim = _solver(dim, n_cells, k)
+    print(f"case global_system dim={dim} n={n_cells} k={k} variant={variant}", flush=True)
+    s, prob, params, adim = _solver(dim, n_cells, k, variant)
     o = H.oracle_for(prob, params, adim)
     cell, tr = _state(prob, adim, k, seed=7)
     s.set_state(cell, tr)
@@ -123,10 +130,23 @@ def test_condense_is_deterministic():
     assert np.array_equal(A1.data, A2.data) and np.array_equal(b1, b2)
 
 
+def _oracle_stage(o, enabled, thermodyn=False):
+    return o.run_newton(enabled=enabled, thermodyn=thermodyn, max_it=100)
+
+
+def _close(s, o, tol=1e-9):
+    cg, tg = s.get_state()
+    co, to = o.get_state()
+    return (np.abs(cg - co).max() <= tol * np.abs(co).max() and
+            np.abs(tg - to).max() <= tol * np.abs(to).max())
+
+
 @pytest.mark.parametrize("dim,n_cells,k", [(2, 12, 1), (2, 8, 2)])
 def test_newton_diode_matches_oracle(dim, n_cells, k):
-    """equilibrium (V-only Boltzmann Newton) then 0.1 V forward bias: equal
-    Newton iteration counts and final states."""
+    """the reference's diode workflow (equilibrium, n and p with V frozen,
+    coupled system, 0.1 V and 0.3 V forward bias): equal Newton iteration
+    counts at every stage and equal final states.  Both sides start every
+    stage from the same state."""
     _torch()
     print(f"case newton dim={dim} n={n_cells} k={k}", flush=True)
     s, prob, params, adim = _solver(dim, n_cells, k)
@@ -134,25 +154,45 @@ def test_newton_diode_matches_oracle(dim, n_cells, k):
     s.set_enabled_components(True, False, False)
     s.compute_local_charge_neutrality()
     o.set_state(*s.get_state())
-    r_gpu = s._private_run(1e-10, 1e-10, 50, True)
-    r_orc = o.run_newton(enabled=(True, False, False), thermodyn=True, max_it=50)
+    r_gpu = s._private_run(1e-10, 1e-10, 100, True)
+    r_orc = _oracle_stage(o, (True, False, False), True)
     print(f"  equilibrium iterations gpu={r_gpu.iterations} oracle={r_orc.iterations}")
     assert r_gpu.converged and r_orc.converged and r_gpu.iterations == r_orc.iterations
-    cg, tg = s.get_state()
-    co, to = o.get_state()
-    assert np.abs(cg - co).max() <= 1e-9 * np.abs(co).max()
+    assert _close(s, o)
+    s.project_densities_from_potential()
+    stages = [("n,p", (False, True, True), None), ("coupled 0 V", (True, True, True), None),
+              ("0.1 V", (True, True, True), 0.1), ("0.3 V", (True, True, True), 0.3)]
+    for name, en, bias in stages:
+        s.set_enabled_components(*en)
+        if bias is not None:
+            configs.set_bias(s, bias)
+        o = H.oracle_for(prob, params, adim)
+        o.set_state(*s.get_state())
+        r_gpu = s.run()
+        r_orc = _oracle_stage(o, en)
+        print(f"  {name}: iterations gpu={r_gpu.iterations} oracle={r_orc.iterations}")
+        assert r_gpu.converged and r_orc.converged, name
+        assert r_gpu.iterations == r_orc.iterations, name
+        assert _close(s, o), name
+
+
+def test_newton_reference_linearisation_matches_oracle():
+    """the reference's own drift linearisation (inexact Newton) on a lightly
+    doped diode where it converges: same iteration count, same state"""
+    _torch()
+    print("case newton reference linearisation 2D n=8 k=1 doping 1e21", flush=True)
+    s, prob, params, adim = _solver(2, 8, 1, doping=configs.step_doping(1e21, 1e21),
+                                    consistent_drift_jacobian=False)
+    s.set_enabled_components(True, False, False)
+    s.compute_local_charge_neutrality()
+    s._private_run(1e-10, 1e-10, 100, True)
     s.project_densities_from_potential()
     s.set_enabled_components(True, True, True)
-    o.set_state(*s.get_state())
     configs.set_bias(s, 0.1)
-    prob.boundary_handler  # same handler object: refresh the oracle's values
-    o2 = H.oracle_for(prob, params, adim)
-    o2.set_state(*s.get_state())
+    o = H.oracle_for(prob, params, adim)
+    o.set_state(*s.get_state())
     r_gpu = s.run()
-    r_orc = o2.run_newton(max_it=100)
-    print(f"  0.1 V iterations gpu={r_gpu.iterations} oracle={r_orc.iterations}")
+    r_orc = o.run_newton(max_it=100)
+    print(f"  iterations gpu={r_gpu.iterations} oracle={r_orc.iterations}")
     assert r_gpu.converged and r_orc.converged and r_gpu.iterations == r_orc.iterations
-    cg, tg = s.get_state()
-    co, to = o2.get_state()
-    assert np.abs(cg - co).max() <= 1e-9 * np.abs(co).max()
-    assert np.abs(tg - to).max() <= 1e-9 * np.abs(to).max()
+    assert _close(s, o)

@@ -175,3 +175,52 @@ def test_oracle_equilibrium_newton_converges():
     o.set_state(*H.equilibrium_guess(prob, adim, 1))
     res = o.run_newton(enabled=(True, False, False), thermodyn=True, max_it=30)
     assert res.converged and res.iterations < 15
+
+
+def _fd_jacobian_mismatch(consistent, cell):
+    """relative mismatch between the local Jacobian and a central difference
+    of the local residual, n rows and p rows (R = 0 so that the reference's
+    SRH-derivative scaling does not enter)."""
+    prob, params, adim = configs.diode(2, 4, 1, consistent_drift_jacobian=consistent)
+    prob.recombination_term = LinearRecombinationTerm(0, 0, 0)
+    o = H.oracle_for(prob, params, adim)
+    c0, t0 = H.perturbed(*H.equilibrium_guess(prob, adim, 1), seed=4)
+    o.set_state(c0, t0)
+    A, CT, TC, TT, cc, t = o.local_system(cell)
+    _, _, dofs, _, _ = o.condensed_cell(cell)
+    rng = np.random.default_rng(0)
+    vu, vl = rng.standard_normal(A.shape[0]), rng.standard_normal(TT.shape[0])
+    NC, h = o.NC, 1e-6
+
+    def resid(e):
+        ce, te = c0.copy(), t0.copy()
+        ce[cell * NC:(cell + 1) * NC] += e * vu
+        te[dofs] += e * vl
+        o.set_state(ce, te)
+        out = o.local_system(cell)
+        return out[4], out[5]
+    (cp, tp), (cm, tm) = resid(h), resid(-h)
+    dc, dt = (cp - cm) / (2 * h), (tp - tm) / (2 * h)
+    pc, pt = -(A @ vu + CT @ vl), -(-TC @ vu + TT @ vl)
+    nb = o.nb
+    rows = {c: np.arange(c * 3 * nb, (c + 1) * 3 * nb) for c in range(3)}
+    return {c: H.rel_fro(dc[r], pc[r]) for c, r in rows.items()}, H.rel_fro(dt, pt)
+
+
+def test_local_jacobian_consistent_with_residual():
+    """with the consistent drift linearisation the local matrix is the
+    derivative of the local residual (central differences, n- and p-side
+    cells)"""
+    for cell in (5, 6):
+        rows, trace = _fd_jacobian_mismatch(True, cell)
+        assert max(rows.values()) < 1e-8 and trace < 1e-8, (cell, rows, trace)
+
+
+def test_reference_drift_linearisation_is_inexact():
+    """the reference's volume drift term (template.h:500, 521-522) has the
+    opposite sign of the derivative of its residual (:656, :673): the n rows
+    are off on the n side, the p rows on the p side, nothing else"""
+    rows5, tr5 = _fd_jacobian_mismatch(False, 5)   # n-side cell
+    rows6, tr6 = _fd_jacobian_mismatch(False, 6)   # p-side cell
+    assert rows5[1] > 1e-3 and rows6[2] > 1e-3
+    assert rows5[0] < 1e-8 and rows6[0] < 1e-8 and tr5 < 1e-8 and tr6 < 1e-8
