@@ -177,8 +177,9 @@ def test_newton_diode_matches_oracle(dim, n_cells, k):
 
 
 def test_newton_reference_linearisation_matches_oracle():
-    """the reference's own drift linearisation (inexact Newton) on a lightly
-    doped diode where it converges: same iteration count, same state"""
+    """the reference's own drift linearisation (an inexact Newton that only
+    converges linearly) on a lightly doped diode: the same iterates, step by
+    step, for a fixed number of iterations"""
     _torch()
     print("case newton reference linearisation 2D n=8 k=1 doping 1e21", flush=True)
     s, prob, params, adim = _solver(2, 8, 1, doping=configs.step_doping(1e21, 1e21),
@@ -191,8 +192,10 @@ def test_newton_reference_linearisation_matches_oracle():
     configs.set_bias(s, 0.1)
     o = H.oracle_for(prob, params, adim)
     o.set_state(*s.get_state())
-    r_gpu = s.run()
-    r_orc = o.run_newton(max_it=100)
-    print(f"  iterations gpu={r_gpu.iterations} oracle={r_orc.iterations}")
-    assert r_gpu.converged and r_orc.converged and r_gpu.iterations == r_orc.iterations
+    ng, no = [], []
+    r_gpu = s.run(max_number_of_iterations=15, log=lambda st, u, c: ng.append(u))
+    r_orc = o.run_newton(max_it=15, log=lambda st, u, c: no.append(u))
+    print(f"  update norms gpu={ng[-1]:.3e} oracle={no[-1]:.3e} after {len(ng)} steps")
+    assert r_gpu.iterations == r_orc.iterations == 15
+    assert np.allclose(ng, no, rtol=1e-8, atol=0)
     assert _close(s, o)
