@@ -4,17 +4,22 @@
 //   lanes [0, NS)          scalar trial dofs   (NS = ne*(k+1)^2)
 //   lanes [NS, NS+T)       trace trial dofs    (T = 4*ne*(k+1))
 //   lane  NS+T             right-hand side (residual; Dirichlet update folded in)
-// Each lane keeps its whole column (NS+T rows) in VGPRs.  The column is
-// assembled from its "bundle" (values, lifted fluxes and traces of the trial
-// function at the quadrature points) with the test tables read as uniform
-// scalars, exactly the terms of include/assembling_methods.template.h:
-// 269-2118 after the flux unknowns are eliminated (hdg_tables.h).  Then NS
-// steps of Gaussian elimination with scaled column pivoting: equation k picks
-// the scalar unknown (lane) with the largest scaled |entry|, the pivot
-// column is broadcast through LDS, every lane updates its own column.  The
-// trace rows end as [S | g] and are written straight into the trace CSR
-// (two colour launches, deterministic).  Workgroups of 4 waves are
-// persistent; the shared tables are staged in LDS once per workgroup.
+// A lane keeps only the NS scalar-test rows of its column in VGPRs.  Its
+// column is assembled from its "bundle" (values, lifted fluxes and traces of
+// the trial function at the quadrature points) with the test tables read as
+// uniform scalars: the terms of include/assembling_methods.template.h:
+// 269-2118 after the flux unknowns are eliminated (hdg_tables.h).
+//   1. scalar-test rows [A' | CT' | c'] in registers
+//   2. trace-test rows of the scalar columns, TC', staged in LDS
+//   3. Gauss-Jordan on the scalar rows with scaled column pivoting: equation k
+//      picks the unknown (lane) with the largest scaled |entry|, the pivot
+//      column goes through LDS; afterwards row k of a trace/rhs lane holds
+//      X(p_k, m), X = A'^-1 [CT' | c']
+//   4. trace/rhs lanes form their trace-test rows TT', t' face by face and
+//      subtract TC' X: [S | g] = [TT' - TC' A'^-1 CT' | t' - TC' A'^-1 c'],
+//      written straight into the trace CSR (two colour launches,
+//      deterministic)
+// Workgroups of 4 waves are persistent; shared tables are staged in LDS once.
 #include "hdg_device.h"
 #include "hdg_kernels.h"
 
@@ -41,6 +46,7 @@ namespace hdg
       static constexpr int NS = NE * NB, TDF = NE * NF, T = NFC * TDF;
       static constexpr int NROW = NS + T, NCOL = NROW + 1;
       static_assert(NCOL <= 64, "one column per lane");
+      static constexpr int NROW2 = NS; // rows held in registers
       // workgroup tables (doubles)
       static constexpr int tPHIC  = 0;
       static constexpr int tLDC   = tPHIC + QC * NB;
@@ -55,7 +61,9 @@ namespace hdg
       static constexpr int tPMC   = tJXWF + NFC * QF;
       static constexpr int tPMF   = tPMC + QC * NB;
       static constexpr int tMFI   = tPMF + NFC * QF * NB;
-      static constexpr int TAB    = tMFI + 2 * NF * NF;
+      static constexpr int tBZ    = tMFI + 2 * NF * NF;              // (QC*3+NFC*QF)*NB
+      static constexpr int tBT    = tBZ + (QC * 3 + NFC * QF) * NB;  // NFC*QF*NF
+      static constexpr int TAB    = tBT + NFC * QF * NF;
       // per-wave scratch (doubles)
       static constexpr int wXS   = 0;
       static constexpr int wXT   = wXS + NCD;
@@ -74,7 +82,8 @@ namespace hdg
       static constexpr int wRESF = wRESC + QC * NE * 3;
       static constexpr int wNEU  = wRESF + NFC * QF * NE;
       static constexpr int wBC   = wNEU + NFC * NE * NF;
-      static constexpr int wBLK  = wBC + 64;      // 4 long long (blkptr of the faces)
+      static constexpr int wTCL  = wBC + 64;      // NS*T: TC' column by column
+      static constexpr int wBLK  = wTCL + NS * T; // 4 long long (blkptr of the faces)
       static constexpr int wFACE = wBLK + NFC;    // 4 long long (face index)
       static constexpr int WDBL  = wFACE + NFC;
       // per-wave ints
@@ -83,7 +92,8 @@ namespace hdg
       static constexpr int iBID  = iNEU + NFC * 3;
       static constexpr int iNBK  = iBID + NFC;
       static constexpr int iSLOT = iNBK + NFC;    // NFC*NFC
-      static constexpr int WINT  = iSLOT + NFC * NFC;
+      static constexpr int iPIV  = iSLOT + NFC * NFC; // NS pivot lanes
+      static constexpr int WINT  = iPIV + NS;
       static constexpr size_t bytes =
         (size_t)(TAB + WPB * WDBL) * sizeof(double) + (size_t)WPB * WINT * sizeof(int);
     };
@@ -97,7 +107,7 @@ namespace hdg
       constexpr int NS     = S::NS, TDF = S::TDF, T = S::T, NROW = S::NROW;
       constexpr int K1     = S::K1;
       extern __shared__ double smem[];
-      double       *tab  = smem;
+      double       *tab0 = smem;
       const int     tid  = threadIdx.x;
       const int     lane = tid & 63, wv = tid >> 6;
       double       *ws   = smem + S::TAB + wv * S::WDBL;
@@ -109,36 +119,38 @@ namespace hdg
       // ---- stage the shared tables once per workgroup
       for (int i = tid; i < QC * NB; i += 64 * WPB)
         {
-          tab[S::tPHIC + i] = tb.phic[i];
-          tab[S::tPMC + i]  = tb.PhiMinvC[i];
+          tab0[S::tPHIC + i] = tb.phic[i];
+          tab0[S::tPMC + i]  = tb.PhiMinvC[i];
         }
       for (int i = tid; i < QC * 2 * NB; i += 64 * WPB)
         {
-          tab[S::tLDC + i]   = tb.LDc[i];
-          tab[S::tGRADC + i] = tb.gradc[i];
+          tab0[S::tLDC + i]   = tb.LDc[i];
+          tab0[S::tGRADC + i] = tb.gradc[i];
         }
       for (int i = tid; i < NFC * QC * NF; i += 64 * WPB)
-        tab[S::tLCC + i] = tb.LCc[i];
+        tab0[S::tLCC + i] = tb.LCc[i];
       for (int i = tid; i < NFC * QF * NB; i += 64 * WPB)
         {
-          tab[S::tPHIF + i] = tb.phif[i];
-          tab[S::tLDF + i]  = tb.LDf[i];
-          tab[S::tPMF + i]  = tb.PhiMinvF[i];
+          tab0[S::tPHIF + i] = tb.phif[i];
+          tab0[S::tLDF + i]  = tb.LDf[i];
+          tab0[S::tPMF + i]  = tb.PhiMinvF[i];
         }
       for (int i = tid; i < NFC * NFC * QF * NF; i += 64 * WPB)
-        tab[S::tLCF + i] = tb.LCf[i];
+        tab0[S::tLCF + i] = tb.LCf[i];
       for (int i = tid; i < QF * NF; i += 64 * WPB)
-        tab[S::tXI + i] = tb.xi[i];
+        tab0[S::tXI + i] = tb.xi[i];
       for (int i = tid; i < QC; i += 64 * WPB)
-        tab[S::tJXWC + i] = tb.JxWc[i];
+        tab0[S::tJXWC + i] = tb.JxWc[i];
       for (int i = tid; i < NFC * QF; i += 64 * WPB)
-        tab[S::tJXWF + i] = tb.JxWf[i];
+        tab0[S::tJXWF + i] = tb.JxWf[i];
       for (int i = tid; i < 2 * NF * NF; i += 64 * WPB)
-        tab[S::tMFI + i] = tb.Mfinv[i];
+        tab0[S::tMFI + i] = tb.Mfinv[i];
+      for (int i = tid; i < (QC * 3 + NFC * QF) * NB; i += 64 * WPB)
+        tab0[S::tBZ + i] = tb.Bz[i];
+      for (int i = tid; i < NFC * QF * NF; i += 64 * WPB)
+        tab0[S::tBT + i] = tb.Bt[i];
       __syncthreads();
 
-      const double *__restrict__ Bz = tb.Bz;
-      const double *__restrict__ Bt = tb.Bt;
       double *u0 = ws + S::wU0, *W0 = ws + S::wW0, *u0f = ws + S::wU0F, *W0f = ws + S::wW0F;
       double *tr0 = ws + S::wTR0, *qd = ws + S::wQD, *rhsq = ws + S::wRHSQ;
       double *Wrc = ws + S::wWRC, *Wrf = ws + S::wWRF, *lamf = ws + S::wLAMF;
@@ -174,6 +186,12 @@ namespace hdg
       for (int idx = blockIdx.x * WPB + wv; idx < ncell; idx += gridDim.x * WPB)
         {
           const long long cell = cells[idx];
+          // the tables are loop invariant; hiding their addresses behind an
+          // empty asm keeps the compiler from hoisting every table load out
+          // of the cell loop (hundreds of live values, all spilled)
+          double *tab = tab0;
+          asm volatile("" : "+v"(tab));
+          const double *Bz = tab + S::tBZ, *Bt = tab + S::tBT;
           int             ic[2];
           ic[0] = (int)(cell % p.n[0]);
           ic[1] = (int)(cell / p.n[0]);
@@ -410,15 +428,70 @@ namespace hdg
             }
           WAVE_SYNC();
 
-          // ---- E: this lane's column of the reduced system, in registers
-          double R[NROW];
-#pragma unroll
-          for (int r = 0; r < NROW; ++r)
-            R[r] = 0.0;
+          // ---- E: bundle of this lane's column and the forms it feeds.
+          // Branch-free: every table value is loaded (indices valid for all
+          // lanes) and blended with the lane's 0/1 weights, so no per-lane
+          // masks have to be kept alive.
           const bool   isU = kind == 0, isL = kind == 1, isR = kind == 2;
           const double isr = isR ? 1.0 : 0.0;
           const int    ju = isU ? jj : 0, jl = isL ? jj : 0, fl = isL ? ff : 0;
+          double       mU[NE], mL[NE];
 #pragma unroll
+          for (int ci = 0; ci < NE; ++ci)
+            {
+              mU[ci] = (isU && cc == ci) ? 1.0 : 0.0;
+              mL[ci] = (isL && cc == ci) ? 1.0 : 0.0;
+            }
+          const double mLx = (fl >> 1) == 0 ? 1.0 : 0.0, mLy = 1.0 - mLx;
+          double      *tcl  = ws + S::wTCL;
+          int         *pivs = wi + S::iPIV;
+          // numerical flux F_c of the column at face qp (g, s), c = ci-th
+          // enabled component: the HDG flux of component c (use_cell)
+          auto face_flux = [&](int g, int s, int ci) -> double {
+            const int    fi = g * QF + s;
+            const double ph = tab[S::tPHIF + fi * NB + ju];
+            const double ld = tab[S::tLDF + fi * NB + ju];
+            const double lc = tab[S::tLCF + ((fl * NFC + g) * QF + s) * NF + jl];
+            const double xv = tab[S::tXI + s * NF + jl] * (fl == g ? 1.0 : 0.0);
+            const double E0n = W0f[fi];
+            auto uu = [&](int c2) { return mU[c2] * ph; };
+            auto WN = [&](int c2) { return mU[c2] * ld + mL[c2] * lc - isr * Wrf[c2 * NFC * QF + fi]; };
+            auto LL = [&](int c2) { return mL[c2] * xv - isr * lamf[c2 * NFC * QF + fi]; };
+            double Fv;
+            if (ci == 0)
+              Fv = eps * WN(0) + tV * uu(0) - tV * LL(0);
+            else if (ci == 1)
+              {
+                const double n0 = u0f[NFC * QF + fi];
+                Fv = uu(1) * mun * E0n + n0 * mun * WN(0) - UT * mun * WN(1) - tN * uu(1) +
+                     tN * LL(1);
+              }
+            else
+              {
+                const double p0 = u0f[2 * NFC * QF + fi];
+                Fv = -(uu(2) * mup * E0n + p0 * mup * WN(0)) - UT * mup * WN(2) - tP * uu(2) +
+                     tP * LL(2);
+              }
+            return Fv + isr * resf[fi * NE + ci];
+          };
+          // trace-test row (face g, component ci, basis i); zero on Dirichlet
+          auto trace_row = [&](int g, int ci, int i) -> double {
+            const int comp = NE == 1 ? 0 : ci;
+            double    v    = isr * neu[(g * NE + ci) * NF + i];
+#pragma unroll
+            for (int s = 0; s < QF; ++s)
+              v += Bt[(g * QF + s) * NF + i] * face_flux(g, s, ci);
+            return isdir[g * 3 + comp] ? 0.0 : v;
+          };
+
+          // ---- 1: scalar-test rows in registers
+          double R[NS];
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            R[r] = 0.0;
+          // q stays a runtime index: unrolled, the scheduler hoists every
+          // (uniform) Bz load of all quadrature points at once
+#pragma unroll 1
           for (int q = 0; q < QC; ++q)
             {
               double uu[3] = {0, 0, 0}, WX[3] = {0, 0, 0}, WY[3] = {0, 0, 0};
@@ -429,14 +502,9 @@ namespace hdg
 #pragma unroll
               for (int ci = 0; ci < NE; ++ci)
                 {
-                  const bool mine = cc == ci;
-                  uu[ci] = (isU && mine) ? ph : 0.0;
-                  WX[ci] = (isU && mine) ? lx
-                         : (isL && mine && (fl >> 1) == 0) ? lc
-                         : isR ? -Wrc[(ci * 2 + 0) * QC + q] : 0.0;
-                  WY[ci] = (isU && mine) ? ly
-                         : (isL && mine && (fl >> 1) == 1) ? lc
-                         : isR ? -Wrc[(ci * 2 + 1) * QC + q] : 0.0;
+                  uu[ci] = mU[ci] * ph;
+                  WX[ci] = mU[ci] * lx + mL[ci] * mLx * lc - isr * Wrc[(ci * 2 + 0) * QC + q];
+                  WY[ci] = mU[ci] * ly + mL[ci] * mLy * lc - isr * Wrc[(ci * 2 + 1) * QC + q];
                 }
               const double E0x = W0[q], E0y = W0[QC + q];
 #pragma unroll
@@ -478,161 +546,146 @@ namespace hdg
                                       Bz[(q * 3 + 2) * NB + i] * by;
                 }
             }
+          // face terms: the face rows of Bz vanish (exactly) off the face's
+          // nodes, so all NB rows are swept and R keeps static indices
+#pragma unroll 1
+          for (int fs = 0; fs < NFC * QF; ++fs)
+            {
+              const int g = fs / QF, s = fs % QF;
 #pragma unroll
-          for (int g = 0; g < NFC; ++g)
+              for (int ci = 0; ci < NE; ++ci)
+                {
+                  const double Fv = face_flux(g, s, ci);
 #pragma unroll
-            for (int s = 0; s < QF; ++s)
-              {
-                double uu[3] = {0, 0, 0}, WN[3] = {0, 0, 0}, LL[3] = {0, 0, 0};
-                const int    fi = g * QF + s;
-                const double ph = tab[S::tPHIF + fi * NB + ju];
-                const double ld = tab[S::tLDF + fi * NB + ju];
-                const double lc = tab[S::tLCF + ((fl * NFC + g) * QF + s) * NF + jl];
-                const double xv = tab[S::tXI + s * NF + jl];
-#pragma unroll
-                for (int ci = 0; ci < NE; ++ci)
-                  {
-                    const bool mine = cc == ci;
-                    uu[ci] = (isU && mine) ? ph : 0.0;
-                    WN[ci] = (isU && mine) ? ld : (isL && mine) ? lc
-                           : isR ? -Wrf[ci * NFC * QF + fi] : 0.0;
-                    LL[ci] = (isL && mine && fl == g) ? xv : isR ? -lamf[ci * NFC * QF + fi] : 0.0;
-                  }
-                const double E0n = W0f[fi];
-#pragma unroll
-                for (int ci = 0; ci < NE; ++ci)
-                  {
-                    double Fv;
-                    if (ci == 0)
-                      Fv = eps * WN[0] + tV * uu[0] - tV * LL[0];
-                    else if (ci == 1)
-                      {
-                        const double n0 = u0f[NFC * QF + fi];
-                        Fv = uu[1] * mun * E0n + n0 * mun * WN[0] - UT * mun * WN[1] -
-                             tN * uu[1] + tN * LL[1];
-                      }
-                    else
-                      {
-                        const int    c2 = NE > 2 ? 2 : 1;
-                        const double p0 = u0f[2 * NFC * QF + fi];
-                        Fv = -(uu[c2] * mup * E0n + p0 * mup * WN[0]) - UT * mup * WN[c2] -
-                             tP * uu[c2] + tP * LL[c2];
-                      }
-                    Fv += isr * resf[fi * NE + ci];
-                    // z rows with support on face g
-                    const int a = g >> 1, side = g & 1;
-#pragma unroll
-                    for (int t = 0; t < K1; ++t)
-                      {
-                        const int i = a == 0 ? side * DEG + K1 * t : t + K1 * side * DEG;
-                        R[ci * NB + i] += Bz[(QC * 3 + fi) * NB + i] * Fv;
-                      }
-                    const int comp = NE == 1 ? 0 : ci;
-                    if (!isdir[g * 3 + comp])
-#pragma unroll
-                      for (int i = 0; i < NF; ++i)
-                        R[NS + g * TDF + ci * NF + i] += Bt[fi * NF + i] * Fv;
-                  }
-              }
-#pragma unroll
-          for (int g = 0; g < NFC; ++g)
-#pragma unroll
-            for (int ci = 0; ci < NE; ++ci)
-#pragma unroll
-              for (int i = 0; i < NF; ++i)
-                R[NS + g * TDF + ci * NF + i] += isr * neu[(g * NE + ci) * NF + i];
+                  for (int i = 0; i < NB; ++i)
+                    R[ci * NB + i] += Bz[(QC * 3 + fs) * NB + i] * Fv;
+                }
+            }
           // LocalCondenser::condense_ct_matrix: the Dirichlet update is in the
           // rhs bundle already; the Dirichlet columns are dropped
-          if (isL && isdir[ff * 3 + (NE == 1 ? 0 : cc)])
+          const bool   dircol = isL && isdir[ff * 3 + (NE == 1 ? 0 : cc)];
+          const double keep   = dircol ? 0.0 : 1.0;
 #pragma unroll
-            for (int r = 0; r < NROW; ++r)
-              R[r] = 0.0;
+          for (int r = 0; r < NS; ++r)
+            R[r] *= keep;
 
-          // ---- F: elimination, scaled column pivoting
+          // ---- 2: TC' (trace-test rows of the scalar columns) into LDS
+          if (isU)
+            for (int g = 0; g < NFC; ++g)
+#pragma unroll
+              for (int ci = 0; ci < NE; ++ci)
+#pragma unroll
+                for (int i = 0; i < NF; ++i)
+                  tcl[lane * T + g * TDF + ci * NF + i] = trace_row(g, ci, i);
+
+          // ---- 3: Gauss-Jordan, scaled column pivoting
           double cs = 0;
 #pragma unroll
           for (int r = 0; r < NS; ++r)
             cs = fmax(cs, fabs(R[r]));
-          cs            = cs > 0 ? 1.0 / cs : 0.0;
+          cs           = cs > 0 ? 1.0 / cs : 0.0;
           bool pivoted = !isU;
 #pragma unroll
           for (int k = 0; k < NS; ++k)
             {
-              double best = pivoted ? -1.0 : fabs(R[k]) * cs;
-              int    bl   = lane;
+              // order-preserving key: non-negative doubles compare as integers
+              const double       sc  = pivoted ? 0.0 : fabs(R[k]) * cs;
+              unsigned long long key = pivoted ? 0ull
+                                               : (((unsigned long long)__double_as_longlong(sc) &
+                                                   ~63ull) | (unsigned long long)(63 - lane)) + 1ull;
 #pragma unroll
               for (int off = 32; off > 0; off >>= 1)
                 {
-                  const double ob = __shfl_xor(best, off);
-                  const int    ol = __shfl_xor(bl, off);
-                  if (ob > best || (ob == best && ol < bl))
-                    {
-                      best = ob;
-                      bl   = ol;
-                    }
+                  const unsigned long long o = __shfl_xor(key, off);
+                  key                        = o > key ? o : key;
                 }
-              const int pl = __builtin_amdgcn_readfirstlane(bl);
+              const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane((unsigned)(key - 1ull))) & 63u);
               if (lane == pl)
                 {
                   pivoted = true;
 #pragma unroll
-                  for (int r = k; r < NROW; ++r)
+                  for (int r = 0; r < NS; ++r)
                     bc[r] = R[r];
                 }
+              if (lane == 0)
+                pivs[k] = pl;
               WAVE_SYNC();
               const double f = R[k] / bc[k];
 #pragma unroll
-              for (int r = k + 1; r < NROW; ++r)
-                R[r] -= bc[r] * f;
+              for (int r = 0; r < NS; ++r)
+                if (r != k)
+                  R[r] -= bc[r] * f;
+              R[k] = f;
               WAVE_SYNC();
             }
 
-          // ---- G: output
-          if (mode == 1)
+          // ---- 4: [S | g] face by face, output
+#pragma unroll 1
+          for (int g = 0; g < NFC; ++g)
             {
-              if (isL)
+              double acc[TDF];
 #pragma unroll
-                for (int i = 0; i < T; ++i)
-                  p.Sout[(long long)idx * T * T + i * T + (lane - NS)] = R[NS + i];
-              if (isR)
+              for (int ci = 0; ci < NE; ++ci)
 #pragma unroll
-                for (int i = 0; i < T; ++i)
-                  p.gout[(long long)idx * T + i] = R[NS + i];
-              continue;
-            }
-          if (isL)
-            {
-              const int lt = lane - NS, gj = lt / TDF, rj = lt % TDF;
+                for (int i = 0; i < NF; ++i)
+                  acc[ci * NF + i] = (isL || isR) ? trace_row(g, ci, i) : 0.0;
+              if (isL || isR)
 #pragma unroll
-              for (int i = 0; i < T; ++i)
+                for (int k = 0; k < NS; ++k)
+                  {
+                    const double *tc = tcl + pivs[k] * T + g * TDF;
+                    const double  x  = R[k];
+#pragma unroll
+                    for (int r = 0; r < TDF; ++r)
+                      acc[r] -= tc[r] * x;
+                  }
+              if (mode == 1)
                 {
-                  const int  gi = i / TDF, ri = i % TDF, ci = ri / NF;
-                  const bool dir = isdir[gi * 3 + (NE == 1 ? 0 : ci)];
-                  const double v = dir ? (i == lt ? 1.0 : 0.0) : R[NS + i];
-                  const long long pos = blk[gi] * TDF * TDF + (long long)ri * nbk[gi] * TDF +
-                                        (long long)slot[gi * NFC + gj] * TDF + rj;
-                  if (color == 1 && gi == gj && bidv[gi] < 0)
-                    p.vals[pos] += v;
-                  else
-                    p.vals[pos] = v;
+                  if (isL)
+#pragma unroll
+                    for (int r = 0; r < TDF; ++r)
+                      p.Sout[(long long)idx * T * T + (g * TDF + r) * T + (lane - NS)] =
+                        dircol ? 0.0 : acc[r];
+                  if (isR)
+#pragma unroll
+                    for (int r = 0; r < TDF; ++r)
+                      p.gout[(long long)idx * T + g * TDF + r] = acc[r];
+                  continue;
                 }
-            }
-          if (isR)
-            {
-#pragma unroll
-              for (int i = 0; i < T; ++i)
+              if (isL)
                 {
-                  const int  gi = i / TDF, ri = i % TDF, ci = ri / NF;
-                  const int  comp = NE == 1 ? 0 : ci;
-                  const bool dir  = isdir[gi * 3 + comp];
-                  const long long row = facev[gi] * TDF + ri;
-                  const double v = dir ? 0.0 : R[NS + i];
-                  if (color == 1 && bidv[gi] < 0)
-                    p.rhs[row] += v;
-                  else
-                    p.rhs[row] = v;
-                  if (dir)
-                    p.dirout[row] = lupd[(gi * 3 + comp) * NF + (ri % NF)];
+                  const int lt = lane - NS, gj = lt / TDF, rj = lt % TDF;
+#pragma unroll
+                  for (int r = 0; r < TDF; ++r)
+                    {
+                      const int  i   = g * TDF + r, ci = r / NF;
+                      const bool dir = isdir[g * 3 + (NE == 1 ? 0 : ci)];
+                      const double v = dir ? (i == lt ? 1.0 : 0.0) : (dircol ? 0.0 : acc[r]);
+                      const long long pos = blk[g] * TDF * TDF + (long long)r * nbk[g] * TDF +
+                                            (long long)slot[g * NFC + gj] * TDF + rj;
+                      if (color == 1 && g == gj && bidv[g] < 0)
+                        p.vals[pos] += v;
+                      else
+                        p.vals[pos] = v;
+                    }
+                }
+              if (isR)
+                {
+#pragma unroll
+                  for (int r = 0; r < TDF; ++r)
+                    {
+                      const int  ci   = r / NF;
+                      const int  comp = NE == 1 ? 0 : ci;
+                      const bool dir  = isdir[g * 3 + comp];
+                      const long long row = facev[g] * TDF + r;
+                      const double v = dir ? 0.0 : acc[r];
+                      if (color == 1 && bidv[g] < 0)
+                        p.rhs[row] += v;
+                      else
+                        p.rhs[row] = v;
+                      if (dir)
+                        p.dirout[row] = lupd[(g * 3 + comp) * NF + (r % NF)];
+                    }
                 }
             }
           WAVE_SYNC();
