@@ -224,3 +224,32 @@ def test_reference_drift_linearisation_is_inexact():
     rows6, tr6 = _fd_jacobian_mismatch(False, 6)   # p-side cell
     assert rows5[1] > 1e-3 and rows6[2] > 1e-3
     assert rows5[0] < 1e-8 and rows6[0] < 1e-8 and tr5 < 1e-8 and tr6 < 1e-8
+
+
+def test_equilibrium_jacobian_consistent_with_residual():
+    """thermodynamic-equilibrium operator (V only, Boltzmann n and p,
+    template.h:442-459, 677-693): the local matrix is the derivative of the
+    local residual (central differences, an n-side and a p-side cell)"""
+    prob, params, adim = configs.diode(2, 4, 1)
+    o = H.oracle_for(prob, params, adim, (True, False, False))
+    c0, t0 = H.perturbed(*H.equilibrium_guess(prob, adim, 1), seed=8)
+    nb, NC, h = o.nb, o.NC, 1e-6
+    for cell in (5, 6):
+        o.set_state(c0, t0)
+        A, CT, TC, TT, cc, t = o.local_system(cell, True)
+        _, _, dofs, _, _ = o.condensed_cell(cell, True)
+        dofs = o.restricted_to_full((True, False, False))[dofs]
+        rng = np.random.default_rng(cell)
+        vu, vl = rng.standard_normal(A.shape[0]), rng.standard_normal(TT.shape[0])
+
+        def resid(e):
+            ce, te = c0.copy(), t0.copy()
+            ce[cell * NC:cell * NC + 3 * nb] += e * vu
+            te[dofs] += e * vl
+            o.set_state(ce, te)
+            out = o.local_system(cell, True)
+            return out[4], out[5]
+        (cp, tp), (cm, tm) = resid(h), resid(-h)
+        dc, dt = (cp - cm) / (2 * h), (tp - tm) / (2 * h)
+        assert H.rel_fro(dc, -(A @ vu + CT @ vl)) < 1e-8
+        assert H.rel_fro(dt, -(-TC @ vu + TT @ vl)) < 1e-8
