@@ -67,6 +67,10 @@ def main():
     ap.add_argument("--bias", type=float, default=0.3)
     ap.add_argument("--cpu-sample", type=int, default=96)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB "
+                         "(3 launches): the known-size kernel that calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the per-cell path has no multi-GPU form in this "
@@ -75,6 +79,17 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8,
+                          "launches": 3}), flush=True)
+        return
 
     from ddhdg_amd import configs
     from ddhdg_amd.np_solver import NPSolver

@@ -37,6 +37,9 @@ namespace hdg
   namespace
   {
     constexpr int WPB = 4; // waves per workgroup
+#ifndef WPE
+#define WPE 2
+#endif
 
     template <int DEG, int NE>
     struct F2
@@ -99,7 +102,7 @@ namespace hdg
     };
 
     template <int DEG, int NE, bool THERMO>
-    __global__ void __launch_bounds__(64 * WPB)
+    __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(WPE)))
       fast_condense(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
     {
       using S              = F2<DEG, NE>;
@@ -445,52 +448,14 @@ namespace hdg
           const double mLx = (fl >> 1) == 0 ? 1.0 : 0.0, mLy = 1.0 - mLx;
           double      *tcl  = ws + S::wTCL;
           int         *pivs = wi + S::iPIV;
-          // numerical flux F_c of the column at face qp (g, s), c = ci-th
-          // enabled component: the HDG flux of component c (use_cell)
-          auto face_flux = [&](int g, int s, int ci) -> double {
-            const int    fi = g * QF + s;
-            const double ph = tab[S::tPHIF + fi * NB + ju];
-            const double ld = tab[S::tLDF + fi * NB + ju];
-            const double lc = tab[S::tLCF + ((fl * NFC + g) * QF + s) * NF + jl];
-            const double xv = tab[S::tXI + s * NF + jl] * (fl == g ? 1.0 : 0.0);
-            const double E0n = W0f[fi];
-            auto uu = [&](int c2) { return mU[c2] * ph; };
-            auto WN = [&](int c2) { return mU[c2] * ld + mL[c2] * lc - isr * Wrf[c2 * NFC * QF + fi]; };
-            auto LL = [&](int c2) { return mL[c2] * xv - isr * lamf[c2 * NFC * QF + fi]; };
-            double Fv;
-            if (ci == 0)
-              Fv = eps * WN(0) + tV * uu(0) - tV * LL(0);
-            else if (ci == 1)
-              {
-                const double n0 = u0f[NFC * QF + fi];
-                Fv = uu(1) * mun * E0n + n0 * mun * WN(0) - UT * mun * WN(1) - tN * uu(1) +
-                     tN * LL(1);
-              }
-            else
-              {
-                const double p0 = u0f[2 * NFC * QF + fi];
-                Fv = -(uu(2) * mup * E0n + p0 * mup * WN(0)) - UT * mup * WN(2) - tP * uu(2) +
-                     tP * LL(2);
-              }
-            return Fv + isr * resf[fi * NE + ci];
-          };
-          // trace-test row (face g, component ci, basis i); zero on Dirichlet
-          auto trace_row = [&](int g, int ci, int i) -> double {
-            const int comp = NE == 1 ? 0 : ci;
-            double    v    = isr * neu[(g * NE + ci) * NF + i];
-#pragma unroll
-            for (int s = 0; s < QF; ++s)
-              v += Bt[(g * QF + s) * NF + i] * face_flux(g, s, ci);
-            return isdir[g * 3 + comp] ? 0.0 : v;
-          };
 
-          // ---- 1: scalar-test rows in registers
+          // ---- 1a: scalar-test rows, cell quadrature points.  q stays a
+          // runtime index: unrolled, the scheduler hoists every (uniform) Bz
+          // load of all quadrature points at once
           double R[NS];
 #pragma unroll
           for (int r = 0; r < NS; ++r)
             R[r] = 0.0;
-          // q stays a runtime index: unrolled, the scheduler hoists every
-          // (uniform) Bz load of all quadrature points at once
 #pragma unroll 1
           for (int q = 0; q < QC; ++q)
             {
@@ -546,20 +511,101 @@ namespace hdg
                                       Bz[(q * 3 + 2) * NB + i] * by;
                 }
             }
-          // face terms: the face rows of Bz vanish (exactly) off the face's
-          // nodes, so all NB rows are swept and R keeps static indices
+
+          // ---- 1b: face quadrature points.  The numerical flux F_c of the
+          // column (the HDG flux of component c, use_cell) at face point
+          // (g, s); it feeds the scalar-test rows (phi_i F_c) and the
+          // trace-test rows (xi_i F_c).
+          auto face_flux = [&](int g, int s, double (&F)[NE]) {
+            const int    fi  = g * QF + s;
+            const double ph  = tab[S::tPHIF + fi * NB + ju];
+            const double ld  = tab[S::tLDF + fi * NB + ju];
+            const double lc  = tab[S::tLCF + ((fl * NFC + g) * QF + s) * NF + jl];
+            const double xv  = tab[S::tXI + s * NF + jl] * (fl == g ? 1.0 : 0.0);
+            const double E0n = W0f[fi];
+            const double WN0 = mU[0] * ld + mL[0] * lc - isr * Wrf[fi];
+#pragma unroll
+            for (int ci = 0; ci < NE; ++ci)
+              {
+                const double uuc = mU[ci] * ph;
+                const double WNc = mU[ci] * ld + mL[ci] * lc - isr * Wrf[ci * NFC * QF + fi];
+                const double LLc = mL[ci] * xv - isr * lamf[ci * NFC * QF + fi];
+                double       Fv;
+                if (ci == 0)
+                  Fv = eps * WNc + tV * uuc - tV * LLc;
+                else if (ci == 1)
+                  Fv = uuc * mun * E0n + u0f[NFC * QF + fi] * mun * WN0 - UT * mun * WNc -
+                       tN * uuc + tN * LLc;
+                else
+                  Fv = -(uuc * mup * E0n + u0f[2 * NFC * QF + fi] * mup * WN0) -
+                       UT * mup * WNc - tP * uuc + tP * LLc;
+                F[ci] = Fv + isr * resf[fi * NE + ci];
+              }
+          };
+          // trace-test rows of face g: xi_i F_c summed over the face points,
+          // Neumann data on the rhs lane, zero rows on Dirichlet faces
+          auto trace_rows = [&](int g, double (&acc)[TDF]) {
+#pragma unroll
+            for (int r = 0; r < TDF; ++r)
+              acc[r] = 0.0;
 #pragma unroll 1
-          for (int fs = 0; fs < NFC * QF; ++fs)
+            for (int s = 0; s < QF; ++s)
+              {
+                double F[NE];
+                face_flux(g, s, F);
+#pragma unroll
+                for (int ci = 0; ci < NE; ++ci)
+#pragma unroll
+                  for (int i = 0; i < NF; ++i)
+                    acc[ci * NF + i] += Bt[(g * QF + s) * NF + i] * F[ci];
+              }
+#pragma unroll
+            for (int ci = 0; ci < NE; ++ci)
+              {
+                const double keepr = isdir[g * 3 + (NE == 1 ? 0 : ci)] ? 0.0 : 1.0;
+#pragma unroll
+                for (int i = 0; i < NF; ++i)
+                  acc[ci * NF + i] = keepr * (acc[ci * NF + i] + isr * neu[(g * NE + ci) * NF + i]);
+              }
+          };
+#pragma unroll
+          for (int g = 0; g < NFC; ++g)
             {
-              const int g = fs / QF, s = fs % QF;
+              const int a = g >> 1, side = g & 1;
+              double    acc[TDF];
 #pragma unroll
-              for (int ci = 0; ci < NE; ++ci)
+              for (int r = 0; r < TDF; ++r)
+                acc[r] = 0.0;
+#pragma unroll 1
+              for (int s = 0; s < QF; ++s)
                 {
-                  const double Fv = face_flux(g, s, ci);
+                  const int fi = g * QF + s;
+                  double    F[NE];
+                  face_flux(g, s, F);
 #pragma unroll
-                  for (int i = 0; i < NB; ++i)
-                    R[ci * NB + i] += Bz[(QC * 3 + fs) * NB + i] * Fv;
+                  for (int ci = 0; ci < NE; ++ci)
+                    {
+#pragma unroll
+                      for (int t = 0; t < K1; ++t)
+                        {
+                          const int i = a == 0 ? side * DEG + K1 * t : t + K1 * side * DEG;
+                          R[ci * NB + i] += Bz[(QC * 3 + fi) * NB + i] * F[ci];
+                        }
+#pragma unroll
+                      for (int i = 0; i < NF; ++i)
+                        acc[ci * NF + i] += Bt[fi * NF + i] * F[ci];
+                    }
                 }
+              // ---- 2: TC' (trace-test rows of the scalar columns) into LDS
+              if (isU)
+#pragma unroll
+                for (int ci = 0; ci < NE; ++ci)
+                  {
+                    const double keepr = isdir[g * 3 + (NE == 1 ? 0 : ci)] ? 0.0 : 1.0;
+#pragma unroll
+                    for (int i = 0; i < NF; ++i)
+                      tcl[lane * T + g * TDF + ci * NF + i] = keepr * acc[ci * NF + i];
+                  }
             }
           // LocalCondenser::condense_ct_matrix: the Dirichlet update is in the
           // rhs bundle already; the Dirichlet columns are dropped
@@ -568,15 +614,6 @@ namespace hdg
 #pragma unroll
           for (int r = 0; r < NS; ++r)
             R[r] *= keep;
-
-          // ---- 2: TC' (trace-test rows of the scalar columns) into LDS
-          if (isU)
-            for (int g = 0; g < NFC; ++g)
-#pragma unroll
-              for (int ci = 0; ci < NE; ++ci)
-#pragma unroll
-                for (int i = 0; i < NF; ++i)
-                  tcl[lane * T + g * TDF + ci * NF + i] = trace_row(g, ci, i);
 
           // ---- 3: Gauss-Jordan, scaled column pivoting
           double cs = 0;
@@ -619,26 +656,21 @@ namespace hdg
               WAVE_SYNC();
             }
 
-          // ---- 4: [S | g] face by face, output
+          // ---- 4: [S | g] = [TT' | t'] - TC' X, face by face
 #pragma unroll 1
           for (int g = 0; g < NFC; ++g)
             {
               double acc[TDF];
+              trace_rows(g, acc);
 #pragma unroll
-              for (int ci = 0; ci < NE; ++ci)
+              for (int k = 0; k < NS; ++k)
+                {
+                  const double *tc = tcl + pivs[k] * T + g * TDF;
+                  const double  x  = R[k];
 #pragma unroll
-                for (int i = 0; i < NF; ++i)
-                  acc[ci * NF + i] = (isL || isR) ? trace_row(g, ci, i) : 0.0;
-              if (isL || isR)
-#pragma unroll
-                for (int k = 0; k < NS; ++k)
-                  {
-                    const double *tc = tcl + pivs[k] * T + g * TDF;
-                    const double  x  = R[k];
-#pragma unroll
-                    for (int r = 0; r < TDF; ++r)
-                      acc[r] -= tc[r] * x;
-                  }
+                  for (int r = 0; r < TDF; ++r)
+                    acc[r] -= tc[r] * x;
+                }
               if (mode == 1)
                 {
                   if (isL)
@@ -655,15 +687,17 @@ namespace hdg
               if (isL)
                 {
                   const int lt = lane - NS, gj = lt / TDF, rj = lt % TDF;
+                  const long long base = blk[g] * TDF * TDF + (long long)slot[g * NFC + gj] * TDF + rj;
+                  const long long rs   = (long long)nbk[g] * TDF;
+                  const bool      addd = color == 1 && g == gj && bidv[g] < 0;
 #pragma unroll
                   for (int r = 0; r < TDF; ++r)
                     {
                       const int  i   = g * TDF + r, ci = r / NF;
                       const bool dir = isdir[g * 3 + (NE == 1 ? 0 : ci)];
                       const double v = dir ? (i == lt ? 1.0 : 0.0) : (dircol ? 0.0 : acc[r]);
-                      const long long pos = blk[g] * TDF * TDF + (long long)r * nbk[g] * TDF +
-                                            (long long)slot[g * NFC + gj] * TDF + rj;
-                      if (color == 1 && g == gj && bidv[g] < 0)
+                      const long long pos = base + r * rs;
+                      if (addd)
                         p.vals[pos] += v;
                       else
                         p.vals[pos] = v;
@@ -671,6 +705,7 @@ namespace hdg
                 }
               if (isR)
                 {
+                  const bool addr = color == 1 && bidv[g] < 0;
 #pragma unroll
                   for (int r = 0; r < TDF; ++r)
                     {
@@ -679,7 +714,7 @@ namespace hdg
                       const bool dir  = isdir[g * 3 + comp];
                       const long long row = facev[g] * TDF + r;
                       const double v = dir ? 0.0 : acc[r];
-                      if (color == 1 && bidv[g] < 0)
+                      if (addr)
                         p.rhs[row] += v;
                       else
                         p.rhs[row] = v;
