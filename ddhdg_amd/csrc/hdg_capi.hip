@@ -557,7 +557,9 @@ int hdg_reconstruct(hdg_ctx *C, const double *x_cell, const double *x_trace,
   p.xt  = x_trace;
   p.dxt = dx_trace;
   p.dxc = dx_cell;
-  hipError_t e = hdg::launch_reconstruct(C->d, C->k, p, (int)C->ncells, (hipStream_t)stream);
+  hipError_t e = use_fast(C, thermodyn)
+                   ? hdg::launch_fast_reconstruct(C->k, p, (int)C->ncells, (hipStream_t)stream)
+                   : hdg::launch_reconstruct(C->d, C->k, p, (int)C->ncells, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_reconstruct launch");
 }
 

@@ -40,6 +40,9 @@ namespace hdg
 #ifndef WPE
 #define WPE 2
 #endif
+#ifndef WPE2
+#define WPE2 2
+#endif
 
     template <int DEG, int NE>
     struct F2
@@ -85,10 +88,11 @@ namespace hdg
       static constexpr int wRESF = wRESC + QC * NE * 3;
       static constexpr int wNEU  = wRESF + NFC * QF * NE;
       static constexpr int wBC   = wNEU + NFC * NE * NF;
-      static constexpr int wTCL  = wBC + 64;      // NS*T: TC' column by column
-      static constexpr int wBLK  = wTCL + NS * T; // 4 long long (blkptr of the faces)
+      static constexpr int wBLK  = wBC + 64;      // 4 long long (blkptr of the faces)
       static constexpr int wFACE = wBLK + NFC;    // 4 long long (face index)
-      static constexpr int WDBL  = wFACE + NFC;
+      static constexpr int wTCL  = wFACE + NFC;   // NS*T: TC' column by column (pass 1)
+      static constexpr int WDBL  = wTCL + NS * T; // per-wave doubles, pass 1
+      static constexpr int WDBL2 = wTCL;          // pass 2 needs no TC'
       // per-wave ints
       static constexpr int iDIR  = 0;             // NFC*3
       static constexpr int iNEU  = iDIR + NFC * 3;
@@ -97,15 +101,22 @@ namespace hdg
       static constexpr int iSLOT = iNBK + NFC;    // NFC*NFC
       static constexpr int iPIV  = iSLOT + NFC * NFC; // NS pivot lanes
       static constexpr int WINT  = iPIV + NS;
-      static constexpr size_t bytes =
-        (size_t)(TAB + WPB * WDBL) * sizeof(double) + (size_t)WPB * WINT * sizeof(int);
+      static constexpr size_t bytes_for(bool pass2)
+      {
+        return (size_t)(TAB + WPB * (pass2 ? WDBL2 : WDBL)) * sizeof(double) +
+               (size_t)WPB * WINT * sizeof(int);
+      }
     };
 
-    template <int DEG, int NE, bool THERMO>
-    __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(WPE)))
+    // PASS2: the reconstruction pass (cells = all cells in order, columns =
+    // scalar dofs + rhs with the trace update folded in, output dx_cell)
+    template <int DEG, int NE, bool THERMO, bool PASS2>
+    __global__ void __launch_bounds__(64 * WPB)
+      __attribute__((amdgpu_waves_per_eu(PASS2 ? WPE2 : WPE)))
       fast_condense(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
     {
       using S              = F2<DEG, NE>;
+      constexpr int WSTR   = PASS2 ? S::WDBL2 : S::WDBL;
       constexpr int NB     = S::NB, NF = S::NF, QC = S::QC, QF = S::QF, NFC = S::NFC;
       constexpr int NS     = S::NS, TDF = S::TDF, T = S::T, NROW = S::NROW;
       constexpr int K1     = S::K1;
@@ -113,8 +124,8 @@ namespace hdg
       double       *tab0 = smem;
       const int     tid  = threadIdx.x;
       const int     lane = tid & 63, wv = tid >> 6;
-      double       *ws   = smem + S::TAB + wv * S::WDBL;
-      int *wi = reinterpret_cast<int *>(smem + S::TAB + WPB * S::WDBL) + wv * S::WINT;
+      double       *ws   = smem + S::TAB + wv * WSTR;
+      int *wi = reinterpret_cast<int *>(smem + S::TAB + WPB * WSTR) + wv * S::WINT;
       long long *blk   = reinterpret_cast<long long *>(ws + S::wBLK);
       long long *facev = reinterpret_cast<long long *>(ws + S::wFACE);
       const DevTables &tb = p.tb;
@@ -188,7 +199,7 @@ namespace hdg
 
       for (int idx = blockIdx.x * WPB + wv; idx < ncell; idx += gridDim.x * WPB)
         {
-          const long long cell = cells[idx];
+          const long long cell = PASS2 ? (long long)idx : (long long)cells[idx];
           // the tables are loop invariant; hiding their addresses behind an
           // empty asm keeps the compiler from hoisting every table load out
           // of the cell loop (hundreds of live values, all spilled)
@@ -309,7 +320,9 @@ namespace hdg
             {
               const int g = e / (3 * NF), c = (e / NF) % 3, m = e % NF;
               double    v = 0;
-              if (!THERMO && isdir[g * 3 + c])
+              if (PASS2)
+                v = p.dxt[facev[g] * 3 * NF + c * NF + m];
+              else if (!THERMO && isdir[g * 3 + c])
                 {
                   const int a = g >> 1;
 #pragma unroll
@@ -591,13 +604,14 @@ namespace hdg
                           const int i = a == 0 ? side * DEG + K1 * t : t + K1 * side * DEG;
                           R[ci * NB + i] += Bz[(QC * 3 + fi) * NB + i] * F[ci];
                         }
+                      if (!PASS2)
 #pragma unroll
-                      for (int i = 0; i < NF; ++i)
-                        acc[ci * NF + i] += Bt[fi * NF + i] * F[ci];
+                        for (int i = 0; i < NF; ++i)
+                          acc[ci * NF + i] += Bt[fi * NF + i] * F[ci];
                     }
                 }
               // ---- 2: TC' (trace-test rows of the scalar columns) into LDS
-              if (isU)
+              if (!PASS2 && isU)
 #pragma unroll
                 for (int ci = 0; ci < NE; ++ci)
                   {
@@ -656,6 +670,48 @@ namespace hdg
               WAVE_SYNC();
             }
 
+          if (PASS2)
+            {
+              // the rhs lane holds u(p_k) in row k: publish u, then
+              // W = M^-1 (rhs_q + D_a u - sum_f C_{a,f} lambda_upd,f) and u
+              // go out lane-parallel (reference template.h:2688-2697)
+              double *sol = bc;
+              if (isR)
+#pragma unroll
+                for (int k = 0; k < NS; ++k)
+                  sol[pivs[k]] = R[k];
+              WAVE_SYNC();
+              for (int e = lane; e < S::NCD; e += 64)
+                {
+                  const int c = e / (3 * NB), r = e % (3 * NB), blkc = r / NB, i = r % NB;
+                  double    v = 0;
+                  if (c < NE)
+                    {
+                      const double *u = sol + c * NB;
+                      if (blkc == 2)
+                        v = u[i];
+                      else
+                        {
+                          const int a = blkc;
+#pragma unroll
+                          for (int l = 0; l < NB; ++l)
+                            v += tb.Minv[i * NB + l] * rhsq[(c * 2 + a) * NB + l] +
+                                 tb.MinvD[(a * NB + i) * NB + l] * u[l];
+#pragma unroll
+                          for (int sd2 = 0; sd2 < 2; ++sd2)
+                            {
+                              const int f = 2 * a + sd2;
+#pragma unroll
+                              for (int m = 0; m < NF; ++m)
+                                v -= tb.MinvC[(f * NB + i) * NF + m] * lupd[(f * 3 + c) * NF + m];
+                            }
+                        }
+                    }
+                  p.dxc[cell * S::NCD + e] = v;
+                }
+              WAVE_SYNC();
+              continue;
+            }
           // ---- 4: [S | g] = [TT' | t'] - TC' X, face by face
 #pragma unroll 1
           for (int g = 0; g < NFC; ++g)
@@ -727,17 +783,18 @@ namespace hdg
         }
     }
 
-    template <int DEG, int NE, bool THERMO>
+    template <int DEG, int NE, bool THERMO, bool PASS2 = false>
     hipError_t launch_fast(const KParams &p, const int *cells, int ncell, int color, int mode,
                            hipStream_t s)
     {
-      using S = F2<DEG, NE>;
+      using S                = F2<DEG, NE>;
+      constexpr size_t bytes = S::bytes_for(PASS2);
       static bool attr = false;
       if (!attr)
         {
-          hipError_t e = hipFuncSetAttribute((const void *)fast_condense<DEG, NE, THERMO>,
+          hipError_t e = hipFuncSetAttribute((const void *)fast_condense<DEG, NE, THERMO, PASS2>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)S::bytes);
+                                             (int)bytes);
           if (e != hipSuccess)
             return e;
           attr = true;
@@ -749,8 +806,8 @@ namespace hdg
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
       const int want = (ncell + WPB - 1) / WPB;
       const int grid = want < 4 * ncu ? want : 4 * ncu;
-      hipLaunchKernelGGL((fast_condense<DEG, NE, THERMO>), dim3(grid), dim3(64 * WPB), S::bytes,
-                         s, p, cells, ncell, color, mode);
+      hipLaunchKernelGGL((fast_condense<DEG, NE, THERMO, PASS2>), dim3(grid), dim3(64 * WPB),
+                         bytes, s, p, cells, ncell, color, mode);
       return hipGetLastError();
     }
   } // namespace
@@ -772,5 +829,14 @@ namespace hdg
                       : launch_fast<2, 1, true>(p, cells, ncell, color, mode, s);
     return deg == 1 ? launch_fast<1, 3, false>(p, cells, ncell, color, mode, s)
                     : launch_fast<2, 3, false>(p, cells, ncell, color, mode, s);
+  }
+
+  hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s)
+  {
+    if (p.thermo)
+      return deg == 1 ? launch_fast<1, 1, true, true>(p, nullptr, ncells, 0, 2, s)
+                      : launch_fast<2, 1, true, true>(p, nullptr, ncells, 0, 2, s);
+    return deg == 1 ? launch_fast<1, 3, false, true>(p, nullptr, ncells, 0, 2, s)
+                    : launch_fast<2, 3, false, true>(p, nullptr, ncells, 0, 2, s);
   }
 } // namespace hdg

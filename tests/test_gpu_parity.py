@@ -96,10 +96,12 @@ def test_global_system_matches_oracle(dim, n_cells, k, variant):
 
 @pytest.mark.parametrize("dim,n_cells,k", CASES)
 @pytest.mark.parametrize("thermodyn", [False, True])
-def test_reconstruct_matches_oracle(dim, n_cells, k, thermodyn):
+@pytest.mark.parametrize("variant", [1, 0])
+def test_reconstruct_matches_oracle(dim, n_cells, k, thermodyn, variant):
     t = _torch()
-    print(f"case reconstruct dim={dim} n={n_cells} k={k} thermo={thermodyn}", flush=True)
-    s, prob, params, adim = _solver(dim, n_cells, k)
+    print(f"case reconstruct dim={dim} n={n_cells} k={k} thermo={thermodyn} variant={variant}",
+          flush=True)
+    s, prob, params, adim = _solver(dim, n_cells, k, variant)
     enabled = (True, False, False) if thermodyn else (True, True, True)
     s.set_enabled_components(*enabled)
     o = H.oracle_for(prob, params, adim, enabled)
