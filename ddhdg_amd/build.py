@@ -21,7 +21,9 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-I" + os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function"]
-SOURCES = ["hdg_tables.cpp", "hdg_kernels.hip", "hdg_fast2d.hip", "hdg_capi.hip"]
+SOURCES = ["hdg_tables.cpp", "hdg_topology.cpp", "hdg_nd_plan.cpp", "hdg_kernels.hip",
+           "hdg_fast2d.hip", "hdg_capi.hip", "hdg_trace_solver.hip"]
+LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def _run(cmd):
@@ -44,7 +46,7 @@ def build(verbose: bool = True) -> str:
         _run([HIPCC, *COMMON, *lang, "-c", path, "-o", obj])
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, *LIBS])
     os.replace(tmp, LIB)
     return LIB
 

@@ -54,8 +54,10 @@ def diode(dim, n_cells, degree, doping=None, bias=0.0, tau=None, nonlinear=None,
     The diode runs use the consistent linearisation of the volume drift term
     (the reference's own converges at best linearly and diverges on these
     1e23 m^-3 junctions); pass consistent_drift_jacobian=False for the
-    reference's."""
-    mesh = BoxMesh(dim, [n_cells] * dim, [0.0] * dim, [1.0] * dim)
+    reference's.  n_cells: cells per axis (one number for all axes, or one
+    per axis)."""
+    ncs = list(n_cells) if hasattr(n_cells, "__len__") else [n_cells] * dim
+    mesh = BoxMesh(dim, ncs, [0.0] * dim, [1.0] * dim)
     doping = doping or step_doping()
     bh = BoundaryConditionHandler()
     prob = HomogeneousProblem(

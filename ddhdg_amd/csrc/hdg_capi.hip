@@ -8,9 +8,8 @@
 #include <string>
 #include <vector>
 
-#include "../../include/ddhdg_hip.h"
-#include "hdg_kernels.h"
-#include "hdg_tables.h"
+#include "hdg_ctx.h"
+#include "hdg_nd_plan.h"
 
 namespace
 {
@@ -21,6 +20,16 @@ namespace
     g_err = m;
     return code;
   }
+} // namespace
+
+int hdg::set_error(const std::string &msg, int code)
+{
+  g_err = msg;
+  return code;
+}
+
+namespace
+{
 
   int hip_fail(hipError_t e, const char *what)
   {
@@ -29,67 +38,11 @@ namespace
   }
 } // namespace
 
-struct hdg_ctx
-{
-  hdg_mesh_desc mesh;
-  hdg_phys      phys;
-  int           d, k, nb, nf, qc, qf, nfc, NC;
-  long long     ncells, nfaces;
-  int           n[3];
-  double        h[3], lo[3];
-  long long     face_off[4], face_dims[3][3];
-  int           en[3], ne, encomp[3];
-  int           variant = 0; // 0 auto, 1 generic, 2 register-resident
-  hdg::HostTables ht;
-  // block pattern: per face, sorted faces it couples with
-  std::vector<long long>     blk_cols;  // concatenated
-  std::vector<long long>     blkptr;    // nfaces+1
-  std::vector<int>           nbk;       // nfaces
-  std::vector<unsigned char> cslot;     // ncells*nfc*nfc
-  std::vector<int>           color_cells[2];
-  // device
-  double         *d_tab = nullptr;
-  hdg::DevTables  tb{};
-  double         *d_doping = nullptr, *d_dirv = nullptr, *d_neuv = nullptr;
-  long long      *d_blkptr = nullptr;
-  int            *d_nbk    = nullptr;
-  unsigned char  *d_cslot  = nullptr;
-  int            *d_cells[2] = {nullptr, nullptr};
-};
-
 namespace
 {
-  long long face_index(const hdg_ctx *C, int a, const int pos[3])
-  {
-    const long long *m = C->face_dims[a];
-    return C->face_off[a] + pos[0] + m[0] * (pos[1] + m[1] * (long long)pos[2]);
-  }
-
-  void cell_coords(const hdg_ctx *C, long long c, int ic[3])
-  {
-    ic[0]       = (int)(c % C->n[0]);
-    long long r = c / C->n[0];
-    ic[1]       = (int)(r % C->n[1]);
-    ic[2]       = (int)(r / C->n[1]);
-  }
-
-  long long cell_face(const hdg_ctx *C, long long c, int lf)
-  {
-    int ic[3];
-    cell_coords(C, c, ic);
-    ic[lf / 2] += lf % 2;
-    return face_index(C, lf / 2, ic);
-  }
-
-  int boundary_id(const hdg_ctx *C, long long c, int lf)
-  {
-    int ic[3];
-    cell_coords(C, c, ic);
-    const int a = lf / 2, s = lf % 2;
-    if ((s == 0 && ic[a] == 0) || (s == 1 && ic[a] == C->n[a] - 1))
-      return lf;
-    return -1;
-  }
+  using hdg::boundary_id;
+  using hdg::cell_coords;
+  using hdg::cell_face;
 
   void set_enabled(hdg_ctx *C, int V, int n, int p)
   {
@@ -209,16 +162,13 @@ int hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys, hdg_ctx **ou
       C->h[a]  = a < C->d ? (mesh->upper[a] - mesh->lower[a]) / mesh->n_cells[a] : 1;
       C->ncells *= C->n[a];
     }
-  C->face_off[0] = 0;
+  hdg::build_topology(C->d, mesh->n_cells, C->topo);
+  C->nfaces = C->topo.nfaces;
+  for (int a = 0; a < 4; ++a)
+    C->face_off[a] = C->topo.face_off[a];
   for (int a = 0; a < 3; ++a)
-    {
-      for (int b = 0; b < 3; ++b)
-        C->face_dims[a][b] = C->n[b] + (a == b ? 1 : 0);
-      const long long cnt =
-        a < C->d ? C->face_dims[a][0] * C->face_dims[a][1] * C->face_dims[a][2] : 0;
-      C->face_off[a + 1] = C->face_off[a] + cnt;
-    }
-  C->nfaces = C->face_off[C->d];
+    for (int b = 0; b < 3; ++b)
+      C->face_dims[a][b] = C->topo.face_dims[a][b];
   if (!hdg::build_tables(C->d, C->k, C->h, C->ht))
     {
       delete C;
@@ -232,40 +182,6 @@ int hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys, hdg_ctx **ou
   C->nfc = T.nfc;
   C->NC  = 3 * (C->d + 1) * C->nb;
   set_enabled(C, 1, 1, 1);
-
-  // block pattern and per-cell scatter slots
-  std::vector<std::vector<long long>> fcells(C->nfaces);
-  for (long long c = 0; c < C->ncells; ++c)
-    for (int lf = 0; lf < C->nfc; ++lf)
-      fcells[cell_face(C, c, lf)].push_back(c);
-  C->blkptr.assign(C->nfaces + 1, 0);
-  C->nbk.assign(C->nfaces, 0);
-  std::vector<std::vector<long long>> cols(C->nfaces);
-  for (long long f = 0; f < C->nfaces; ++f)
-    {
-      for (long long c : fcells[f])
-        for (int lf = 0; lf < C->nfc; ++lf)
-          cols[f].push_back(cell_face(C, c, lf));
-      std::sort(cols[f].begin(), cols[f].end());
-      cols[f].erase(std::unique(cols[f].begin(), cols[f].end()), cols[f].end());
-      C->nbk[f]         = (int)cols[f].size();
-      C->blkptr[f + 1] = C->blkptr[f] + C->nbk[f];
-    }
-  C->blk_cols.reserve(C->blkptr[C->nfaces]);
-  for (long long f = 0; f < C->nfaces; ++f)
-    C->blk_cols.insert(C->blk_cols.end(), cols[f].begin(), cols[f].end());
-  C->cslot.assign(C->ncells * C->nfc * C->nfc, 0);
-  for (long long c = 0; c < C->ncells; ++c)
-    for (int a = 0; a < C->nfc; ++a)
-      {
-        const long long fa = cell_face(C, c, a);
-        for (int b = 0; b < C->nfc; ++b)
-          {
-            const long long fb = cell_face(C, c, b);
-            const auto it = std::lower_bound(cols[fa].begin(), cols[fa].end(), fb);
-            C->cslot[(c * C->nfc + a) * C->nfc + b] = (unsigned char)(it - cols[fa].begin());
-          }
-      }
   for (long long c = 0; c < C->ncells; ++c)
     {
       int ic[3];
@@ -310,13 +226,13 @@ int hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys, hdg_ctx **ou
       tb.MinvC           = b + off[16];
       tb.Mfinv           = b + off[17];
     }
-  std::vector<long long> bp(C->blkptr.begin(), C->blkptr.end() - 1);
+  std::vector<long long> bp(C->topo.blkptr.begin(), C->topo.blkptr.end() - 1);
   if (rc == 0)
     rc = upload(&C->d_blkptr, bp);
   if (rc == 0)
-    rc = upload(&C->d_nbk, C->nbk);
+    rc = upload(&C->d_nbk, C->topo.nbk);
   if (rc == 0)
-    rc = upload(&C->d_cslot, C->cslot);
+    rc = upload(&C->d_cslot, C->topo.cslot);
   for (int col = 0; col < 2 && rc == 0; ++col)
     rc = upload(&C->d_cells[col], C->color_cells[col]);
   if (rc == 0)
@@ -363,7 +279,7 @@ int hdg_sizes(const hdg_ctx *C, long long out[HDG_NSIZES])
   out[HDG_NS]     = (long long)C->ne * C->nb;
   out[HDG_T]      = C->nfc * tdf;
   out[HDG_NTRACE] = C->nfaces * tdf;
-  out[HDG_NNZ]    = C->blkptr[C->nfaces] * tdf * tdf;
+  out[HDG_NNZ]    = C->topo.blkptr[C->nfaces] * tdf * tdf;
   return 0;
 }
 
@@ -402,9 +318,9 @@ int hdg_csr_pattern(const hdg_ctx *C, long long *rowptr, int *colind)
     for (long long r = 0; r < tdf; ++r)
       {
         rowptr[f * tdf + r] = pos;
-        for (long long b = C->blkptr[f]; b < C->blkptr[f + 1]; ++b)
+        for (long long b = C->topo.blkptr[f]; b < C->topo.blkptr[f + 1]; ++b)
           for (long long c = 0; c < tdf; ++c)
-            colind[pos++] = (int)(C->blk_cols[b] * tdf + c);
+            colind[pos++] = (int)(C->topo.blk_cols[b] * tdf + c);
       }
   rowptr[C->nfaces * tdf] = pos;
   return 0;
@@ -579,6 +495,23 @@ int hdg_cell_systems(hdg_ctx *C, const double *x_cell, const double *x_trace, in
                    ? hdg::launch_fast_condense(C->k, p, cells, ncell, 0, 1, (hipStream_t)stream)
                    : hdg::launch_condense(C->d, C->k, p, cells, ncell, 0, 1, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_cell_systems launch");
+}
+
+int hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face, unsigned seed,
+                            const double *csr_vals, double out[3])
+{
+  if (!n_cells || !out || (dim != 2 && dim != 3) || dofs_per_face < 1)
+    return fail("hdg_trace_plan_selftest: invalid argument");
+  for (int a = 0; a < dim; ++a)
+    if (n_cells[a] < 1)
+      return fail("hdg_trace_plan_selftest: invalid mesh", -2);
+  int n[3] = {1, 1, 1};
+  for (int a = 0; a < dim; ++a)
+    n[a] = n_cells[a];
+  std::string err;
+  if (!hdg::nd_plan_selftest(dim, n, dofs_per_face, seed, csr_vals, out, err))
+    return fail("hdg_trace_plan_selftest: " + err, -4);
+  return 0;
 }
 
 } // extern "C"

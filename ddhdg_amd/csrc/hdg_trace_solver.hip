@@ -1,0 +1,492 @@
+// hdg_trace_solver.hip -- the global trace solve on the GPU: a nested-
+// dissection multifrontal LU of the condensed trace system.
+//
+// The reference factors the restricted trace matrix with SparseDirectUMFPACK
+// and solves once per Newton step (src/np_solver.cpp:1326-1354).  Here the
+// plan of hdg_nd_plan.h (tree, fronts, assembly and extend-add maps) is built
+// once per mesh and component set; each factorisation then runs level by
+// level: one strided-batched rocSOLVER getrf of every front's F11 (partial
+// pivoting inside the eliminated block), its row interchanges applied to F12
+// in HIP, two rocBLAS trsm (U12 = L11^-1 P F12, L21 = F21 U11^-1), one gemm
+// (F22 -= L21 U12), and an extend-add of the
+// update blocks into the parents written in HIP, one launch per child slot so
+// that no two writes meet (deterministic sums, no atomics).  The fronts stay
+// in HBM as the factors for the forward and backward sweeps.
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include "hdg_ctx.h"
+#include "hdg_nd_plan.h"
+
+namespace
+{
+  struct DevGroup
+  {
+    long long *d_pf = nullptr, *d_pw = nullptr, *d_edof = nullptr, *d_idof = nullptr;
+    int       *d_pn = nullptr, *d_slot = nullptr, *d_pos = nullptr;
+  };
+
+  constexpr int TPB = 256;
+
+  __global__ void k_fill(double *a, long long n, double v)
+  {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+      a[i] = v;
+  }
+
+  __global__ void k_fill_int(int *a, long long n, int v)
+  {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+      a[i] = v;
+  }
+
+  __global__ void k_scatter(const double *__restrict__ vals, const long long *__restrict__ map,
+                            long long nnz, double *__restrict__ F)
+  {
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nnz;
+         k += (long long)gridDim.x * blockDim.x)
+      F[map[k]] = vals[k];
+  }
+
+  __global__ void k_set_one(double *F, const long long *__restrict__ idx, long long n)
+  {
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
+         k += (long long)gridDim.x * blockDim.x)
+      F[idx[k]] = 1.0;
+  }
+
+  // F22 of each node of one group (child slot want) added into its parent
+  __global__ void k_extend_add(double *F, long long fbase, int n, int S, int I, int batch,
+                               const long long *__restrict__ pf, const int *__restrict__ pn,
+                               const int *__restrict__ slot, const int *__restrict__ pos,
+                               int want, int yoff)
+  {
+    const int b = blockIdx.y + yoff;
+    if (b >= batch || slot[b] != want || pf[b] < 0)
+      return;
+    const double *U  = F + fbase + (long long)b * n * n + S + (long long)S * n;
+    double       *P  = F + pf[b];
+    const long long np = pn[b];
+    const int    *ps = pos + (long long)b * I;
+    const long long II = (long long)I * I;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < II;
+         e += (long long)gridDim.x * blockDim.x)
+      {
+        const int r = (int)(e % I), c = (int)(e / I);
+        const int pr = ps[r], pc = ps[c];
+        if (pr >= 0 && pc >= 0)
+          P[pr + pc * np] += U[r + (long long)c * n];
+      }
+  }
+
+  // getrf's row interchanges applied to F12 (columns S..n-1) of every front
+  // of a group; one thread per column, the swaps in order
+  __global__ void k_laswp_cols(double *F, long long fbase, int n, int S, int I,
+                               const int *__restrict__ ipiv, long long pbase, int batch, int yoff)
+  {
+    const int b = blockIdx.y + yoff;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch || c >= I)
+      return;
+    double    *col = F + fbase + (long long)b * n * n + (long long)(S + c) * n;
+    const int *pv  = ipiv + pbase + (long long)b * S;
+    for (int j = 0; j < S; ++j)
+      {
+        const int p = pv[j] - 1;
+        if (p != j)
+          {
+            const double t = col[j];
+            col[j]         = col[p];
+            col[p]         = t;
+          }
+      }
+  }
+
+  __global__ void k_gather_rhs(const double *__restrict__ b, const long long *__restrict__ wmap,
+                               long long ndof, double *__restrict__ w)
+  {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < ndof;
+         i += (long long)gridDim.x * blockDim.x)
+      w[wmap[i]] = b[i];
+  }
+
+  // row interchanges of getrf applied to the eliminated part of each work vector
+  __global__ void k_laswp(double *w, long long wbase, int n, int S, const int *__restrict__ ipiv,
+                          long long pbase, int batch)
+  {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch)
+      return;
+    double    *x  = w + wbase + (long long)b * n;
+    const int *pv = ipiv + pbase + (long long)b * S;
+    for (int j = 0; j < S; ++j)
+      {
+        const int p = pv[j] - 1;
+        if (p != j)
+          {
+            const double t = x[j];
+            x[j]           = x[p];
+            x[p]           = t;
+          }
+      }
+  }
+
+  __global__ void k_vec_extend_add(double *w, long long wbase, int n, int S, int I, int batch,
+                                   const long long *__restrict__ pw,
+                                   const long long *__restrict__ pf, const int *__restrict__ slot,
+                                   const int *__restrict__ pos, int want, int yoff)
+  {
+    const int b = blockIdx.y + yoff;
+    if (b >= batch || slot[b] != want || pf[b] < 0)
+      return;
+    const double *u  = w + wbase + (long long)b * n + S;
+    const int    *ps = pos + (long long)b * I;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < I; r += gridDim.x * blockDim.x)
+      if (ps[r] >= 0)
+        w[pw[b] + ps[r]] += u[r];
+  }
+
+  __global__ void k_gather_x(const double *__restrict__ x, double *w, long long wbase, int n,
+                             int S, int I, int batch, const long long *__restrict__ idof, int yoff)
+  {
+    const int b = blockIdx.y + yoff;
+    if (b >= batch)
+      return;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < I; r += gridDim.x * blockDim.x)
+      {
+        const long long dof = idof[(long long)b * I + r];
+        w[wbase + (long long)b * n + S + r] = dof >= 0 ? x[dof] : 0.0;
+      }
+  }
+
+  __global__ void k_scatter_x(double *x, const double *__restrict__ w, long long wbase, int n,
+                              int S, int batch, const long long *__restrict__ edof, int yoff)
+  {
+    const int b = blockIdx.y + yoff;
+    if (b >= batch)
+      return;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < S; j += gridDim.x * blockDim.x)
+      {
+        const long long dof = edof[(long long)b * S + j];
+        if (dof >= 0)
+          x[dof] = w[wbase + (long long)b * n + j];
+      }
+  }
+
+  constexpr int YCHUNK = 32768; // nodes per launch along grid y
+
+  int grid_for(long long work)
+  {
+    const long long g = (work + TPB - 1) / TPB;
+    return (int)std::max<long long>(1, std::min<long long>(g, 8192));
+  }
+
+  template <class T>
+  int upload(T **dst, const std::vector<T> &src, const char *what)
+  {
+    *dst = nullptr;
+    if (src.empty())
+      return 0;
+    hipError_t e = hipMalloc((void **)dst, src.size() * sizeof(T));
+    if (e == hipSuccess)
+      e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+      return hdg::set_error(std::string(what) + ": " + hipGetErrorString(e), -3);
+    return 0;
+  }
+} // namespace
+
+struct hdg_trace_solver
+{
+  hdg::NDPlan           P;
+  std::vector<DevGroup> dg;
+  rocblas_handle        h      = nullptr;
+  double               *d_F    = nullptr, *d_w = nullptr;
+  int                  *d_ipiv = nullptr, *d_info = nullptr;
+  long long            *d_map = nullptr, *d_pad = nullptr, *d_wmap = nullptr;
+};
+
+extern "C" {
+
+int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
+{
+  if (!C || !out)
+    return hdg::set_error("hdg_trace_solver_create: null argument", -1);
+  *out                = nullptr;
+  hdg_trace_solver *T = new hdg_trace_solver();
+  std::string       err;
+  if (!hdg::build_nd_plan(C->topo, C->ne * C->nf, T->P, err))
+    {
+      delete T;
+      return hdg::set_error("hdg_trace_solver_create: " + err, -4);
+    }
+  const hdg::NDPlan &P = T->P;
+  int                rc = 0;
+  T->dg.resize(P.groups.size());
+  for (size_t gi = 0; gi < P.groups.size() && rc == 0; ++gi)
+    {
+      const hdg::NDGroup &G = P.groups[gi];
+      DevGroup           &D = T->dg[gi];
+      if (rc == 0)
+        rc = upload(&D.d_pf, G.pf, "trace solver tables");
+      if (rc == 0)
+        rc = upload(&D.d_pw, G.pw, "trace solver tables");
+      if (rc == 0)
+        rc = upload(&D.d_pn, G.pn, "trace solver tables");
+      if (rc == 0)
+        rc = upload(&D.d_slot, G.slot, "trace solver tables");
+      if (rc == 0)
+        rc = upload(&D.d_pos, G.pos, "trace solver tables");
+      if (rc == 0)
+        rc = upload(&D.d_edof, G.edof, "trace solver tables");
+      if (rc == 0)
+        rc = upload(&D.d_idof, G.idof, "trace solver tables");
+    }
+  if (rc == 0)
+    rc = upload(&T->d_map, P.map, "trace solver map");
+  if (rc == 0)
+    rc = upload(&T->d_pad, P.pad, "trace solver padding");
+  if (rc == 0)
+    rc = upload(&T->d_wmap, P.wmap, "trace solver rhs map");
+  if (rc == 0)
+    {
+      hipError_t e = hipMalloc((void **)&T->d_F, (size_t)P.Ftot * sizeof(double));
+      if (e == hipSuccess)
+        e = hipMalloc((void **)&T->d_w, (size_t)std::max<long long>(1, P.wtot) * sizeof(double));
+      if (e == hipSuccess)
+        e = hipMalloc((void **)&T->d_ipiv, (size_t)std::max<long long>(1, P.ptot) * sizeof(int));
+      if (e == hipSuccess)
+        e = hipMalloc((void **)&T->d_info, (size_t)P.nnodes * sizeof(int));
+      if (e != hipSuccess)
+        rc = hdg::set_error(std::string("hdg_trace_solver_create: fronts: ") +
+                              hipGetErrorString(e),
+                            -3);
+    }
+  if (rc == 0 && rocblas_create_handle(&T->h) != rocblas_status_success)
+    rc = hdg::set_error("hdg_trace_solver_create: rocblas_create_handle failed", -3);
+  if (rc != 0)
+    {
+      hdg_trace_solver_destroy(T);
+      return rc;
+    }
+  *out = T;
+  return 0;
+}
+
+void hdg_trace_solver_destroy(hdg_trace_solver *T)
+{
+  if (!T)
+    return;
+  for (DevGroup &D : T->dg)
+    for (void *p : {(void *)D.d_pf, (void *)D.d_pw, (void *)D.d_pn, (void *)D.d_slot,
+                    (void *)D.d_pos, (void *)D.d_edof, (void *)D.d_idof})
+      if (p)
+        (void)hipFree(p);
+  for (void *p : {(void *)T->d_F, (void *)T->d_w, (void *)T->d_ipiv, (void *)T->d_info,
+                  (void *)T->d_map, (void *)T->d_pad, (void *)T->d_wmap})
+    if (p)
+      hipFree(p);
+  if (T->h)
+    rocblas_destroy_handle(T->h);
+  delete T;
+}
+
+int hdg_trace_solver_info(const hdg_trace_solver *T, double out[HDG_TS_NINFO])
+{
+  if (!T || !out)
+    return hdg::set_error("hdg_trace_solver_info: null argument", -1);
+  out[HDG_TS_FLOPS]     = T->P.flops;
+  out[HDG_TS_BYTES]     = (double)T->P.Ftot * sizeof(double);
+  out[HDG_TS_LEVELS]    = (double)T->P.groups.size();
+  out[HDG_TS_MAX_FRONT] = (double)T->P.max_front;
+  out[HDG_TS_NODES]     = (double)T->P.nnodes;
+  return 0;
+}
+
+int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
+{
+  if (!T || !csr_vals)
+    return hdg::set_error("hdg_trace_factor: null argument", -1);
+  hipStream_t s = (hipStream_t)stream;
+  rocblas_set_stream(T->h, s);
+  rocblas_set_pointer_mode(T->h, rocblas_pointer_mode_host);
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(T->P.Ftot)), dim3(TPB), 0, s, T->d_F, T->P.Ftot, 0.0);
+  hipLaunchKernelGGL(k_fill_int, dim3(grid_for(T->P.nnodes)), dim3(TPB), 0, s,
+                     T->d_info, T->P.nnodes, 0);
+  hipLaunchKernelGGL(k_scatter, dim3(grid_for(T->P.nnz)), dim3(TPB), 0, s, csr_vals, T->d_map,
+                     T->P.nnz, T->d_F);
+  if ((long long)T->P.pad.size() > 0)
+    hipLaunchKernelGGL(k_set_one, dim3(grid_for((long long)T->P.pad.size())), dim3(TPB), 0, s, T->d_F, T->d_pad,
+                       (long long)T->P.pad.size());
+  const double one = 1.0, mone = -1.0;
+  long long    info_off = 0;
+  for (size_t gi = 0; gi < T->P.groups.size(); ++gi)
+    {
+      const hdg::NDGroup &G = T->P.groups[gi];
+      const DevGroup     &D = T->dg[gi];
+      double             *F = T->d_F + G.fbase;
+      if (G.S > 0)
+        {
+          // F11 = P L11 U11 (partial pivoting inside the eliminated block)
+          rocblas_status st = rocsolver_dgetrf_strided_batched(
+            T->h, G.S, G.S, F, G.n, (rocblas_stride)G.n * G.n, T->d_ipiv + G.pbase,
+            (rocblas_stride)G.S, T->d_info + info_off, G.batch);
+          if (st != rocblas_status_success)
+            return hdg::set_error("hdg_trace_factor: rocsolver getrf failed", -3);
+          if (G.I > 0)
+            {
+              // U12 = L11^-1 P F12, L21 = F21 U11^-1, F22 -= L21 U12
+              for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+                hipLaunchKernelGGL(k_laswp_cols,
+                                   dim3((G.I + 63) / 64, std::min(YCHUNK, G.batch - y0)),
+                                   dim3(64), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, T->d_ipiv,
+                                   G.pbase, G.batch, y0);
+              st = rocblas_dtrsm_strided_batched(T->h, rocblas_side_left, rocblas_fill_lower,
+                                                 rocblas_operation_none, rocblas_diagonal_unit,
+                                                 G.S, G.I, &one, F, G.n,
+                                                 (rocblas_stride)G.n * G.n,
+                                                 F + (long long)G.S * G.n, G.n,
+                                                 (rocblas_stride)G.n * G.n, G.batch);
+              if (st == rocblas_status_success)
+                st = rocblas_dtrsm_strided_batched(T->h, rocblas_side_right, rocblas_fill_upper,
+                                                   rocblas_operation_none,
+                                                   rocblas_diagonal_non_unit, G.I, G.S, &one, F,
+                                                   G.n, (rocblas_stride)G.n * G.n, F + G.S, G.n,
+                                                   (rocblas_stride)G.n * G.n, G.batch);
+              if (st == rocblas_status_success)
+                st = rocblas_dgemm_strided_batched(
+                  T->h, rocblas_operation_none, rocblas_operation_none, G.I, G.I, G.S, &mone,
+                  F + G.S, G.n, (rocblas_stride)G.n * G.n, F + (long long)G.S * G.n, G.n,
+                  (rocblas_stride)G.n * G.n, &one, F + G.S + (long long)G.S * G.n, G.n,
+                  (rocblas_stride)G.n * G.n, G.batch);
+              if (st != rocblas_status_success)
+                return hdg::set_error("hdg_trace_factor: rocblas trsm/gemm failed", -3);
+            }
+        }
+      info_off += G.batch;
+      if (G.I > 0)
+        for (int want = 0; want < 2; ++want)
+          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+            hipLaunchKernelGGL(
+              k_extend_add,
+              dim3(std::max(1, std::min(4096, (int)(((long long)G.I * G.I + TPB - 1) / TPB))),
+                   std::min(YCHUNK, G.batch - y0)),
+              dim3(TPB), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, G.batch, D.d_pf, D.d_pn,
+              D.d_slot, D.d_pos, want, y0);
+    }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hdg::set_error(std::string("hdg_trace_factor: ") + hipGetErrorString(e), -3);
+}
+
+int hdg_trace_factor_status(hdg_trace_solver *T, void *stream)
+{
+  if (!T)
+    return hdg::set_error("hdg_trace_factor_status: null argument", -1);
+  std::vector<int> info((size_t)T->P.nnodes);
+  hipError_t e = hipMemcpyAsync(info.data(), T->d_info, info.size() * sizeof(int),
+                                hipMemcpyDeviceToHost, (hipStream_t)stream);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess)
+    return hdg::set_error(std::string("hdg_trace_factor_status: ") + hipGetErrorString(e), -3);
+  size_t k = 0;
+  for (size_t gi = 0; gi < T->P.groups.size(); ++gi)
+    for (int b = 0; b < T->P.groups[gi].batch; ++b, ++k)
+      if (info[k] != 0)
+        return hdg::set_error("hdg_trace_factor: exactly zero pivot at level " +
+                                std::to_string(gi) + ", front " + std::to_string(b) +
+                                ", column " + std::to_string(info[k]),
+                              -5);
+  return 0;
+}
+
+int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *stream)
+{
+  if (!T || !rhs || !x)
+    return hdg::set_error("hdg_trace_solve: null argument", -1);
+  hipStream_t s = (hipStream_t)stream;
+  rocblas_set_stream(T->h, s);
+  rocblas_set_pointer_mode(T->h, rocblas_pointer_mode_host);
+  const double one = 1.0, mone = -1.0;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(T->P.wtot)), dim3(TPB), 0, s, T->d_w, T->P.wtot, 0.0);
+  hipLaunchKernelGGL(k_gather_rhs, dim3(grid_for(T->P.ndof)), dim3(TPB), 0, s, rhs, T->d_wmap,
+                     T->P.ndof, T->d_w);
+  // forward: z = L^-1 P b, update vectors to the parents
+  for (size_t gi = 0; gi < T->P.groups.size(); ++gi)
+    {
+      const hdg::NDGroup &G = T->P.groups[gi];
+      const DevGroup     &D = T->dg[gi];
+      const double       *F = T->d_F + G.fbase;
+      double       *w = T->d_w + G.wbase;
+      if (G.S > 0)
+        {
+          hipLaunchKernelGGL(k_laswp, dim3((G.batch + 63) / 64), dim3(64), 0, s, T->d_w,
+                             G.wbase, G.n, G.S, T->d_ipiv, G.pbase, G.batch);
+          rocblas_status st = rocblas_dtrsv_strided_batched(
+            T->h, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, G.S, F, G.n,
+            (rocblas_stride)G.n * G.n, w, 1, (rocblas_stride)G.n, G.batch);
+          if (st == rocblas_status_success && G.I > 0)
+            st = rocblas_dgemv_strided_batched(T->h, rocblas_operation_none, G.I, G.S, &mone,
+                                               F + G.S, G.n, (rocblas_stride)G.n * G.n, w, 1,
+                                               (rocblas_stride)G.n, &one, w + G.S, 1,
+                                               (rocblas_stride)G.n, G.batch);
+          if (st != rocblas_status_success)
+            return hdg::set_error("hdg_trace_solve: rocblas forward sweep failed", -3);
+        }
+      if (G.I > 0)
+        for (int want = 0; want < 2; ++want)
+          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+            hipLaunchKernelGGL(k_vec_extend_add,
+                               dim3((G.I + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),
+                               dim3(TPB), 0, s, T->d_w, G.wbase, G.n, G.S, G.I, G.batch, D.d_pw,
+                               D.d_pf, D.d_slot, D.d_pos, want, y0);
+    }
+  // backward: x_s = U11^-1 (z - U12 x_i), root first
+  for (int gi = (int)T->P.groups.size() - 1; gi >= 0; --gi)
+    {
+      const hdg::NDGroup &G = T->P.groups[gi];
+      const DevGroup     &D = T->dg[gi];
+      const double *F = T->d_F + G.fbase;
+      double       *w = T->d_w + G.wbase;
+      if (G.I > 0)
+        for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+          hipLaunchKernelGGL(k_gather_x,
+                             dim3((G.I + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),
+                             dim3(TPB), 0, s, x, T->d_w, G.wbase, G.n, G.S, G.I, G.batch,
+                             D.d_idof, y0);
+      if (G.S > 0)
+        {
+          rocblas_status st = rocblas_status_success;
+          if (G.I > 0)
+            st = rocblas_dgemv_strided_batched(T->h, rocblas_operation_none, G.S, G.I, &mone,
+                                               F + (long long)G.S * G.n, G.n,
+                                               (rocblas_stride)G.n * G.n, w + G.S, 1,
+                                               (rocblas_stride)G.n, &one, w, 1,
+                                               (rocblas_stride)G.n, G.batch);
+          if (st == rocblas_status_success)
+            st = rocblas_dtrsv_strided_batched(T->h, rocblas_fill_upper, rocblas_operation_none,
+                                               rocblas_diagonal_non_unit, G.S, F, G.n,
+                                               (rocblas_stride)G.n * G.n, w, 1,
+                                               (rocblas_stride)G.n, G.batch);
+          if (st != rocblas_status_success)
+            return hdg::set_error("hdg_trace_solve: rocblas backward sweep failed", -3);
+          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+            hipLaunchKernelGGL(k_scatter_x,
+                               dim3((G.S + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),
+                               dim3(TPB), 0, s, x, T->d_w, G.wbase, G.n, G.S, G.batch, D.d_edof,
+                               y0);
+        }
+    }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hdg::set_error(std::string("hdg_trace_solve: ") + hipGetErrorString(e), -3);
+}
+
+} // extern "C"

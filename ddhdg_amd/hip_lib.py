@@ -25,7 +25,13 @@ EXPORTS = [
     "hdg_csr_pattern", "hdg_dirichlet_dofs", "hdg_cell_qpoints", "hdg_face_qpoints",
     "hdg_set_doping", "hdg_set_bc_values", "hdg_condense", "hdg_reconstruct",
     "hdg_cell_systems", "hdg_set_kernel_variant",
+    "hdg_trace_solver_create", "hdg_trace_solver_destroy", "hdg_trace_solver_info",
+    "hdg_trace_factor", "hdg_trace_factor_status", "hdg_trace_solve",
+    "hdg_trace_plan_selftest",
 ]
+
+(HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
+ HDG_TS_NINFO) = range(6)
 
 
 class HdgError(RuntimeError):
@@ -71,6 +77,13 @@ def lib():
             "hdg_condense": (i, [v, v, v, i, v, v, v, v]),
             "hdg_reconstruct": (i, [v, v, v, v, i, v, v]),
             "hdg_cell_systems": (i, [v, v, v, i, v, i, v, v, v]),
+            "hdg_trace_solver_create": (i, [v, P(v)]),
+            "hdg_trace_solver_destroy": (None, [v]),
+            "hdg_trace_solver_info": (i, [v, dp]),
+            "hdg_trace_factor": (i, [v, v, v]),
+            "hdg_trace_factor_status": (i, [v, v]),
+            "hdg_trace_solve": (i, [v, v, v, v]),
+            "hdg_trace_plan_selftest": (i, [i, ip, i, ct.c_uint, dp, dp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -186,3 +199,54 @@ class HdgContext:
         _check(self.L.hdg_cell_systems(self.ctx, _ptr(x_cell), _ptr(x_trace), int(thermodyn),
                                        _ptr(cells), int(cells.numel()), _ptr(S), _ptr(g),
                                        _stream()))
+
+
+class TraceSolver:
+    """GPU nested-dissection multifrontal LU of a context's restricted trace
+    system (the global solve; SparseDirectUMFPACK in the reference).  Built
+    for the context's current enabled components."""
+
+    def __init__(self, ctx: HdgContext):
+        self.L = lib()
+        self.h = ct.c_void_p()
+        _check(self.L.hdg_trace_solver_create(ctx.ctx, ct.byref(self.h)))
+        info = np.zeros(HDG_TS_NINFO)
+        _check(self.L.hdg_trace_solver_info(self.h, _dp(info)))
+        self.flops, self.bytes = float(info[HDG_TS_FLOPS]), float(info[HDG_TS_BYTES])
+        self.levels, self.max_front = int(info[HDG_TS_LEVELS]), int(info[HDG_TS_MAX_FRONT])
+        self.nodes = int(info[HDG_TS_NODES])
+        self._ctx = ctx
+
+    def close(self):
+        if self.h:
+            self.L.hdg_trace_solver_destroy(self.h)
+            self.h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def factor(self, vals, check=False):
+        _check(self.L.hdg_trace_factor(self.h, _ptr(vals), _stream()))
+        if check:
+            _check(self.L.hdg_trace_factor_status(self.h, _stream()))
+
+    def solve(self, rhs, x):
+        _check(self.L.hdg_trace_solve(self.h, _ptr(rhs), _ptr(x), _stream()))
+
+
+def trace_plan_selftest(dim, n_cells, dofs_per_face, seed=1, csr_vals=None):
+    """host-only check of the trace solver's plan (no GPU): returns the
+    relative error of the plan's multifrontal solve against dense
+    elimination (on csr_vals, a host array in the trace CSR order, or a
+    random matrix), the number of tree nodes and of levels."""
+    n = (ct.c_int * 3)(*(list(n_cells) + [1] * (3 - len(n_cells))))
+    out = np.zeros(3)
+    vals = None
+    if csr_vals is not None:
+        vals = np.ascontiguousarray(csr_vals, dtype=np.float64)
+    _check(lib().hdg_trace_plan_selftest(int(dim), n, int(dofs_per_face), int(seed),
+                                         _dp(vals) if vals is not None else None, _dp(out)))
+    return float(out[0]), int(out[1]), int(out[2])

@@ -22,7 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import fe
-from .hip_lib import HdgContext, MeshDesc, Phys
+from .hip_lib import HdgContext, MeshDesc, Phys, TraceSolver
 from .problem import (DIRICHLET, EV, KB, NEUMANN, Q, Adimensionalizer,
                       FixedTauNPSolverParameters, LinearRecombinationTerm,
                       ShockleyReadHallFixedTemperature, compute_density, evaluate,
@@ -116,6 +116,7 @@ class NPSolver:
         self.x_trace = torch.zeros(self.ctx.nfaces * 3 * self.ctx.nf, dtype=torch.float64,
                                    device=self.device)
         self._pattern_cache = {}
+        self._solver_cache = {}
 
     # ------------------------------------------------------------ set-up
     def _upload_problem_data(self):
@@ -266,33 +267,63 @@ class NPSolver:
         A = sp.csr_matrix((vals.cpu().numpy(), ci, rp), shape=(self.ctx.ntrace,) * 2)
         return A, rhs.cpu().numpy(), dirs, dirv.cpu().numpy()[dirs]
 
-    def _private_run(self, abs_tol, rel_tol, max_it, thermodyn, log=None):
+    def trace_solver(self):
+        """the GPU trace solver for the current enabled components (built on
+        first use, kept per component set)."""
+        key = self.enabled
+        if key not in self._solver_cache:
+            self._solver_cache[key] = TraceSolver(self.ctx)
+        return self._solver_cache[key]
+
+    def _solve_trace(self, vals, rhs, sol):
+        """solve_linear_problem (src/np_solver.cpp:1326-1354): the condensed
+        trace system, on the device or with SuperLU on the host."""
+        if self.parameters.trace_solver == "gpu":
+            ts = self.trace_solver()
+            ts.factor(vals)
+            ts.solve(rhs, sol)
+            return
+        if self.parameters.trace_solver != "host":
+            raise ValueError(f"trace_solver must be 'gpu' or 'host', not "
+                             f"{self.parameters.trace_solver!r}")
+        import scipy.sparse as sp
         import scipy.sparse.linalg as spla
+        rp, ci, _, _ = self._pattern()
+        n_ = self.ctx.ntrace
+        A = sp.csc_matrix(sp.csr_matrix((vals.cpu().numpy(), ci, rp), shape=(n_, n_)))
+        sol.copy_(self.torch.as_tensor(spla.splu(A).solve(rhs.cpu().numpy())))
+
+    def _private_run(self, abs_tol, rel_tol, max_it, thermodyn, log=None):
+        """NPSolver::private_run (src/np_solver.cpp:1483-1587): Newton with the
+        L-infinity update criterion against the state before the update.
+        Every vector stays on the device; one host synchronisation per
+        iteration reads the norms."""
         t = self.torch
-        rp, ci, dirs, r2f = self._pattern()
+        _, _, dirs, r2f = self._pattern()
         ctx = self.ctx
         vals = t.empty(ctx.nnz, dtype=t.float64, device=self.device)
         rhs = t.empty(ctx.ntrace, dtype=t.float64, device=self.device)
+        sol = t.empty(ctx.ntrace, dtype=t.float64, device=self.device)
         dirv = t.zeros(ctx.ntrace, dtype=t.float64, device=self.device)
         upd_trace = t.zeros_like(self.x_trace)
         upd_cell = t.zeros_like(self.x_cell)
         dirs_t = t.as_tensor(dirs, device=self.device)
         alpha = self.parameters.nonlinear_parameters.alpha
         converged, step, upd_cell_norm = False, 0, 0.0
-        import scipy.sparse as sp
         for step in range(1, max_it + 1 if max_it >= 0 else 1 << 30):
             ctx.condense(self.x_cell, self.x_trace, thermodyn, vals, rhs, dirv)
-            A = sp.csc_matrix(sp.csr_matrix((vals.cpu().numpy(), ci, rp),
-                                            shape=(ctx.ntrace, ctx.ntrace)))
-            sol = spla.splu(A).solve(rhs.cpu().numpy())
-            sol_t = t.as_tensor(sol, device=self.device)
-            sol_t[dirs_t] = dirv[dirs_t]
+            self._solve_trace(vals, rhs, sol)
+            sol[dirs_t] = dirv[dirs_t]
             upd_trace.zero_()
-            upd_trace[r2f] = sol_t
+            upd_trace[r2f] = sol
             ctx.reconstruct(self.x_cell, self.x_trace, upd_trace, thermodyn, upd_cell)
-            upd_cell_norm = float(upd_cell.abs().max())
-            upd_norm = max(upd_cell_norm, float(upd_trace.abs().max()))
-            cur_norm = max(float(self.x_cell.abs().max()), float(self.x_trace.abs().max()))
+            norms = t.stack([upd_cell.abs().max(), upd_trace.abs().max(),
+                             self.x_cell.abs().max(), self.x_trace.abs().max()]).cpu().numpy()
+            upd_cell_norm = float(norms[0])
+            upd_norm = max(upd_cell_norm, float(norms[1]))
+            cur_norm = max(float(norms[2]), float(norms[3]))
+            if not np.isfinite(upd_norm) and self.parameters.trace_solver == "gpu":
+                self.trace_solver().factor(vals, check=True)
             self.x_cell.add_(upd_cell, alpha=alpha)
             self.x_trace.add_(upd_trace, alpha=alpha)
             if log is not None:

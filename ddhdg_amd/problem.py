@@ -176,6 +176,10 @@ class FixedTauNPSolverParameters:
     multithreading: bool = True
     dd_flux_type: str = "use_cell"
     phi_linearize: bool = False
+    # global trace solve: "gpu" = nested-dissection multifrontal LU on the
+    # device (the default), "host" = SciPy's SuperLU on the host (the
+    # reference's plain sparse direct form, UMFPACK, src/np_solver.cpp:1347-1351)
+    trace_solver: str = "gpu"
     # False: the reference's linearisation of the volume drift term
     # (include/assembling_methods.template.h:500, 521-522), whose sign is
     # opposite to the derivative of its own residual (:656, :673), so that

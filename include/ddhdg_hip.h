@@ -171,6 +171,49 @@ int hdg_cell_systems(hdg_ctx *ctx, const double *x_cell,
                      const double *x_trace, int thermodyn, const int *cells,
                      int ncell, double *S, double *g, void *stream);
 
+/* ---- Global trace solve ------------------------------------------------
+ * Sparse direct LU of the restricted trace system, on the GPU: a nested-
+ * dissection multifrontal factorisation over the structured cell grid
+ * (dense fronts in HBM, batched rocSOLVER/rocBLAS per tree level).  Replaces
+ * the reference's SparseDirectUMFPACK factorize + solve of system_matrix
+ * (src/np_solver.cpp:1326-1354).  A solver is built for the context's mesh
+ * and its enabled components at creation; rebuild it after
+ * hdg_set_enabled. */
+typedef struct hdg_trace_solver hdg_trace_solver;
+
+enum
+{
+  HDG_TS_FLOPS = 0,  /* factorisation flops (dense fronts, padded) */
+  HDG_TS_BYTES,      /* device bytes held by the fronts (= the factors) */
+  HDG_TS_LEVELS,     /* tree levels (batched launches per sweep) */
+  HDG_TS_MAX_FRONT,  /* largest front dimension */
+  HDG_TS_NODES,      /* tree nodes */
+  HDG_TS_NINFO
+};
+
+int  hdg_trace_solver_create(const hdg_ctx *ctx, hdg_trace_solver **out);
+void hdg_trace_solver_destroy(hdg_trace_solver *solver);
+int  hdg_trace_solver_info(const hdg_trace_solver *solver, double out[HDG_TS_NINFO]);
+/* Factorise the matrix whose values (device, NNZ, hdg_csr_pattern order)
+ * hdg_condense wrote; asynchronous on stream. */
+int  hdg_trace_factor(hdg_trace_solver *solver, const double *csr_vals,
+                      void *stream);
+/* Synchronises stream; -5 if a front met an exactly zero pivot. */
+int  hdg_trace_factor_status(hdg_trace_solver *solver, void *stream);
+/* x = A^-1 rhs for the last factorised A (device vectors, NTRACE). */
+int  hdg_trace_solve(hdg_trace_solver *solver, const double *rhs, double *x,
+                     void *stream);
+/* Host-only check of the solver's plan for a dim-D n_cells grid with
+ * dofs_per_face trace dofs per face: every index against its buffer, then the
+ * plan's multifrontal arithmetic in host loops against dense elimination, on
+ * csr_vals (host, NNZ, hdg_csr_pattern order) or, when csr_vals is NULL, on a
+ * random diagonally dominant matrix in the trace pattern.  out = {relative
+ * error, tree nodes, levels}.  Small grids only (about 3000 trace dofs at
+ * most).  No counterpart in the reference (UMFPACK is a library call
+ * there). */
+int  hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face,
+                             unsigned seed, const double *csr_vals, double out[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,3 +72,16 @@ def test_charge_neutrality_closed_form():
     v = local_charge_neutrality_potential(prob, np.array([1e23, -1e23, 0.0]), 300.0)
     assert v[0] == pytest.approx(ut * np.arcsinh(1e23 / (2 * 1.45e16)), rel=1e-14)
     assert v[1] == pytest.approx(-v[0], rel=1e-14) and v[2] == 0.0
+
+
+@pytest.mark.parametrize("dim,n_cells,tdf", [
+    (2, (4, 4), 9), (2, (6, 10), 3), (2, (5, 3), 9), (2, (1, 7), 6), (2, (9, 1), 3),
+    (2, (1, 1), 9), (2, (12, 12), 2), (3, (2, 3, 2), 12), (3, (4, 4, 4), 4),
+    (3, (3, 5, 2), 4), (3, (1, 1, 5), 12)])
+def test_trace_solver_plan(dim, n_cells, tdf):
+    """the GPU trace solver's nested-dissection plan, on the host: every index
+    within its buffer, each dof eliminated once, and its multifrontal
+    arithmetic (the device solver's order of operations, in host loops) equal
+    to dense Gaussian elimination on a random matrix with the trace pattern"""
+    err, nodes, levels = hip_lib.trace_plan_selftest(dim, n_cells, tdf, seed=3)
+    assert err < 1e-13, (err, nodes, levels)

@@ -1,0 +1,81 @@
+"""The GPU trace solver (nested-dissection multifrontal LU) against SciPy's
+sparse direct solve of the same condensed systems (GPU tests).
+
+Bars: relative residual ||A x - b|| / ||b|| <= 1e-12 and relative difference
+to SciPy's SuperLU solution <= 1e-8 (both backward-stable direct solves; the
+difference is bounded by the condition number times the rounding)."""
+import numpy as np
+import pytest
+
+from ddhdg_amd import configs
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, (4, 4), 1), (2, (8, 8), 2), (2, (6, 10), 2), (2, (5, 3), 3), (2, (1, 6), 1),
+         (3, (2, 3, 2), 1), (3, (4, 4, 4), 1)]
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU test on a machine without a GPU"
+    return torch
+
+
+@pytest.mark.parametrize("dim,n,k", CASES)
+@pytest.mark.parametrize("thermodyn", [False, True])
+def test_trace_solver_matches_superlu(dim, n, k, thermodyn):
+    import scipy.sparse.linalg as spla
+    t = _torch()
+    print(f"case trace solve dim={dim} n={n} k={k} thermo={thermodyn}", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(dim, n, k)
+    s = NPSolver(prob, params, adim)
+    if thermodyn:
+        s.set_enabled_components(True, False, False)
+    s.set_state(*H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=5))
+    A, b, dirs, dv = s.assemble_system(thermodyn)
+    ctx = s.ctx
+    vals = t.empty(ctx.nnz, dtype=t.float64, device="cuda")
+    rhs = t.empty(ctx.ntrace, dtype=t.float64, device="cuda")
+    dirv = t.zeros(ctx.ntrace, dtype=t.float64, device="cuda")
+    ctx.condense(s.x_cell, s.x_trace, thermodyn, vals, rhs, dirv)
+    ts = s.trace_solver()
+    x = t.empty(ctx.ntrace, dtype=t.float64, device="cuda")
+    ts.factor(vals, check=True)
+    ts.solve(rhs, x)
+    t.cuda.synchronize()
+    x = x.cpu().numpy()
+    xr = spla.splu(A.tocsc()).solve(b)
+    res = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
+    diff = H.rel_fro(x, xr)
+    print(f"  levels={ts.levels} max front={ts.max_front} residual={res:.2e} "
+          f"vs SuperLU={diff:.2e}")
+    assert res < 1e-12 and diff < 1e-8
+
+
+def test_trace_solver_refactor_and_resolve():
+    """a second factorisation of different values and two right-hand sides
+    reuse the solver's buffers without stale state"""
+    import scipy.sparse.linalg as spla
+    t = _torch()
+    print("case trace solve refactor 2D 6x6 k=2", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(2, 6, 2)
+    s = NPSolver(prob, params, adim)
+    ts = s.trace_solver()
+    for seed in (1, 2):
+        s.set_state(*H.perturbed(*H.equilibrium_guess(prob, adim, 2), seed=seed))
+        A, b, _, _ = s.assemble_system()
+        ctx = s.ctx
+        vals = t.empty(ctx.nnz, dtype=t.float64, device="cuda")
+        rhs = t.empty(ctx.ntrace, dtype=t.float64, device="cuda")
+        dirv = t.zeros(ctx.ntrace, dtype=t.float64, device="cuda")
+        ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv)
+        ts.factor(vals)
+        lu = spla.splu(A.tocsc())
+        for scale in (1.0, -3.0):
+            x = t.empty_like(rhs)
+            ts.solve(rhs * scale, x)
+            t.cuda.synchronize()
+            assert H.rel_fro(x.cpu().numpy(), lu.solve(b * scale)) < 1e-8
