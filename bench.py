@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--bias", type=float, default=0.3)
     ap.add_argument("--cpu-sample", type=int, default=96)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--newton-steps", type=int, default=3,
+                    help="whole Newton steps timed beside the metric")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(3 launches): the known-size kernel that calibrates the "
@@ -124,17 +126,43 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     cells = ctx.ncells
     value = cells * args.steps / wall
-    # reconstruct pass (reported beside the metric)
     dxt = torch.zeros_like(s.x_trace)
     dxc = torch.empty_like(s.x_cell)
-    for _ in range(2):
-        s.ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc)
+
+    # a whole Newton step on the device: pass 1, the trace factorisation and
+    # solve (nested-dissection multifrontal LU), pass 2
+    ts = s.trace_solver()
+    sol = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
+    _, _, _, r2f = s._pattern()
+    phases = {"condense": [], "factor": [], "solve": [], "reconstruct": []}
+
+    def timed(name, fn):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        phases[name].append((e0, e1))
+
+    def newton_step():
+        timed("condense", step)
+        timed("factor", lambda: ts.factor(vals))
+        timed("solve", lambda: ts.solve(rhs, sol))
+        dxt.zero_()
+        dxt[r2f] = sol
+        timed("reconstruct", lambda: s.ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc))
+
+    newton_step()
     torch.cuda.synchronize()
-    r0 = time.perf_counter()
-    for _ in range(args.steps):
-        s.ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc)
+    for v in phases.values():
+        v.clear()
+    n_newton = max(1, min(args.steps, args.newton_steps))
+    w0 = time.perf_counter()
+    for _ in range(n_newton):
+        newton_step()
     torch.cuda.synchronize()
-    recon_ms = (time.perf_counter() - r0) * 1e3 / args.steps
+    newton_ms = (time.perf_counter() - w0) * 1e3 / n_newton
+    phase_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in phases.items()}
 
     launch_ms = kernel_ms / 2.0
     cells_per_launch = cells / 2.0
@@ -163,7 +191,13 @@ def main():
         "hbm": {"achieved_GBs": achieved_gbs, "peak_GBs": HBM_PEAK_GBS,
                 "frac": achieved_gbs / HBM_PEAK_GBS, "bytes_per_cell": bytes_per_cell(N, T)},
         "kernel_ms_per_launch": launch_ms,
-        "reconstruct_ms": recon_ms,
+        "newton_step": {
+            "ms": newton_ms, "steps": n_newton,
+            "phase_ms": phase_ms,
+            "trace_solver": {"levels": ts.levels, "max_front": ts.max_front,
+                             "front_GB": ts.bytes / 1e9, "factor_GFLOP": ts.flops / 1e9,
+                             "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12},
+        },
     }
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
