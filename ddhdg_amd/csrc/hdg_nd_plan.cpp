@@ -125,15 +125,51 @@ namespace hdg
           return false;
         }
 
-    // groups by height
+    // groups: nodes of one height with one front shape (eliminated and
+    // interface faces); shapes shared by fewer than kMinClass nodes of a
+    // height go to one padded group of that height
+    constexpr int kMinClass = 32;
     int hmax = 0;
     for (const Node &N : nodes)
       hmax = std::max(hmax, N.height);
-    P.groups.resize(hmax + 1);
+    std::vector<int> group_of_node(nodes.size(), -1);
+    for (int h = 0; h <= hmax; ++h)
+      {
+        std::vector<std::pair<std::pair<int, int>, int>> shape; // ((S, I), node)
+        for (int id = 0; id < (int)nodes.size(); ++id)
+          if (nodes[id].height == h)
+            shape.push_back({{(int)nodes[id].elim.size(), (int)nodes[id].intf.size()}, id});
+        std::sort(shape.begin(), shape.end());
+        int rest = -1;
+        for (size_t a = 0; a < shape.size();)
+          {
+            size_t e = a;
+            while (e < shape.size() && shape[e].first == shape[a].first)
+              ++e;
+            int g;
+            if ((int)(e - a) >= kMinClass || (a == 0 && e == shape.size()))
+              {
+                g = (int)P.groups.size();
+                P.groups.emplace_back();
+              }
+            else
+              {
+                if (rest < 0)
+                  {
+                    rest = (int)P.groups.size();
+                    P.groups.emplace_back();
+                  }
+                g = rest;
+              }
+            for (size_t k = a; k < e; ++k)
+              group_of_node[shape[k].second] = g;
+            a = e;
+          }
+      }
     for (int id = 0; id < (int)nodes.size(); ++id)
       {
         Node    &N = nodes[id];
-        NDGroup &G = P.groups[N.height];
+        NDGroup &G = P.groups[group_of_node[id]];
         N.bidx     = (int)G.nodes.size();
         G.nodes.push_back(id);
         G.S = std::max(G.S, (int)N.elim.size() * tdf);
@@ -163,7 +199,7 @@ namespace hdg
     for (int id = 0; id < (int)nodes.size(); ++id)
       for (size_t i = 0; i < nodes[id].intf.size(); ++i)
         intf_pos[id][nodes[id].intf[i]] = (int)i;
-    auto group_of = [&](int id) -> const NDGroup & { return P.groups[nodes[id].height]; };
+    auto group_of = [&](int id) -> const NDGroup & { return P.groups[group_of_node[id]]; };
     auto front_index = [&](int id, long long f) -> long long {
       if (elim_node[f] == id)
         return (long long)elim_pos[f] * tdf;

@@ -11,11 +11,13 @@
 // assembled into the front of whichever of the two eliminating nodes comes
 // first; both faces are in that front.
 //
-// Nodes of one height form a group; a group's fronts are padded to one size
-// n = S + I (S eliminated dofs, I interface dofs), stored column-major one
-// after the other: group g's node b has its front at fbase + b*n*n and its
-// work vector at wbase + b*n.  Padded eliminated dofs carry a unit diagonal;
-// padded interface dofs stay zero and are not passed up.
+// Nodes of one height and one front shape form a group (rare shapes of a
+// height share one group); a group's fronts are padded to one size n = S + I
+// (S eliminated dofs, I interface dofs), stored column-major one after the
+// other: group g's node b has its front at fbase + b*n*n and its work vector
+// at wbase + b*n.  Groups are ordered by height, so every child is factored
+// before its parent.  Padded eliminated dofs carry a unit diagonal; padded
+// interface dofs stay zero and are not passed up.
 #pragma once
 #include <string>
 #include <vector>
@@ -47,7 +49,7 @@ namespace hdg
     long long nnodes = 0;
     int       max_front = 0;
     double    flops = 0;
-    std::vector<NDGroup>   groups; // leaves first
+    std::vector<NDGroup>   groups; // by height, leaves first
     std::vector<long long> map;    // CSR entry -> front offset
     std::vector<long long> pad;    // front offsets of padded unit diagonals
     std::vector<long long> wmap;   // trace dof -> work-vector offset

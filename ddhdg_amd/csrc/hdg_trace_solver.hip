@@ -110,6 +110,62 @@ namespace
       }
   }
 
+  // Batched triangular solves for fronts with a small eliminated block
+  // (S <= TRSM_SMAX), one launch per level.  A lane owns one row (right solve)
+  // or one column (left solve) of the off-diagonal block; the values it has
+  // solved stay in LDS, the triangular factor is read as wave-uniform
+  // broadcasts.
+  constexpr int TRSM_SMAX = 160;
+
+  // B := B U^-1, B = F21 (I x S, rows S.. of columns 0..S-1), U = upper(F11)
+  __global__ void __launch_bounds__(64)
+    k_trsm_right_upper(double *F, long long fbase, int n, int S, int I, int batch, int yoff)
+  {
+    extern __shared__ double xs[]; // S x 64
+    const int b = blockIdx.y + yoff;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (b >= batch)
+      return;
+    const double *U   = F + fbase + (long long)b * n * n;
+    double       *B   = F + fbase + (long long)b * n * n + S;
+    const bool    act = i < I;
+    for (int j = 0; j < S; ++j)
+      {
+        const double *Uj  = U + (long long)j * n;
+        double        acc = act ? B[i + (long long)j * n] : 0.0;
+        for (int l = 0; l < j; ++l)
+          acc -= xs[l * 64 + threadIdx.x] * Uj[l];
+        const double x              = acc / Uj[j];
+        xs[j * 64 + threadIdx.x]    = x;
+        if (act)
+          B[i + (long long)j * n] = x;
+      }
+  }
+
+  // B := L^-1 B, B = F12 (S x I, rows 0..S-1 of columns S..), L = unit lower(F11)
+  __global__ void __launch_bounds__(64)
+    k_trsm_left_lower_unit(double *F, long long fbase, int n, int S, int I, int batch, int yoff)
+  {
+    extern __shared__ double ys[]; // S x 64
+    const int b = blockIdx.y + yoff;
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    if (b >= batch)
+      return;
+    const double *L   = F + fbase + (long long)b * n * n;
+    double       *B   = F + fbase + (long long)b * n * n + (long long)S * n;
+    const bool    act = c < I;
+    double       *Bc  = B + (long long)(act ? c : 0) * n;
+    for (int j = 0; j < S; ++j)
+      {
+        double acc = act ? Bc[j] : 0.0;
+        for (int l = 0; l < j; ++l)
+          acc -= L[j + (long long)l * n] * ys[l * 64 + threadIdx.x];
+        ys[j * 64 + threadIdx.x] = acc;
+        if (act)
+          Bc[j] = acc;
+      }
+  }
+
   __global__ void k_gather_rhs(const double *__restrict__ b, const long long *__restrict__ wmap,
                                long long ndof, double *__restrict__ w)
   {
@@ -270,6 +326,17 @@ int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
                               hipGetErrorString(e),
                             -3);
     }
+  if (rc == 0)
+    {
+      const int  lds = TRSM_SMAX * 64 * (int)sizeof(double);
+      hipError_t e   = hipFuncSetAttribute((const void *)k_trsm_right_upper,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void *)k_trsm_left_lower_unit,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+      if (e != hipSuccess)
+        rc = hdg::set_error(std::string("hdg_trace_solver_create: ") + hipGetErrorString(e), -3);
+    }
   if (rc == 0 && rocblas_create_handle(&T->h) != rocblas_status_success)
     rc = hdg::set_error("hdg_trace_solver_create: rocblas_create_handle failed", -3);
   if (rc != 0)
@@ -349,18 +416,33 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
                                    dim3((G.I + 63) / 64, std::min(YCHUNK, G.batch - y0)),
                                    dim3(64), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, T->d_ipiv,
                                    G.pbase, G.batch, y0);
-              st = rocblas_dtrsm_strided_batched(T->h, rocblas_side_left, rocblas_fill_lower,
-                                                 rocblas_operation_none, rocblas_diagonal_unit,
-                                                 G.S, G.I, &one, F, G.n,
-                                                 (rocblas_stride)G.n * G.n,
-                                                 F + (long long)G.S * G.n, G.n,
-                                                 (rocblas_stride)G.n * G.n, G.batch);
-              if (st == rocblas_status_success)
-                st = rocblas_dtrsm_strided_batched(T->h, rocblas_side_right, rocblas_fill_upper,
-                                                   rocblas_operation_none,
-                                                   rocblas_diagonal_non_unit, G.I, G.S, &one, F,
-                                                   G.n, (rocblas_stride)G.n * G.n, F + G.S, G.n,
-                                                   (rocblas_stride)G.n * G.n, G.batch);
+              if (G.S <= TRSM_SMAX)
+                {
+                  // rocBLAS's strided-batched trsm above 64 loops over the
+                  // batch; these are one launch per level
+                  const size_t lds = (size_t)G.S * 64 * sizeof(double);
+                  for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+                    {
+                      const dim3 grid((G.I + 63) / 64, std::min(YCHUNK, G.batch - y0));
+                      hipLaunchKernelGGL(k_trsm_left_lower_unit, grid, dim3(64), lds, s, T->d_F,
+                                         G.fbase, G.n, G.S, G.I, G.batch, y0);
+                      hipLaunchKernelGGL(k_trsm_right_upper, grid, dim3(64), lds, s, T->d_F,
+                                         G.fbase, G.n, G.S, G.I, G.batch, y0);
+                    }
+                }
+              else
+                {
+                  st = rocblas_dtrsm_strided_batched(
+                    T->h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                    rocblas_diagonal_unit, G.S, G.I, &one, F, G.n, (rocblas_stride)G.n * G.n,
+                    F + (long long)G.S * G.n, G.n, (rocblas_stride)G.n * G.n, G.batch);
+                  if (st == rocblas_status_success)
+                    st = rocblas_dtrsm_strided_batched(
+                      T->h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                      rocblas_diagonal_non_unit, G.I, G.S, &one, F, G.n,
+                      (rocblas_stride)G.n * G.n, F + G.S, G.n, (rocblas_stride)G.n * G.n,
+                      G.batch);
+                }
               if (st == rocblas_status_success)
                 st = rocblas_dgemm_strided_batched(
                   T->h, rocblas_operation_none, rocblas_operation_none, G.I, G.I, G.S, &mone,
