@@ -497,6 +497,21 @@ int hdg_cell_systems(hdg_ctx *C, const double *x_cell, const double *x_trace, in
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_cell_systems launch");
 }
 
+int hdg_terminal_current(hdg_ctx *C, const double *x_cell, const double *x_trace,
+                         int boundary_id, double *out, void *stream)
+{
+  if (!C || !x_cell || !x_trace || !out)
+    return fail("hdg_terminal_current: null argument");
+  if (boundary_id < 0 || boundary_id >= 2 * C->d)
+    return fail("hdg_terminal_current: boundary id out of range", -2);
+  hdg::KParams p = params(C, 0);
+  p.xc           = x_cell;
+  p.xt           = x_trace;
+  hipError_t e   = hdg::launch_current(C->d, C->nb, C->nf, C->qf, p, boundary_id, out,
+                                       (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "hdg_terminal_current launch");
+}
+
 int hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face, unsigned seed,
                             const double *csr_vals, double out[3])
 {

@@ -56,6 +56,10 @@ namespace hdg
   bool       fast_supported(int dim, int deg, int ne, int thermo, const int en[3]);
   hipError_t launch_fast_condense(int deg, const KParams &p, const int *cells, int ncell,
                                   int color, int mode, hipStream_t s);
+  // terminal current through boundary id bid: out[0] = integral of the
+  // numerical flux of n, out[1] of p (device, dimensionless)
+  hipError_t launch_current(int dim, int nb, int nf, int qf, const KParams &p, int bid,
+                            double *out, hipStream_t s);
   // register-resident pass 2 (same shapes as launch_fast_condense)
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s);
 } // namespace hdg

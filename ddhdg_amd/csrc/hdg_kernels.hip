@@ -734,6 +734,86 @@ namespace hdg
     }
   } // namespace
 
+  namespace
+  {
+    // Terminal current: the HDG numerical fluxes of n and p integrated over
+    // the faces of one boundary id (outward normal).  The fluxes are the ones
+    // the trace equations make single-valued (add_tc_matrix_terms_to_tt_rhs,
+    // include/assembling_methods.template.h:1311-1447, use_cell):
+    //   F_n =  n mu_n E.nu - U_T mu_n W_n.nu - tau_n (n - lambda_n)
+    //   F_p = -p mu_p E.nu - U_T mu_p W_p.nu - tau_p (p - lambda_p)
+    // One workgroup; fixed-order reduction (deterministic).
+    constexpr int CUR_T = 256;
+
+    __global__ void __launch_bounds__(CUR_T)
+      current_kernel(KParams p, int dim, int nb, int nf, int qf, int bid, double *out)
+    {
+      __shared__ double red[2][CUR_T];
+      const int a = bid >> 1, side = bid & 1;
+      const int o1 = (a + 1) % 3, o2 = (a + 2) % 3;
+      const long long cnt = (long long)p.n[o1] * p.n[o2];
+      const double    nu  = side ? 1.0 : -1.0;
+      const int       ncd = 3 * (dim + 1) * nb;
+      const DevTables &tb = p.tb;
+      double sn = 0, sp = 0;
+      for (long long e = threadIdx.x; e < cnt; e += CUR_T)
+        {
+          int ic[3];
+          ic[a]  = side ? p.n[a] - 1 : 0;
+          ic[o1] = (int)(e % p.n[o1]);
+          ic[o2] = (int)(e / p.n[o1]);
+          const long long cell = ic[0] + (long long)p.n[0] * (ic[1] + (long long)p.n[1] * ic[2]);
+          const long long F    = face_of(p, ic[0], ic[1], ic[2], bid);
+          const double   *x    = p.xc + cell * ncd;
+          const double   *xt   = p.xt + F * 3 * nf;
+          for (int s = 0; s < qf; ++s)
+            {
+              const double *ph = tb.phif + (bid * qf + s) * nb;
+              double        u[3] = {0, 0, 0}, w[3] = {0, 0, 0}, lam[3] = {0, 0, 0};
+              for (int c = 0; c < 3; ++c)
+                {
+                  const double *xc = x + c * (dim + 1) * nb;
+                  for (int j = 0; j < nb; ++j)
+                    {
+                      u[c] += xc[dim * nb + j] * ph[j];
+                      w[c] += xc[a * nb + j] * ph[j];
+                    }
+                  w[c] *= nu;
+                  for (int m = 0; m < nf; ++m)
+                    lam[c] += xt[c * nf + m] * tb.xi[s * nf + m];
+                }
+              const double JW = tb.JxWf[bid * qf + s];
+              sn += JW * (u[1] * p.mun * w[0] - p.UT * p.mun * w[1] - p.tau[1] * (u[1] - lam[1]));
+              sp += JW * (-u[2] * p.mup * w[0] - p.UT * p.mup * w[2] - p.tau[2] * (u[2] - lam[2]));
+            }
+        }
+      red[0][threadIdx.x] = sn;
+      red[1][threadIdx.x] = sp;
+      __syncthreads();
+      for (int off = CUR_T / 2; off > 0; off >>= 1)
+        {
+          if ((int)threadIdx.x < off)
+            {
+              red[0][threadIdx.x] += red[0][threadIdx.x + off];
+              red[1][threadIdx.x] += red[1][threadIdx.x + off];
+            }
+          __syncthreads();
+        }
+      if (threadIdx.x == 0)
+        {
+          out[0] = red[0][0];
+          out[1] = red[1][0];
+        }
+    }
+  } // namespace
+
+  hipError_t launch_current(int dim, int nb, int nf, int qf, const KParams &p, int bid,
+                            double *out, hipStream_t s)
+  {
+    hipLaunchKernelGGL(current_kernel, dim3(1), dim3(CUR_T), 0, s, p, dim, nb, nf, qf, bid, out);
+    return hipGetLastError();
+  }
+
   bool supported(int dim, int deg)
   {
     return (dim == 2 && deg >= 1 && deg <= 3) || (dim == 3 && deg == 1);

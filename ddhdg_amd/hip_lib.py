@@ -27,7 +27,7 @@ EXPORTS = [
     "hdg_cell_systems", "hdg_set_kernel_variant",
     "hdg_trace_solver_create", "hdg_trace_solver_destroy", "hdg_trace_solver_info",
     "hdg_trace_factor", "hdg_trace_factor_status", "hdg_trace_solve",
-    "hdg_trace_plan_selftest",
+    "hdg_trace_plan_selftest", "hdg_terminal_current",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -84,6 +84,7 @@ def lib():
             "hdg_trace_factor_status": (i, [v, v]),
             "hdg_trace_solve": (i, [v, v, v, v]),
             "hdg_trace_plan_selftest": (i, [i, ip, i, ct.c_uint, dp, dp]),
+            "hdg_terminal_current": (i, [v, v, v, i, v, v]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -199,6 +200,12 @@ class HdgContext:
         _check(self.L.hdg_cell_systems(self.ctx, _ptr(x_cell), _ptr(x_trace), int(thermodyn),
                                        _ptr(cells), int(cells.numel()), _ptr(S), _ptr(g),
                                        _stream()))
+
+    def terminal_current(self, x_cell, x_trace, boundary_id, out):
+        """out (device, 2): integrals of the numerical fluxes of n and p over
+        the faces of boundary_id (dimensionless)."""
+        _check(self.L.hdg_terminal_current(self.ctx, _ptr(x_cell), _ptr(x_trace),
+                                           int(boundary_id), _ptr(out), _stream()))
 
 
 class TraceSolver:

@@ -333,6 +333,23 @@ class NPSolver:
                 break
         return NonlinearIterationResults(converged, step, upd_cell_norm)
 
+    def current_scale(self):
+        """dimensionless flux integral -> physical current: q mu_mag N V_rf
+        l^(dim-2) with V_rf = k_B T_mag / q (A per metre of depth in 2D)."""
+        a = self.adim
+        return (a.mobility_magnitude * a.doping_magnitude * KB * a.temperature_magnitude *
+                a.scale_length ** (self.mesh.dim - 2))
+
+    def terminal_current(self, boundary_id):
+        """Current leaving the device through the faces of boundary_id, from
+        the HDG numerical fluxes of n and p on those faces (the reference
+        has no contact-current integral: a quantity this build defines)."""
+        t = self.torch
+        out = t.empty(2, dtype=t.float64, device=self.device)
+        self.ctx.terminal_current(self.x_cell, self.x_trace, boundary_id, out)
+        fn, fp = out.cpu().numpy()
+        return float(fn - fp) * self.current_scale()
+
     def run(self, absolute_tol=None, relative_tol=None, max_number_of_iterations=None, log=None):
         nl = self.parameters.nonlinear_parameters
         return self._private_run(

@@ -171,6 +171,20 @@ int hdg_cell_systems(hdg_ctx *ctx, const double *x_cell,
                      const double *x_trace, int thermodyn, const int *cells,
                      int ncell, double *S, double *g, void *stream);
 
+/* Terminal current through the faces of one boundary id (outward normal):
+ * out (device, 2 doubles) receives the integrals of the HDG numerical fluxes
+ *   F_n =  n mu_n E.nu - U_T mu_n W_n.nu - tau_n (n - lambda_n)
+ *   F_p = -p mu_p E.nu - U_T mu_p W_p.nu - tau_p (p - lambda_p)
+ * in dimensionless units; the physical current leaving through the boundary
+ * is mu_mag N k_B T_mag l^(dim-2) (out[0] - out[1]) (A per metre of depth in
+ * 2D, A in 3D).  The reference has no contact-current integral (it projects
+ * Jn, Jp fields, src/derivative_quantities.cpp:1080-1115); the fluxes are the
+ * ones of its trace equations (include/assembling_methods.template.h:
+ * 1311-1447). */
+int hdg_terminal_current(hdg_ctx *ctx, const double *x_cell,
+                         const double *x_trace, int boundary_id, double *out,
+                         void *stream);
+
 /* ---- Global trace solve ------------------------------------------------
  * Sparse direct LU of the restricted trace system, on the GPU: a nested-
  * dissection multifrontal factorisation over the structured cell grid

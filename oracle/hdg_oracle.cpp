@@ -1967,6 +1967,42 @@ double orc_pass2(orc_ctx *C, int thermo, int nthreads,
   return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// Terminal current (a quantity this build defines; the reference has no
+// contact-current integral): the numerical fluxes of its trace equations
+// (add_tc_matrix_terms_to_tt_rhs, template.h:1311-1447, use_cell)
+//   F_n =  n mu_n E.nu - U_T mu_n W_n.nu - tau_n (n - lambda_n)
+//   F_p = -p mu_p E.nu - U_T mu_p W_p.nu - tau_p (p - lambda_p)
+// integrated over the faces of one boundary id, outward normal.
+int orc_terminal_current(orc_ctx *C, int bid, double out[2])
+{
+  if (bid < 0 || bid >= 2 * C->P.dim)
+    {
+      g_err = "oracle: boundary id out of range";
+      return -1;
+    }
+  Work   W(C);
+  double sn = 0, sp = 0;
+  for (long long cell = 0; cell < C->ncells; ++cell)
+    {
+      if (boundary_id(C, cell, bid) != bid)
+        continue;
+      W.prepare_face(cell, bid, false, nullptr);
+      for (int s = 0; s < C->Qf; ++s)
+        {
+          const double En  = dot(W.fE0[s], W.normal, C->P.dim);
+          const double Wnn = dot(W.fWn0[s], W.normal, C->P.dim);
+          const double Wpn = dot(W.fWp0[s], W.normal, C->P.dim);
+          sn += W.fJxW[s] * (W.fn0[s] * C->mun_r * En - W.UTf[s] * C->mun_r * Wnn -
+                             C->tau_r[CN] * (W.fn0[s] - W.trn0[s]));
+          sp += W.fJxW[s] * (-W.fp0[s] * C->mup_r * En - W.UTf[s] * C->mup_r * Wpn -
+                             C->tau_r[CP] * (W.fp0[s] - W.trp0[s]));
+        }
+    }
+  out[0] = sn;
+  out[1] = sp;
+  return 0;
+}
+
 void orc_recombination(const orc_problem *prob, double n, double p, double rf,
                        double out[3])
 {

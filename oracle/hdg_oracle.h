@@ -119,6 +119,10 @@ double orc_pass2(orc_ctx *ctx, int thermodyn, int nthreads,
 /* recombination model of prob at one point: out = {R, dR/dn, dR/dp} with the
  * reference's compute_recombination_term / compute_derivative_of_
  * recombination_term semantics (densities rescaled by rf) */
+/* integrals of the numerical fluxes of n and p over the faces of one
+ * boundary id (outward normal, dimensionless): out[0] (n), out[1] (p) */
+int orc_terminal_current(orc_ctx *ctx, int boundary_id, double out[2]);
+
 void orc_recombination(const orc_problem *prob, double n, double p, double rf,
                        double out[3]);
 

@@ -76,6 +76,7 @@ def lib():
             "orc_get_dirichlet": (None, [v, lp, dp]),
             "orc_pass2": (d, [v, i, i, dp, dp]),
             "orc_recombination": (None, [P(OrcProblem), d, d, d, dp]),
+            "orc_terminal_current": (i, [v, i, dp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -217,6 +218,14 @@ class Oracle:
         out = np.zeros(self.ncells * self.NC)
         t = self.L.orc_pass2(self.ctx, int(thermodyn), int(nthreads), _dp(u), _dp(out))
         if t < 0:
+            raise RuntimeError(self.L.orc_last_error().decode())
+        return out
+
+    def terminal_current(self, boundary_id):
+        """integrals of the numerical fluxes of n and p over the faces of
+        boundary_id (dimensionless, outward normal)."""
+        out = np.zeros(2)
+        if self.L.orc_terminal_current(self.ctx, int(boundary_id), _dp(out)) != 0:
             raise RuntimeError(self.L.orc_last_error().decode())
         return out
 

@@ -201,3 +201,60 @@ def test_newton_reference_linearisation_matches_oracle():
     assert r_gpu.iterations == r_orc.iterations == 15
     assert np.allclose(ng, no, rtol=1e-8, atol=0)
     assert _close(s, o)
+
+
+@pytest.mark.parametrize("dim,n_cells,k", [(2, (6, 5), 2), (2, (4, 4), 1), (3, (2, 3, 2), 1)])
+def test_terminal_current_matches_oracle(dim, n_cells, k):
+    """integrals of the numerical fluxes of n and p over every boundary id,
+    GPU against the oracle on the same perturbed state"""
+    t = _torch()
+    print(f"case terminal current dim={dim} n={n_cells} k={k}", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(dim, n_cells, k)
+    s = NPSolver(prob, params, adim)
+    o = H.oracle_for(prob, params, adim)
+    cell, tr = H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=13)
+    s.set_state(cell, tr)
+    o.set_state(cell, tr)
+    out = t.empty(2, dtype=t.float64, device="cuda")
+    for bid in range(2 * dim):
+        s.ctx.terminal_current(s.x_cell, s.x_trace, bid, out)
+        g = out.cpu().numpy()
+        r = o.terminal_current(bid)
+        print(f"  bid {bid}: gpu {g} oracle {r}")
+        assert np.allclose(g, r, rtol=1e-12, atol=1e-14 * np.abs(r).max())
+
+
+def test_iv_points_match_oracle():
+    """forward-bias I-V points of a resolved diode strip (128 x 1 cells,
+    degree 2): every bias solve starts both sides from the same state; equal
+    Newton iteration counts, equal states, equal terminal currents, and the
+    current conserved between the two contacts"""
+    t = _torch()
+    print("case I-V strip 128x1 k=2", flush=True)
+    s, prob, params, adim = _solver(2, (128, 1), 2)
+    s.set_enabled_components(True, False, False)
+    s.compute_local_charge_neutrality()
+    assert s._private_run(1e-10, 1e-10, 100, True).converged
+    s.project_densities_from_potential()
+    for en in ((False, True, True), (True, True, True)):
+        s.set_enabled_components(*en)
+        assert s.run().converged
+    for bias in (0.3, 0.4):
+        configs.set_bias(s, bias)
+        o = H.oracle_for(prob, params, adim)
+        o.set_state(*s.get_state())
+        r_gpu = s.run()
+        r_orc = o.run_newton()
+        I_gpu = [s.terminal_current(b) for b in (0, 1)]
+        I_orc = [(o.terminal_current(b)[0] - o.terminal_current(b)[1]) * s.current_scale()
+                 for b in (0, 1)]
+        print(f"  {bias} V: iterations gpu={r_gpu.iterations} oracle={r_orc.iterations} "
+              f"I gpu={I_gpu} oracle={I_orc} A/m")
+        assert r_gpu.converged and r_orc.converged
+        assert r_gpu.iterations == r_orc.iterations
+        assert _close(s, o)
+        # the contact current is a small difference of O(1) drift and
+        # diffusion fluxes (dimensionless): it agrees to their rounding
+        assert np.allclose(I_gpu, I_orc, rtol=1e-8, atol=1e-13 * s.current_scale())
+        assert abs(I_gpu[0] + I_gpu[1]) < 1e-5 * abs(I_gpu[1]) + 1e-14 * s.current_scale()

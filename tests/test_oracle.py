@@ -253,3 +253,40 @@ def test_equilibrium_jacobian_consistent_with_residual():
         dc, dt = (cp - cm) / (2 * h), (tp - tm) / (2 * h)
         assert H.rel_fro(dc, -(A @ vu + CT @ vl)) < 1e-8
         assert H.rel_fro(dt, -(-TC @ vu + TT @ vl)) < 1e-8
+
+
+def _strip_diode_iv(biases, nx=128, k=2):
+    """a resolved one-cell-high diode strip through the staged workflow, then
+    the given biases; terminal currents (dimensionless) at both contacts"""
+    prob, params, adim = configs.diode(2, (nx, 1), k, bias=0.0)
+    o = H.oracle_for(prob, params, adim)
+    o.set_state(*H.equilibrium_guess(prob, adim, k))
+    assert o.run_newton(enabled=(True, False, False), thermodyn=True, max_it=40).converged
+    assert o.run_newton(enabled=(False, True, True), max_it=60).converged
+    assert o.run_newton(enabled=(True, True, True), max_it=60).converged
+    out = [(0.0, o.terminal_current(0), o.terminal_current(1))]
+    for b in biases:
+        pb, _, _ = configs.diode(2, (nx, 1), k, bias=b)
+        ob = H.oracle_for(pb, params, adim)
+        ob.set_state(*o.get_state())
+        assert ob.run_newton(enabled=(True, True, True), max_it=60).converged
+        out.append((b, ob.terminal_current(0), ob.terminal_current(1)))
+        o = ob
+    return out
+
+
+def test_terminal_current_physics():
+    """the terminal current (integral of the HDG numerical fluxes of n and p
+    over a contact): zero at thermal equilibrium, conserved between the two
+    contacts, and exponential in the forward bias with an ideality factor
+    near 1 (diffusion current of a short diode)"""
+    res = _strip_diode_iv((0.3, 0.4))
+    tot = [(b, I0[0] - I0[1], I1[0] - I1[1]) for b, I0, I1 in res]
+    assert abs(tot[0][1]) < 1e-11 and abs(tot[0][2]) < 1e-11
+    for b, i0, i1 in tot[1:]:
+        # conserved to the Newton tolerance: relative, with a floor at the
+        # rounding level of the flux integrals
+        assert abs(i0 + i1) < 1e-5 * abs(i1) + 1e-14, (b, i0, i1)
+    ut = 1.380649e-23 * 300.0 / 1.602176634e-19
+    eta = 0.1 / (ut * np.log(tot[2][2] / tot[1][2]))
+    assert 1.0 < eta < 1.1, eta
