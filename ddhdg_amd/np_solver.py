@@ -298,40 +298,13 @@ class NPSolver:
         L-infinity update criterion against the state before the update.
         Every vector stays on the device; one host synchronisation per
         iteration reads the norms."""
-        t = self.torch
-        _, _, dirs, r2f = self._pattern()
-        ctx = self.ctx
-        vals = t.empty(ctx.nnz, dtype=t.float64, device=self.device)
-        rhs = t.empty(ctx.ntrace, dtype=t.float64, device=self.device)
-        sol = t.empty(ctx.ntrace, dtype=t.float64, device=self.device)
-        dirv = t.zeros(ctx.ntrace, dtype=t.float64, device=self.device)
-        upd_trace = t.zeros_like(self.x_trace)
-        upd_cell = t.zeros_like(self.x_cell)
-        dirs_t = t.as_tensor(dirs, device=self.device)
-        alpha = self.parameters.nonlinear_parameters.alpha
-        converged, step, upd_cell_norm = False, 0, 0.0
-        for step in range(1, max_it + 1 if max_it >= 0 else 1 << 30):
-            ctx.condense(self.x_cell, self.x_trace, thermodyn, vals, rhs, dirv)
-            self._solve_trace(vals, rhs, sol)
-            sol[dirs_t] = dirv[dirs_t]
-            upd_trace.zero_()
-            upd_trace[r2f] = sol
-            ctx.reconstruct(self.x_cell, self.x_trace, upd_trace, thermodyn, upd_cell)
-            norms = t.stack([upd_cell.abs().max(), upd_trace.abs().max(),
-                             self.x_cell.abs().max(), self.x_trace.abs().max()]).cpu().numpy()
-            upd_cell_norm = float(norms[0])
-            upd_norm = max(upd_cell_norm, float(norms[1]))
-            cur_norm = max(float(norms[2]), float(norms[3]))
-            if not np.isfinite(upd_norm) and self.parameters.trace_solver == "gpu":
-                self.trace_solver().factor(vals, check=True)
-            self.x_cell.add_(upd_cell, alpha=alpha)
-            self.x_trace.add_(upd_trace, alpha=alpha)
-            if log is not None:
-                log(step, upd_norm, cur_norm)
-            if upd_norm < abs_tol or upd_norm < rel_tol * cur_norm:
-                converged = True
+        it = NewtonIteration(self, thermodyn, abs_tol, rel_tol, max_it, log)
+        while True:
+            it.launch()
+            self.torch.cuda.current_stream().synchronize()
+            if it.finish() != "continue":
                 break
-        return NonlinearIterationResults(converged, step, upd_cell_norm)
+        return it.result()
 
     def current_scale(self):
         """dimensionless flux integral -> physical current: q mu_mag N V_rf
@@ -375,3 +348,67 @@ class NPSolver:
         self.project_densities_from_potential(0.0, 0.0)
         self.set_enabled_components(*saved)
         return res
+
+
+class NewtonIteration:
+    """One Newton solve of an NPSolver split into device-side iterations, so
+    that a host loop can interleave several solvers (one stream each):
+    launch() enqueues a whole iteration on the current stream (pass 1, trace
+    factorisation and solve, Dirichlet overwrite, pass 2, update, norms into
+    pinned host memory); after the stream has been synchronised, finish()
+    applies the reference's L-infinity criterion (src/np_solver.cpp:
+    1521-1560) and returns "converged", "continue" or "failed" (iteration
+    limit)."""
+
+    def __init__(self, solver, thermodyn, abs_tol, rel_tol, max_it, log=None):
+        t = solver.torch
+        self.s, self.t = solver, t
+        self.thermodyn, self.abs_tol, self.rel_tol = thermodyn, abs_tol, rel_tol
+        self.max_it = max_it if max_it >= 0 else 1 << 30
+        self.log = log
+        ctx, dev = solver.ctx, solver.device
+        _, _, dirs, self.r2f = solver._pattern()
+        self.vals = t.empty(ctx.nnz, dtype=t.float64, device=dev)
+        self.rhs = t.empty(ctx.ntrace, dtype=t.float64, device=dev)
+        self.sol = t.empty(ctx.ntrace, dtype=t.float64, device=dev)
+        self.dirv = t.zeros(ctx.ntrace, dtype=t.float64, device=dev)
+        self.upd_trace = t.zeros_like(solver.x_trace)
+        self.upd_cell = t.zeros_like(solver.x_cell)
+        self.dirs = t.as_tensor(dirs, device=dev)
+        self.norms = t.zeros(4, dtype=t.float64).pin_memory()
+        self.alpha = solver.parameters.nonlinear_parameters.alpha
+        self.step, self.converged, self.upd_cell_norm = 0, False, 0.0
+
+    def launch(self):
+        s, t = self.s, self.t
+        s.ctx.condense(s.x_cell, s.x_trace, self.thermodyn, self.vals, self.rhs, self.dirv)
+        s._solve_trace(self.vals, self.rhs, self.sol)
+        self.sol[self.dirs] = self.dirv[self.dirs]
+        self.upd_trace.zero_()
+        self.upd_trace[self.r2f] = self.sol
+        s.ctx.reconstruct(s.x_cell, s.x_trace, self.upd_trace, self.thermodyn, self.upd_cell)
+        dn = t.stack([self.upd_cell.abs().max(), self.upd_trace.abs().max(),
+                      s.x_cell.abs().max(), s.x_trace.abs().max()])
+        self.norms.copy_(dn, non_blocking=True)
+        s.x_cell.add_(self.upd_cell, alpha=self.alpha)
+        s.x_trace.add_(self.upd_trace, alpha=self.alpha)
+        self.step += 1
+
+    def finish(self):
+        n_ = self.norms.numpy()
+        self.upd_cell_norm = float(n_[0])
+        upd_norm = max(self.upd_cell_norm, float(n_[1]))
+        cur_norm = max(float(n_[2]), float(n_[3]))
+        if not np.isfinite(upd_norm) and self.s.parameters.trace_solver == "gpu":
+            self.s.trace_solver().factor(self.vals, check=True)
+        if self.log is not None:
+            self.log(self.step, upd_norm, cur_norm)
+        if upd_norm < self.abs_tol or upd_norm < self.rel_tol * cur_norm:
+            self.converged = True
+            return "converged"
+        if not np.isfinite(upd_norm) or self.step >= self.max_it:
+            return "failed"
+        return "continue"
+
+    def result(self):
+        return NonlinearIterationResults(self.converged, self.step, self.upd_cell_norm)

@@ -1,0 +1,218 @@
+"""Continuous-batching server for I-V sweep requests on one GPU.
+
+A request is a diode I-V sweep: a mesh, a degree, a doping profile and a list
+of bias points.  It runs the reference's diode workflow (thermodynamic
+equilibrium, densities from the potential, n and p with V frozen, the coupled
+system; examples/residual.ipynb cells 4-17) and then one coupled Newton solve
+per bias point, the Newton state kept on the device from one point to the
+next; each converged point records the terminal current at both contacts.
+
+Scheduling: requests wait in arrival order and are admitted while a slot is
+free and their device-memory estimate (context tables, trace-solver fronts,
+Newton buffers) fits the HBM budget, so requests arriving while others run
+join the running set as soon as slots and memory free up.  Every tick launches
+one Newton iteration of every active request, each on the stream of its slot
+(default priority), synchronises once and lets every request advance its own
+state machine.  Requests share no buffers and every kernel is deterministic,
+so a request's results are the same alone and in any mix.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import configs
+from .np_solver import NewtonIteration, NPSolver
+
+
+@dataclass
+class IVRequest:
+    rid: int
+    dim: int
+    n_cells: tuple
+    degree: int
+    biases: tuple
+    nd: float = 1e23
+    na: float = 1e23
+    junction: float = 0.5
+    arrival: float = 0.0          # seconds after the server starts
+
+
+@dataclass
+class IVResult:
+    rid: int
+    biases: list
+    currents: list                # A (per metre of depth in 2D), contact at x = 1
+    iterations: list              # Newton iterations per stage
+    converged: bool
+    arrival: float
+    admitted: float
+    finished: float
+    state: tuple = field(default=None, repr=False)
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def memory_estimate(req: IVRequest) -> int:
+    """device bytes a request holds while it runs: cell and trace state and
+    updates, the trace matrix, the trace solver's fronts (closed form of the
+    nested-dissection plan's size, from the face count), tables."""
+    k, d = req.degree, req.dim
+    n = list(req.n_cells) + [1] * (3 - len(req.n_cells))
+    ncells = int(np.prod(n[:d]))
+    nb, nf = (k + 1) ** d, (k + 1) ** (d - 1)
+    nfaces = sum(int(np.prod([n[b] + (1 if a == b else 0) for b in range(d)])) for a in range(d))
+    NC = 3 * (d + 1) * nb
+    tdf = 3 * nf
+    nnz = nfaces * (4 * d - 1) * tdf * tdf
+    # fronts: about 11 dense (face-block) entries per trace dof and level in 2D,
+    # bounded above by the measured plans; 3D fronts grow as ndof^(4/3)
+    ndof = nfaces * tdf
+    fronts = 8 * (11 * ndof * tdf * np.log2(max(ncells, 2)) if d == 2 else 4 * ndof ** (4 / 3) * tdf)
+    return int(8 * (4 * ncells * NC + 6 * nfaces * 3 * nf + nnz) + fronts + (64 << 20))
+
+
+class _Slot:
+    """one admitted request: its solver, stream and stage machine"""
+
+    def __init__(self, req: IVRequest, t0: float, now: float, torch):
+        self.req, self.torch = req, torch
+        self.stream = torch.cuda.Stream()
+        with torch.cuda.stream(self.stream):
+            prob, params, adim = configs.diode(
+                req.dim, req.n_cells, req.degree,
+                doping=configs.step_doping(req.nd, req.na, req.junction))
+            self.solver = NPSolver(prob, params, adim)
+            s = self.solver
+            s.set_enabled_components(True, False, False)
+            s.compute_local_charge_neutrality()
+        self.stage = 0                        # 0 eq, 1 n/p, 2 coupled, 3.. biases
+        self.iterations, self.currents = [], []
+        self.result = None
+        self.admitted = now
+        self.t0 = t0
+        self.ok = True
+        self._start_stage()
+
+    def _start_stage(self):
+        s, nl = self.solver, self.solver.parameters.nonlinear_parameters
+        with self.torch.cuda.stream(self.stream):
+            if self.stage == 1:
+                s.project_densities_from_potential()
+                s.set_enabled_components(False, True, True)
+            elif self.stage == 2:
+                s.set_enabled_components(True, True, True)
+            elif self.stage >= 3:
+                configs.set_bias(s, self.req.biases[self.stage - 3])
+            # a request holds the trace solver of its current stage only
+            for key in [k for k in s._solver_cache if k != s.enabled]:
+                s._solver_cache.pop(key).close()
+            self.it = NewtonIteration(s, self.stage == 0, nl.absolute_tolerance,
+                                      nl.relative_tolerance, nl.max_number_of_iterations)
+
+    def launch(self):
+        with self.torch.cuda.stream(self.stream):
+            self.it.launch()
+
+    def advance(self, now):
+        """after the tick's synchronisation: True when the request is done"""
+        status = self.it.finish()
+        if status == "continue":
+            return False
+        self.iterations.append(self.it.step)
+        if status == "failed":
+            self.ok = False
+        elif self.stage >= 3:
+            with self.torch.cuda.stream(self.stream):
+                self.currents.append(self.solver.terminal_current(1))
+        self.stage += 1
+        if status == "failed" or self.stage >= 3 + len(self.req.biases):
+            return True
+        self._start_stage()
+        return False
+
+
+class IVServer:
+    """Admits I-V requests by HBM budget into at most max_slots concurrent
+    slots and runs them with continuous batching (see the module doc)."""
+
+    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, keep_state=False):
+        import torch
+        self.torch = torch
+        self.budget, self.max_slots = int(hbm_budget_bytes), int(max_slots)
+        self.keep_state = keep_state
+
+    def run(self, requests, clock=None):
+        """serve requests (arrival times in seconds from the start); returns
+        the results in request order and the wall time"""
+        torch = self.torch
+        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        active, done, used = [], {}, 0
+        self.max_active, self.ticks = 0, 0
+        t0 = time.perf_counter() if clock is None else clock()
+        now = lambda: (time.perf_counter() if clock is None else clock()) - t0
+        while pending or active:
+            # admission: arrived, fits the budget, a slot is free
+            while pending and len(active) < self.max_slots and pending[0].arrival <= now():
+                need = memory_estimate(pending[0])
+                if active and used + need > self.budget:
+                    break
+                req = pending.pop(0)
+                used += need
+                active.append((_Slot(req, t0, now(), torch), need))
+            if not active:
+                wait = pending[0].arrival - now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.05))
+                continue
+            self.max_active = max(self.max_active, len(active))
+            self.ticks += 1
+            for slot, _ in active:
+                slot.launch()
+            torch.cuda.synchronize()
+            t = now()
+            keep = []
+            for slot, need in active:
+                if slot.advance(t):
+                    r = slot.req
+                    state = slot.solver.get_state() if self.keep_state else None
+                    done[r.rid] = IVResult(r.rid, list(r.biases), slot.currents, slot.iterations,
+                                           slot.ok, r.arrival, slot.admitted, t, state)
+                    used -= need
+                else:
+                    keep.append((slot, need))
+            active = keep
+        wall = now()
+        return [done[r.rid] for r in sorted(requests, key=lambda r: r.rid)], wall
+
+
+def summarize(results, wall):
+    lat = np.array([r.latency for r in results])
+    ok = sum(r.converged for r in results)
+    return {"requests": len(results), "converged": int(ok), "wall_s": wall,
+            "solves_per_s": len(results) / wall if wall > 0 else 0.0,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "newton_iterations": int(sum(sum(r.iterations) for r in results))}
+
+
+def synthetic_requests(n, seed, rate, sizes=(32, 96), degrees=(1, 2), points=13, vmax=0.6):
+    """requests with Poisson arrivals at `rate` per second, log-uniform mesh
+    sides in `sizes` (2D), degrees drawn from `degrees`, log-uniform doping
+    in [1e21, 1e24] m^-3 and a junction in [0.3, 0.7]; each sweeps `points`
+    bias points over [0, vmax] V."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    for i in range(n):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        side = int(round(np.exp(rng.uniform(np.log(sizes[0]), np.log(sizes[1])))))
+        out.append(IVRequest(
+            rid=i, dim=2, n_cells=(side, side), degree=int(rng.choice(degrees)),
+            biases=tuple(np.linspace(0.0, vmax, points)[1:]),
+            nd=float(10 ** rng.uniform(21, 24)), na=float(10 ** rng.uniform(21, 24)),
+            junction=float(rng.uniform(0.3, 0.7)), arrival=t))
+    return out
