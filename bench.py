@@ -69,6 +69,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=3,
                     help="whole Newton steps timed beside the metric")
+    ap.add_argument("--serve", type=int, default=16,
+                    help="I-V sweep requests served beside the metric (0: none)")
+    ap.add_argument("--serve-rate", type=float, default=4.0)
+    ap.add_argument("--serve-slots", type=int, default=8)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(3 launches): the known-size kernel that calibrates the "
@@ -199,6 +203,20 @@ def main():
                              "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12},
         },
     }
+    if args.serve > 0:
+        # served I-V sweeps: Poisson arrivals, continuous batching, state kept
+        # on the device across the bias points (a bounded sample of the
+        # serving workload: 2D requests only)
+        from ddhdg_amd.serving import IVServer, summarize, synthetic_requests
+        reqs = synthetic_requests(args.serve, seed=20261204, rate=args.serve_rate,
+                                  sizes=(32, 96), degrees=(1, 2), points=13, vmax=0.6)
+        srv = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots)
+        res, swall = srv.run(reqs)
+        line["serving"] = {**summarize(res, swall), "max_concurrent": srv.max_active,
+                           "workload": f"{args.serve} I-V sweeps (13 points 0-0.6 V), 2D, "
+                                       f"{32}^2-{96}^2 cells log-uniform, degree 1-2, Poisson "
+                                       f"arrivals {args.serve_rate}/s, FP64",
+                           "unit": "I-V solves/s"}
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         v, nc, t = cpu_baseline(args.cpu_sample, args.degree, threads, args.bias)
