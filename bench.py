@@ -26,6 +26,10 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
+# HBM bytes one pass-1 launch (32,768 cells of the 256x256 degree-2 workload)
+# moves, from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
+# plain device copy of known size): profiles/condense_traffic_2d_k2_256x256.md
+TRAFFIC_PER_LAUNCH = {(256, 2): (292701.3 * 2.0 + 604514.7 * 1.0) * 1024}
 
 
 def flops_per_cell(N, T):
@@ -135,7 +139,7 @@ def main():
 
     # a whole Newton step on the device: pass 1, the trace factorisation and
     # solve (nested-dissection multifrontal LU), pass 2
-    ts = s.trace_solver()
+    ts = s.trace_solver() if args.newton_steps > 0 else None
     sol = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
     _, _, _, r2f = s._pattern()
     phases = {"condense": [], "factor": [], "solve": [], "reconstruct": []}
@@ -156,17 +160,19 @@ def main():
         dxt[r2f] = sol
         timed("reconstruct", lambda: s.ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc))
 
-    newton_step()
-    torch.cuda.synchronize()
-    for v in phases.values():
-        v.clear()
-    n_newton = max(1, min(args.steps, args.newton_steps))
-    w0 = time.perf_counter()
-    for _ in range(n_newton):
+    n_newton = min(args.steps, args.newton_steps)
+    newton_ms, phase_ms = None, None
+    if n_newton > 0:
         newton_step()
-    torch.cuda.synchronize()
-    newton_ms = (time.perf_counter() - w0) * 1e3 / n_newton
-    phase_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in phases.items()}
+        torch.cuda.synchronize()
+        for v in phases.values():
+            v.clear()
+        w0 = time.perf_counter()
+        for _ in range(n_newton):
+            newton_step()
+        torch.cuda.synchronize()
+        newton_ms = (time.perf_counter() - w0) * 1e3 / n_newton
+        phase_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in phases.items()}
 
     launch_ms = kernel_ms / 2.0
     cells_per_launch = cells / 2.0
@@ -191,18 +197,17 @@ def main():
                    "global_trace_dofs": ctx.ntrace, "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / FP64_PEAK_TFLOPS,
-                     "traffic": None},
+                     "traffic": TRAFFIC_PER_LAUNCH.get((args.cells, args.degree))},
         "hbm": {"achieved_GBs": achieved_gbs, "peak_GBs": HBM_PEAK_GBS,
                 "frac": achieved_gbs / HBM_PEAK_GBS, "bytes_per_cell": bytes_per_cell(N, T)},
         "kernel_ms_per_launch": launch_ms,
-        "newton_step": {
-            "ms": newton_ms, "steps": n_newton,
-            "phase_ms": phase_ms,
+    }
+    if n_newton > 0:
+        line["newton_step"] = {
+            "ms": newton_ms, "steps": n_newton, "phase_ms": phase_ms,
             "trace_solver": {"levels": ts.levels, "max_front": ts.max_front,
                              "front_GB": ts.bytes / 1e9, "factor_GFLOP": ts.flops / 1e9,
-                             "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12},
-        },
-    }
+                             "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12}}
     if args.serve > 0:
         # served I-V sweeps: Poisson arrivals, continuous batching, state kept
         # on the device across the bias points (a bounded sample of the
