@@ -120,6 +120,7 @@ namespace
       for (int c = 0; c < 3; ++c)
         p.bc[b][c] = C->phys.bc_type[b][c];
     p.sdrift = C->phys.drift_jacobian ? -1.0 : 1.0;
+    p.mixed  = C->precision;
     p.tb     = C->tb;
     p.doping = C->d_doping;
     p.dirv   = C->d_dirv;
@@ -303,6 +304,16 @@ int hdg_set_kernel_variant(hdg_ctx *C, int variant)
       !hdg::fast_supported(C->d, C->k, C->ne, 1, C->en))
     return fail("hdg_set_kernel_variant: no register-resident kernel for this shape", -2);
   C->variant = variant;
+  return 0;
+}
+
+int hdg_set_precision(hdg_ctx *C, int precision)
+{
+  if (!C)
+    return fail("hdg_set_precision: null context");
+  if (precision < 0 || precision > 1)
+    return fail("hdg_set_precision: precision must be 0 (FP64) or 1 (FP32 + refinement)", -2);
+  C->precision = precision;
   return 0;
 }
 

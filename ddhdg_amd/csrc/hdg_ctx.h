@@ -20,6 +20,7 @@ struct hdg_ctx
   long long     face_off[4], face_dims[3][3];
   int           en[3], ne, encomp[3];
   int           variant = 0; // 0 auto, 1 generic, 2 register-resident
+  int           precision = 0; // 0 FP64, 1 FP32 elimination + FP64 refinement
   hdg::HostTables ht;
   hdg::Topology   topo;  // numbering and trace block pattern
   std::vector<int> color_cells[2];

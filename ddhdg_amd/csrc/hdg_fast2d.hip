@@ -110,7 +110,10 @@ namespace hdg
 
     // PASS2: the reconstruction pass (cells = all cells in order, columns =
     // scalar dofs + rhs with the trace update folded in, output dx_cell)
-    template <int DEG, int NE, bool THERMO, bool PASS2>
+    // MIXED (pass 1 only): the elimination of the scalar block in FP32 with one
+    // FP64 refinement step; assembly, residual, refinement residual and Schur
+    // product in FP64
+    template <int DEG, int NE, bool THERMO, bool PASS2, bool MIXED = false>
     __global__ void __launch_bounds__(64 * WPB)
       __attribute__((amdgpu_waves_per_eu(PASS2 ? WPE2 : WPE)))
       fast_condense(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
@@ -630,44 +633,137 @@ namespace hdg
             R[r] *= keep;
 
           // ---- 3: Gauss-Jordan, scaled column pivoting
-          double cs = 0;
-#pragma unroll
-          for (int r = 0; r < NS; ++r)
-            cs = fmax(cs, fabs(R[r]));
-          cs           = cs > 0 ? 1.0 / cs : 0.0;
-          bool pivoted = !isU;
-#pragma unroll
-          for (int k = 0; k < NS; ++k)
+          if constexpr (!MIXED)
             {
-              // order-preserving key: non-negative doubles compare as integers
-              const double       sc  = pivoted ? 0.0 : fabs(R[k]) * cs;
-              unsigned long long key = pivoted ? 0ull
-                                               : (((unsigned long long)__double_as_longlong(sc) &
-                                                   ~63ull) | (unsigned long long)(63 - lane)) + 1ull;
-#pragma unroll
-              for (int off = 32; off > 0; off >>= 1)
-                {
-                  const unsigned long long o = __shfl_xor(key, off);
-                  key                        = o > key ? o : key;
-                }
-              const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane((unsigned)(key - 1ull))) & 63u);
-              if (lane == pl)
-                {
-                  pivoted = true;
-#pragma unroll
-                  for (int r = 0; r < NS; ++r)
-                    bc[r] = R[r];
-                }
-              if (lane == 0)
-                pivs[k] = pl;
-              WAVE_SYNC();
-              const double f = R[k] / bc[k];
+            double cs = 0;
+  #pragma unroll
+            for (int r = 0; r < NS; ++r)
+              cs = fmax(cs, fabs(R[r]));
+            cs           = cs > 0 ? 1.0 / cs : 0.0;
+            bool pivoted = !isU;
+  #pragma unroll
+            for (int k = 0; k < NS; ++k)
+              {
+                // order-preserving key: non-negative doubles compare as integers
+                const double       sc  = pivoted ? 0.0 : fabs(R[k]) * cs;
+                unsigned long long key = pivoted ? 0ull
+                                                 : (((unsigned long long)__double_as_longlong(sc) &
+                                                     ~63ull) | (unsigned long long)(63 - lane)) + 1ull;
+  #pragma unroll
+                for (int off = 32; off > 0; off >>= 1)
+                  {
+                    const unsigned long long o = __shfl_xor(key, off);
+                    key                        = o > key ? o : key;
+                  }
+                const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane((unsigned)(key - 1ull))) & 63u);
+                if (lane == pl)
+                  {
+                    pivoted = true;
+  #pragma unroll
+                    for (int r = 0; r < NS; ++r)
+                      bc[r] = R[r];
+                  }
+                if (lane == 0)
+                  pivs[k] = pl;
+                WAVE_SYNC();
+                const double f = R[k] / bc[k];
+  #pragma unroll
+                for (int r = 0; r < NS; ++r)
+                  if (r != k)
+                    R[r] -= bc[r] * f;
+                R[k] = f;
+                WAVE_SYNC();
+              }
+
+            }
+          else
+            {
+              // FP32 elimination with one FP64 refinement step: x0 from the
+              // FP32 elimination, r = b - A' x0 in FP64 (the scalar lanes keep
+              // their FP64 columns of A' untouched in R), the correction from
+              // a second FP32 elimination with the same pivots, x = x0 + dx
+              float *bcf = reinterpret_cast<float *>(bc);
+              float  R32[NS], X0[NS];
+              float  cs32 = 0.f;
 #pragma unroll
               for (int r = 0; r < NS; ++r)
-                if (r != k)
-                  R[r] -= bc[r] * f;
-              R[k] = f;
-              WAVE_SYNC();
+                {
+                  R32[r] = (float)R[r];
+                  cs32   = fmaxf(cs32, fabsf(R32[r]));
+                }
+              cs32         = cs32 > 0.f ? 1.f / cs32 : 0.f;
+              bool pivoted = !isU;
+#pragma unroll
+              for (int k = 0; k < NS; ++k)
+                {
+                  const float sc  = pivoted ? 0.f : fabsf(R32[k]) * cs32;
+                  unsigned    key = pivoted ? 0u
+                                            : ((__float_as_uint(sc) & ~63u) | (unsigned)(63 - lane)) + 1u;
+#pragma unroll
+                  for (int off = 32; off > 0; off >>= 1)
+                    {
+                      const unsigned o = __shfl_xor(key, off);
+                      key              = o > key ? o : key;
+                    }
+                  const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane(key - 1u)) & 63u);
+                  if (lane == pl)
+                    {
+                      pivoted = true;
+#pragma unroll
+                      for (int r = 0; r < NS; ++r)
+                        bcf[r] = R32[r];
+                    }
+                  if (lane == 0)
+                    pivs[k] = pl;
+                  WAVE_SYNC();
+                  const float f = R32[k] / bcf[k];
+#pragma unroll
+                  for (int r = 0; r < NS; ++r)
+                    if (r != k)
+                      R32[r] -= bcf[r] * f;
+                  R32[k] = f;
+                  WAVE_SYNC();
+                }
+#pragma unroll
+              for (int k = 0; k < NS; ++k)
+                X0[k] = R32[k];
+              const double mres = isU ? 0.0 : 1.0;
+#pragma unroll
+              for (int k = 0; k < NS; ++k)
+                {
+                  if (lane == pivs[k])
+#pragma unroll
+                    for (int r = 0; r < NS; ++r)
+                      bc[r] = R[r];
+                  WAVE_SYNC();
+                  const double xk = mres * (double)X0[k];
+#pragma unroll
+                  for (int r = 0; r < NS; ++r)
+                    R[r] -= bc[r] * xk;
+                  WAVE_SYNC();
+                }
+#pragma unroll
+              for (int r = 0; r < NS; ++r)
+                R32[r] = (float)R[r];
+#pragma unroll
+              for (int k = 0; k < NS; ++k)
+                {
+                  if (lane == pivs[k])
+#pragma unroll
+                    for (int r = 0; r < NS; ++r)
+                      bcf[r] = R32[r];
+                  WAVE_SYNC();
+                  const float f = R32[k] / bcf[k];
+#pragma unroll
+                  for (int r = 0; r < NS; ++r)
+                    if (r != k)
+                      R32[r] -= bcf[r] * f;
+                  R32[k] = f;
+                  WAVE_SYNC();
+                }
+#pragma unroll
+              for (int k = 0; k < NS; ++k)
+                R[k] = (double)X0[k] + (double)R32[k];
             }
 
           if (PASS2)
@@ -783,7 +879,7 @@ namespace hdg
         }
     }
 
-    template <int DEG, int NE, bool THERMO, bool PASS2 = false>
+    template <int DEG, int NE, bool THERMO, bool PASS2 = false, bool MIXED = false>
     hipError_t launch_fast(const KParams &p, const int *cells, int ncell, int color, int mode,
                            hipStream_t s)
     {
@@ -792,7 +888,7 @@ namespace hdg
       static bool attr = false;
       if (!attr)
         {
-          hipError_t e = hipFuncSetAttribute((const void *)fast_condense<DEG, NE, THERMO, PASS2>,
+          hipError_t e = hipFuncSetAttribute((const void *)fast_condense<DEG, NE, THERMO, PASS2, MIXED>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)bytes);
           if (e != hipSuccess)
@@ -806,7 +902,7 @@ namespace hdg
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
       const int want = (ncell + WPB - 1) / WPB;
       const int grid = want < 4 * ncu ? want : 4 * ncu;
-      hipLaunchKernelGGL((fast_condense<DEG, NE, THERMO, PASS2>), dim3(grid), dim3(64 * WPB),
+      hipLaunchKernelGGL((fast_condense<DEG, NE, THERMO, PASS2, MIXED>), dim3(grid), dim3(64 * WPB),
                          bytes, s, p, cells, ncell, color, mode);
       return hipGetLastError();
     }
@@ -824,6 +920,14 @@ namespace hdg
   hipError_t launch_fast_condense(int deg, const KParams &p, const int *cells, int ncell,
                                   int color, int mode, hipStream_t s)
   {
+    if (p.mixed)
+      {
+        if (p.thermo)
+          return deg == 1 ? launch_fast<1, 1, true, false, true>(p, cells, ncell, color, mode, s)
+                          : launch_fast<2, 1, true, false, true>(p, cells, ncell, color, mode, s);
+        return deg == 1 ? launch_fast<1, 3, false, false, true>(p, cells, ncell, color, mode, s)
+                        : launch_fast<2, 3, false, false, true>(p, cells, ncell, color, mode, s);
+      }
     if (p.thermo)
       return deg == 1 ? launch_fast<1, 1, true>(p, cells, ncell, color, mode, s)
                       : launch_fast<2, 1, true>(p, cells, ncell, color, mode, s);

@@ -30,6 +30,7 @@ namespace hdg
     double kTq, ec_r, ev_r, nc, nv;
     int    bc[6][3];
     double sdrift; // +1 reference linearisation, -1 consistent
+    int    mixed;  // 1: FP32 elimination + FP64 refinement in the register kernel's pass 1
     DevTables tb;
     // per-mesh data
     const double *doping;   // ncells*QC

@@ -27,7 +27,7 @@ EXPORTS = [
     "hdg_cell_systems", "hdg_set_kernel_variant",
     "hdg_trace_solver_create", "hdg_trace_solver_destroy", "hdg_trace_solver_info",
     "hdg_trace_factor", "hdg_trace_factor_status", "hdg_trace_solve",
-    "hdg_trace_plan_selftest", "hdg_terminal_current",
+    "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -85,6 +85,7 @@ def lib():
             "hdg_trace_solve": (i, [v, v, v, v]),
             "hdg_trace_plan_selftest": (i, [i, ip, i, ct.c_uint, dp, dp]),
             "hdg_terminal_current": (i, [v, v, v, i, v, v]),
+            "hdg_set_precision": (i, [v, i]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -145,6 +146,11 @@ class HdgContext:
     def set_enabled(self, V, n, p):
         _check(self.L.hdg_set_enabled(self.ctx, int(V), int(n), int(p)))
         self._refresh()
+
+    def set_precision(self, precision):
+        """0: FP64; 1: FP32 elimination + one FP64 refinement step (pass 1 of
+        the register kernel)"""
+        _check(self.L.hdg_set_precision(self.ctx, int(precision)))
 
     def set_kernel_variant(self, variant):
         """0 automatic, 1 LDS workgroup-per-cell kernel, 2 register-resident."""

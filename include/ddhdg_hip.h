@@ -117,6 +117,14 @@ int hdg_set_enabled(hdg_ctx *ctx, int V, int n, int p);
  * apply).  No counterpart in the reference (one CPU code path). */
 int hdg_set_kernel_variant(hdg_ctx *ctx, int variant);
 
+/* Precision of pass 1's local elimination: 0 = FP64 (default); 1 = the
+ * scalar block eliminated in FP32 with one FP64 iterative-refinement step
+ * (assembly, the residual, the refinement residual and the Schur product
+ * in FP64).  Applies to the register-resident kernel (2D, degree 1-2);
+ * the LDS kernel and pass 2 always run in FP64.  No counterpart in the
+ * reference (FP64 throughout). */
+int hdg_set_precision(hdg_ctx *ctx, int precision);
+
 /* Restricted trace matrix pattern, host arrays: rowptr (NTRACE+1),
  * colind (NNZ); rows are sorted. */
 int hdg_csr_pattern(const hdg_ctx *ctx, long long *rowptr, int *colind);
