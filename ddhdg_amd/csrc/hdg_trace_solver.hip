@@ -166,6 +166,177 @@ namespace
       }
   }
 
+  // ---------------------------------------------------------------------
+  // Fronts with a small eliminated block (S <= SMALL_S): F11 = P L11 U11
+  // (partial pivoting) in one workgroup per front, F11 in LDS, ipiv in
+  // LAPACK's 1-based form.  The off-diagonal blocks and the Schur update
+  // follow as for the other fronts.
+  constexpr int SMALL_S = 72;
+  constexpr int FT      = 256; // threads of the small-front kernels
+
+  __global__ void __launch_bounds__(FT)
+    k_front_lu_small(double *F, long long fbase, int n, int S, int I, int *ipiv,
+                     long long pbase, int *info, int batch, int yoff)
+  {
+    __shared__ double A[SMALL_S * SMALL_S]; // F11, column-major, ld S
+    __shared__ double red[FT];
+    __shared__ int    redi[FT];
+    __shared__ int    piv[SMALL_S];
+    const int b = blockIdx.x + yoff;
+    if (b >= batch)
+      return;
+    const int t  = threadIdx.x;
+    double   *Fb = F + fbase + (long long)b * n * n;
+    for (int e = t; e < S * S; e += FT)
+      A[e] = Fb[(e % S) + (long long)(e / S) * n];
+    __syncthreads();
+    int bad = 0;
+    for (int k = 0; k < S; ++k)
+      {
+        // pivot: largest |A(i,k)|, i >= k (lowest row on ties)
+        double best = -1.0;
+        int    bi   = k;
+        for (int i = k + t; i < S; i += FT)
+          {
+            const double v = fabs(A[i + k * S]);
+            if (v > best)
+              {
+                best = v;
+                bi   = i;
+              }
+          }
+        red[t]  = best;
+        redi[t] = bi;
+        __syncthreads();
+        for (int off = FT / 2; off > 0; off >>= 1)
+          {
+            if (t < off)
+              {
+                const double o = red[t + off];
+                const int    oi = redi[t + off];
+                if (o > red[t] || (o == red[t] && oi < redi[t]))
+                  {
+                    red[t]  = o;
+                    redi[t] = oi;
+                  }
+              }
+            __syncthreads();
+          }
+        const int p = redi[0];
+        if (t == 0)
+          {
+            piv[k] = p;
+            if (red[0] == 0.0 && !bad)
+              bad = k + 1;
+          }
+        __syncthreads();
+        if (p != k)
+          for (int j = t; j < S; j += FT)
+            {
+              const double tmp = A[k + j * S];
+              A[k + j * S]     = A[p + j * S];
+              A[p + j * S]     = tmp;
+            }
+        __syncthreads();
+        const double d = A[k + k * S];
+        for (int i = k + 1 + t; i < S; i += FT)
+          A[i + k * S] = d != 0.0 ? A[i + k * S] / d : 0.0;
+        __syncthreads();
+        const int m = S - k - 1;
+        for (int e = t; e < m * m; e += FT)
+          {
+            const int i = k + 1 + e % m, j = k + 1 + e / m;
+            A[i + j * S] -= A[i + k * S] * A[k + j * S];
+          }
+        __syncthreads();
+      }
+    if (t == 0)
+      info[b] = bad;
+    for (int e = t; e < S * S; e += FT)
+      Fb[(e % S) + (long long)(e / S) * n] = A[e];
+    for (int k = t; k < S; k += FT)
+      ipiv[pbase + (long long)b * S + k] = piv[k] + 1;
+  }
+
+  // forward sweep of one small front: P, L11^-1, then w_i -= L21 z
+  __global__ void __launch_bounds__(FT)
+    k_fwd_small(const double *F, double *w, long long fbase, long long wbase, int n, int S,
+                int I, const int *ipiv, long long pbase, int batch, int yoff)
+  {
+    __shared__ double z[SMALL_S];
+    const int b = blockIdx.x + yoff;
+    if (b >= batch)
+      return;
+    const int     t  = threadIdx.x;
+    const double *Fb = F + fbase + (long long)b * n * n;
+    double       *wb = w + wbase + (long long)b * n;
+    const int    *pv = ipiv + pbase + (long long)b * S;
+    if (t == 0)
+      {
+        for (int k = 0; k < S; ++k)
+          z[k] = wb[k];
+        for (int k = 0; k < S; ++k)
+          {
+            const int p = pv[k] - 1;
+            if (p != k)
+              {
+                const double tmp = z[k];
+                z[k]             = z[p];
+                z[p]             = tmp;
+              }
+          }
+        for (int j = 0; j < S; ++j)
+          {
+            const double y = z[j];
+            for (int i = j + 1; i < S; ++i)
+              z[i] -= Fb[i + (long long)j * n] * y;
+          }
+      }
+    __syncthreads();
+    for (int k = t; k < S; k += FT)
+      wb[k] = z[k];
+    for (int r = t; r < I; r += FT)
+      {
+        double acc = wb[S + r];
+        for (int j = 0; j < S; ++j)
+          acc -= Fb[S + r + (long long)j * n] * z[j];
+        wb[S + r] = acc;
+      }
+  }
+
+  // backward sweep of one small front: z -= U12 x_i, then U11^-1
+  __global__ void __launch_bounds__(FT)
+    k_bwd_small(const double *F, double *w, long long fbase, long long wbase, int n, int S,
+                int I, int batch, int yoff)
+  {
+    __shared__ double z[SMALL_S];
+    const int b = blockIdx.x + yoff;
+    if (b >= batch)
+      return;
+    const int     t  = threadIdx.x;
+    const double *Fb = F + fbase + (long long)b * n * n;
+    double       *wb = w + wbase + (long long)b * n;
+    for (int j = t; j < S; j += FT)
+      {
+        double acc = wb[j];
+        for (int c = 0; c < I; ++c)
+          acc -= Fb[j + (long long)(S + c) * n] * wb[S + c];
+        z[j] = acc;
+      }
+    __syncthreads();
+    if (t == 0)
+      for (int j = S - 1; j >= 0; --j)
+        {
+          double acc = z[j];
+          for (int l = j + 1; l < S; ++l)
+            acc -= Fb[j + (long long)l * n] * z[l];
+          z[j] = acc / Fb[j + (long long)j * n];
+        }
+    __syncthreads();
+    for (int j = t; j < S; j += FT)
+      wb[j] = z[j];
+  }
+
   __global__ void k_gather_rhs(const double *__restrict__ b, const long long *__restrict__ wmap,
                                long long ndof, double *__restrict__ w)
   {
@@ -403,9 +574,16 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
       if (G.S > 0)
         {
           // F11 = P L11 U11 (partial pivoting inside the eliminated block)
-          rocblas_status st = rocsolver_dgetrf_strided_batched(
-            T->h, G.S, G.S, F, G.n, (rocblas_stride)G.n * G.n, T->d_ipiv + G.pbase,
-            (rocblas_stride)G.S, T->d_info + info_off, G.batch);
+          rocblas_status st = rocblas_status_success;
+          if (G.S <= SMALL_S)
+            for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+              hipLaunchKernelGGL(k_front_lu_small, dim3(std::min(YCHUNK, G.batch - y0)),
+                                 dim3(FT), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, T->d_ipiv,
+                                 G.pbase, T->d_info + info_off, G.batch, y0);
+          else
+            st = rocsolver_dgetrf_strided_batched(
+              T->h, G.S, G.S, F, G.n, (rocblas_stride)G.n * G.n, T->d_ipiv + G.pbase,
+              (rocblas_stride)G.S, T->d_info + info_off, G.batch);
           if (st != rocblas_status_success)
             return hdg::set_error("hdg_trace_factor: rocsolver getrf failed", -3);
           if (G.I > 0)
@@ -508,7 +686,14 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
       const DevGroup     &D = T->dg[gi];
       const double       *F = T->d_F + G.fbase;
       double       *w = T->d_w + G.wbase;
-      if (G.S > 0)
+      if (G.S > 0 && G.S <= SMALL_S)
+        {
+          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+            hipLaunchKernelGGL(k_fwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(FT), 0, s,
+                               T->d_F, T->d_w, G.fbase, G.wbase, G.n, G.S, G.I, T->d_ipiv,
+                               G.pbase, G.batch, y0);
+        }
+      else if (G.S > 0)
         {
           hipLaunchKernelGGL(k_laswp, dim3((G.batch + 63) / 64), dim3(64), 0, s, T->d_w,
                              G.wbase, G.n, G.S, T->d_ipiv, G.pbase, G.batch);
@@ -544,7 +729,18 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
                              dim3((G.I + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),
                              dim3(TPB), 0, s, x, T->d_w, G.wbase, G.n, G.S, G.I, G.batch,
                              D.d_idof, y0);
-      if (G.S > 0)
+      if (G.S > 0 && G.S <= SMALL_S)
+        {
+          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+            hipLaunchKernelGGL(k_bwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(FT), 0, s,
+                               T->d_F, T->d_w, G.fbase, G.wbase, G.n, G.S, G.I, G.batch, y0);
+          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+            hipLaunchKernelGGL(k_scatter_x,
+                               dim3((G.S + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),
+                               dim3(TPB), 0, s, x, T->d_w, G.wbase, G.n, G.S, G.batch, D.d_edof,
+                               y0);
+        }
+      else if (G.S > 0)
         {
           rocblas_status st = rocblas_status_success;
           if (G.I > 0)
