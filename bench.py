@@ -61,6 +61,66 @@ def cpu_baseline(n_sample, degree, threads, bias):
     return o.ncells / t, o.ncells, t
 
 
+def batched_sample(args, torch, stream):
+    """pass 1 of many devices of uneven sizes and degrees in one persistent
+    launch per colour (autotuned grid) against the same devices condensed one
+    after the other (two launches each)"""
+    import numpy as np
+
+    from ddhdg_amd import configs
+    from ddhdg_amd.hip_lib import HdgBatch
+    from ddhdg_amd.np_solver import NPSolver
+
+    rng = np.random.default_rng(20261203)
+    sv = []
+    for _ in range(args.batch):
+        n = int(rng.integers(16, 193))
+        k = int(rng.choice([1, 2]))
+        dop = configs.step_doping(10 ** rng.uniform(21, 24), 10 ** rng.uniform(21, 24),
+                                  rng.uniform(0.3, 0.7))
+        prob, params, adim = configs.diode(2, n, k, doping=dop)
+        s = NPSolver(prob, params, adim, device="cuda")
+        s.compute_local_charge_neutrality()
+        s.project_densities_from_potential()
+        sv.append(s)
+    bufs = [(torch.empty(s.ctx.nnz, dtype=torch.float64, device="cuda"),
+             torch.empty(s.ctx.ntrace, dtype=torch.float64, device="cuda"),
+             torch.zeros(s.ctx.ntrace, dtype=torch.float64, device="cuda")) for s in sv]
+    b = HdgBatch([s.ctx for s in sv])
+    xc, xt = [s.x_cell for s in sv], [s.x_trace for s in sv]
+    v, r, d = [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs]
+    b.condense(False, xc, xt, v, r, d)            # autotunes
+    torch.cuda.synchronize()
+
+    def timed(fn, reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, e0.elapsed_time(e1) / reps
+
+    reps = max(3, min(args.steps, 20))
+    wb, gb = timed(lambda: b.condense(False, xc, xt, v, r, d), reps)
+
+    def sequential():
+        for s, (vv, rr, dd) in zip(sv, bufs):
+            s.ctx.condense(s.x_cell, s.x_trace, False, vv, rr, dd)
+    ws, gs = timed(sequential, reps)
+    cells = sum(s.ctx.ncells for s in sv)
+    return {"devices": len(sv), "cells": int(cells),
+            "degrees": {str(k): sum(1 for s in sv if s.degree == k) for k in (1, 2)},
+            "value": cells / wb, "unit": "cells/s", "gpu_ms": gb,
+            "sequential_value": cells / ws, "sequential_gpu_ms": gs,
+            "autotune": b.info(),
+            "workload": f"{len(sv)} 2D diodes, sides uniform in 16..192 cells, degree 1-2, "
+                        "doping log-uniform 1e21-1e24, junction in [0.3, 0.7]; pass 1, FP64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +133,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=3,
                     help="whole Newton steps timed beside the metric")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="devices condensed together beside the metric (0: none)")
     ap.add_argument("--serve", type=int, default=16,
                     help="I-V sweep requests served beside the metric (0: none)")
     ap.add_argument("--serve-rate", type=float, default=4.0)
@@ -242,6 +304,8 @@ def main():
             "trace_solver": {"levels": ts.levels, "max_front": ts.max_front,
                              "front_GB": ts.bytes / 1e9, "factor_GFLOP": ts.flops / 1e9,
                              "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12}}
+    if args.batch > 0:
+        line["batched"] = batched_sample(args, torch, stream)
     if args.serve > 0:
         # served I-V sweeps: Poisson arrivals, continuous batching, state kept
         # on the device across the bias points (a bounded sample of the

@@ -121,6 +121,7 @@ namespace
         p.bc[b][c] = C->phys.bc_type[b][c];
     p.sdrift = C->phys.drift_jacobian ? -1.0 : 1.0;
     p.mixed  = C->precision;
+    p.deg    = C->k;
     p.tb     = C->tb;
     p.doping = C->d_doping;
     p.dirv   = C->d_dirv;
@@ -521,6 +522,210 @@ int hdg_terminal_current(hdg_ctx *C, const double *x_cell, const double *x_trace
   hipError_t e   = hdg::launch_current(C->d, C->nb, C->nf, C->qf, p, boundary_id, out,
                                        (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_terminal_current launch");
+}
+
+} // extern "C"
+
+// several contexts condensed by one persistent launch per colour
+struct hdg_batch
+{
+  std::vector<hdg_ctx *> ctx;
+  int                    ne = 0, thermo_ok = 0, full_ok = 0, wpb = 4, ncu = 256;
+  int                   *d_chunk_dev[2] = {nullptr, nullptr}, *d_items[2] = {nullptr, nullptr};
+  int                    nchunks[2] = {0, 0};
+  long long              ncells = 0;
+  hdg::KParams          *d_params = nullptr, *h_params = nullptr; // h_params pinned
+  hipEvent_t             copied   = nullptr;
+  int                    grid     = 0; // WGs per launch; 0 until tuned
+  float                  tuned_ms[3] = {0, 0, 0};
+};
+
+namespace
+{
+  int batch_launch(hdg_batch *B, int thermodyn, int mixed, int grid, hipStream_t s)
+  {
+    for (int col = 0; col < 2; ++col)
+      {
+        hipError_t e = hdg::launch_batch_condense(B->ne, thermodyn, mixed, B->d_params,
+                                                  B->d_chunk_dev[col], B->d_items[col],
+                                                  B->nchunks[col], col, grid, s);
+        if (e != hipSuccess)
+          return hip_fail(e, "hdg_batch_condense launch");
+      }
+    return 0;
+  }
+} // namespace
+
+extern "C" {
+
+int hdg_batch_create(hdg_ctx *const *ctxs, int n, hdg_batch **out)
+{
+  if (!ctxs || n < 1 || !out)
+    return fail("hdg_batch_create: invalid argument");
+  hdg_batch *B = new hdg_batch();
+  B->ne        = ctxs[0]->ne;
+  for (int i = 0; i < n; ++i)
+    {
+      hdg_ctx *C = ctxs[i];
+      if (!C || C->d != 2 || C->k < 1 || C->k > 2 || C->ne != B->ne ||
+          C->en[0] != ctxs[0]->en[0] || C->en[1] != ctxs[0]->en[1] ||
+          C->en[2] != ctxs[0]->en[2])
+        {
+          delete B;
+          return fail("hdg_batch_create: contexts must be 2D, degree 1 or 2, with the same "
+                      "enabled components", -2);
+        }
+      B->ctx.push_back(C);
+      B->ncells += C->ncells;
+    }
+  B->wpb = hdg::batch_waves_per_group();
+  int rc = 0;
+  for (int col = 0; col < 2 && rc == 0; ++col)
+    {
+      std::vector<int> cdev, items;
+      for (int i = 0; i < n; ++i)
+        {
+          const auto &cl = B->ctx[i]->color_cells[col];
+          for (size_t a = 0; a < cl.size(); a += B->wpb)
+            {
+              cdev.push_back(i);
+              for (int w = 0; w < B->wpb; ++w)
+                items.push_back(a + w < cl.size() ? cl[a + w] : -1);
+            }
+        }
+      B->nchunks[col] = (int)cdev.size();
+      rc              = upload(&B->d_chunk_dev[col], cdev);
+      if (rc == 0)
+        rc = upload(&B->d_items[col], items);
+    }
+  hipError_t e = hipSuccess;
+  if (rc == 0)
+    e = hipMalloc((void **)&B->d_params, n * sizeof(hdg::KParams));
+  if (rc == 0 && e == hipSuccess)
+    e = hipHostMalloc((void **)&B->h_params, n * sizeof(hdg::KParams), hipHostMallocDefault);
+  if (rc == 0 && e == hipSuccess)
+    e = hipEventCreateWithFlags(&B->copied, hipEventDisableTiming);
+  if (rc == 0 && e == hipSuccess)
+    {
+      int dev = 0;
+      if (hipGetDevice(&dev) == hipSuccess)
+        hipDeviceGetAttribute(&B->ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    }
+  if (rc == 0 && e != hipSuccess)
+    rc = hip_fail(e, "hdg_batch_create");
+  if (rc != 0)
+    {
+      hdg_batch_destroy(B);
+      return rc;
+    }
+  *out = B;
+  return 0;
+}
+
+void hdg_batch_destroy(hdg_batch *B)
+{
+  if (!B)
+    return;
+  for (int col = 0; col < 2; ++col)
+    for (void *p : {(void *)B->d_chunk_dev[col], (void *)B->d_items[col]})
+      if (p)
+        (void)hipFree(p);
+  if (B->d_params)
+    (void)hipFree(B->d_params);
+  if (B->h_params)
+    (void)hipHostFree(B->h_params);
+  if (B->copied)
+    (void)hipEventDestroy(B->copied);
+  delete B;
+}
+
+int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
+                       const double *const *x_trace, double *const *csr_vals,
+                       double *const *rhs, double *const *dir_vals, void *stream)
+{
+  if (!B || !x_cell || !x_trace || !csr_vals || !rhs || !dir_vals)
+    return fail("hdg_batch_condense: null argument");
+  if (thermodyn && (B->ne != 1 || !B->ctx[0]->en[0]))
+    return fail("hdg_batch_condense: thermodynamic equilibrium runs with V only", -2);
+  if (!thermodyn && B->ne != 3)
+    return fail("hdg_batch_condense: the register kernel needs V, n and p or V alone", -2);
+  hipStream_t s = (hipStream_t)stream;
+  // the pinned parameter blocks may still feed the previous call's copy
+  hipError_t e = hipEventSynchronize(B->copied);
+  if (e != hipSuccess)
+    return hip_fail(e, "hdg_batch_condense");
+  const int n     = (int)B->ctx.size();
+  int       mixed = B->ctx[0]->precision;
+  for (int i = 0; i < n; ++i)
+    {
+      if (!x_cell[i] || !x_trace[i] || !csr_vals[i] || !rhs[i] || !dir_vals[i])
+        return fail("hdg_batch_condense: null device pointer");
+      hdg::KParams p = params(B->ctx[i], thermodyn);
+      p.xc           = x_cell[i];
+      p.xt           = x_trace[i];
+      p.vals         = csr_vals[i];
+      p.rhs          = rhs[i];
+      p.dirout       = dir_vals[i];
+      B->h_params[i] = p;
+    }
+  e = hipMemcpyAsync(B->d_params, B->h_params, n * sizeof(hdg::KParams), hipMemcpyHostToDevice,
+                     s);
+  if (e == hipSuccess)
+    e = hipEventRecord(B->copied, s);
+  if (e != hipSuccess)
+    return hip_fail(e, "hdg_batch_condense copy");
+  if (B->grid == 0)
+    {
+      // autotuner: persistent grids of 1, 2 and 4 workgroups per CU, timed
+      // on this batch (complete two-colour passes: the outputs are the same
+      // for every grid); the fastest is kept for the batch
+      const int  cand[3] = {1, 2, 4};
+      hipEvent_t a, b;
+      if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+        return fail("hdg_batch_condense: event creation failed", -3);
+      float best = 1e30f;
+      for (int k = 0; k < 3; ++k)
+        {
+          const int g = std::max(1, std::min(cand[k] * B->ncu, std::max(B->nchunks[0], B->nchunks[1])));
+          int       rc = batch_launch(B, thermodyn, mixed, g, s);
+          if (rc == 0)
+            {
+              (void)hipEventRecord(a, s);
+              rc = batch_launch(B, thermodyn, mixed, g, s);
+              (void)hipEventRecord(b, s);
+              (void)hipEventSynchronize(b);
+            }
+          if (rc != 0)
+            {
+              (void)hipEventDestroy(a);
+              (void)hipEventDestroy(b);
+              return rc;
+            }
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, a, b);
+          B->tuned_ms[k] = ms;
+          if (ms < best)
+            {
+              best    = ms;
+              B->grid = g;
+            }
+        }
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
+    }
+  return batch_launch(B, thermodyn, mixed, B->grid, s);
+}
+
+int hdg_batch_info(const hdg_batch *B, double out[6])
+{
+  if (!B || !out)
+    return fail("hdg_batch_info: null argument");
+  out[0] = (double)B->ncells;
+  out[1] = (double)B->grid;
+  out[2] = (double)B->ncu;
+  for (int k = 0; k < 3; ++k)
+    out[3 + k] = B->tuned_ms[k];
+  return 0;
 }
 
 int hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face, unsigned seed,

@@ -108,32 +108,12 @@ namespace hdg
       }
     };
 
-    // PASS2: the reconstruction pass (cells = all cells in order, columns =
-    // scalar dofs + rhs with the trace update folded in, output dx_cell)
-    // MIXED (pass 1 only): the elimination of the scalar block in FP32 with one
-    // FP64 refinement step; assembly, residual, refinement residual and Schur
-    // product in FP64
-    template <int DEG, int NE, bool THERMO, bool PASS2, bool MIXED = false>
-    __global__ void __launch_bounds__(64 * WPB)
-      __attribute__((amdgpu_waves_per_eu(PASS2 ? WPE2 : WPE)))
-      fast_condense(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
+    // the reference-cell tables of one mesh into LDS (one workgroup)
+    template <int DEG, int NE>
+    __device__ __forceinline__ void stage_tables(const DevTables &tb, double *tab0, int tid)
     {
-      using S              = F2<DEG, NE>;
-      constexpr int WSTR   = PASS2 ? S::WDBL2 : S::WDBL;
-      constexpr int NB     = S::NB, NF = S::NF, QC = S::QC, QF = S::QF, NFC = S::NFC;
-      constexpr int NS     = S::NS, TDF = S::TDF, T = S::T, NROW = S::NROW;
-      constexpr int K1     = S::K1;
-      extern __shared__ double smem[];
-      double       *tab0 = smem;
-      const int     tid  = threadIdx.x;
-      const int     lane = tid & 63, wv = tid >> 6;
-      double       *ws   = smem + S::TAB + wv * WSTR;
-      int *wi = reinterpret_cast<int *>(smem + S::TAB + WPB * WSTR) + wv * S::WINT;
-      long long *blk   = reinterpret_cast<long long *>(ws + S::wBLK);
-      long long *facev = reinterpret_cast<long long *>(ws + S::wFACE);
-      const DevTables &tb = p.tb;
-
-      // ---- stage the shared tables once per workgroup
+      using S          = F2<DEG, NE>;
+      constexpr int NB = S::NB, NF = S::NF, QC = S::QC, QF = S::QF, NFC = S::NFC;
       for (int i = tid; i < QC * NB; i += 64 * WPB)
         {
           tab0[S::tPHIC + i] = tb.phic[i];
@@ -166,7 +146,28 @@ namespace hdg
         tab0[S::tBZ + i] = tb.Bz[i];
       for (int i = tid; i < NFC * QF * NF; i += 64 * WPB)
         tab0[S::tBT + i] = tb.Bt[i];
-      __syncthreads();
+    }
+
+    // one cell (one wave): assembly, condensation and output of pass 1, or
+    // the reconstruction of pass 2; tables staged in smem, per-wave
+    // scratch of wave wv at stride wstr
+    template <int DEG, int NE, bool THERMO, bool PASS2, bool MIXED>
+    __device__ __forceinline__ void cell_body(const KParams &p, double *smem, int wstr,
+                                              long long cell, long long idx, int color,
+                                              int mode)
+    {
+      using S              = F2<DEG, NE>;
+      constexpr int NB     = S::NB, NF = S::NF, QC = S::QC, QF = S::QF, NFC = S::NFC;
+      constexpr int NS     = S::NS, TDF = S::TDF, T = S::T, NROW = S::NROW;
+      constexpr int K1     = S::K1;
+      double       *tab0 = smem;
+      const int     tid  = threadIdx.x;
+      const int     lane = tid & 63, wv = tid >> 6;
+      double       *ws   = smem + S::TAB + wv * wstr;
+      int *wi = reinterpret_cast<int *>(smem + S::TAB + WPB * wstr) + wv * S::WINT;
+      long long *blk   = reinterpret_cast<long long *>(ws + S::wBLK);
+      long long *facev = reinterpret_cast<long long *>(ws + S::wFACE);
+      const DevTables &tb = p.tb;
 
       double *u0 = ws + S::wU0, *W0 = ws + S::wW0, *u0f = ws + S::wU0F, *W0f = ws + S::wW0F;
       double *tr0 = ws + S::wTR0, *qd = ws + S::wQD, *rhsq = ws + S::wRHSQ;
@@ -200,683 +201,763 @@ namespace hdg
       else
         kind = 3;
 
-      for (int idx = blockIdx.x * WPB + wv; idx < ncell; idx += gridDim.x * WPB)
+      // the tables are loop invariant; hiding their addresses behind an
+      // empty asm keeps the compiler from hoisting every table load out
+      // of the cell loop (hundreds of live values, all spilled)
+      double *tab = tab0;
+      asm volatile("" : "+v"(tab));
+      const double *Bz = tab + S::tBZ, *Bt = tab + S::tBT;
+      int             ic[2];
+      ic[0] = (int)(cell % p.n[0]);
+      ic[1] = (int)(cell / p.n[0]);
+      // ---- A: topology, flags, state
+      if (lane < NFC)
         {
-          const long long cell = PASS2 ? (long long)idx : (long long)cells[idx];
-          // the tables are loop invariant; hiding their addresses behind an
-          // empty asm keeps the compiler from hoisting every table load out
-          // of the cell loop (hundreds of live values, all spilled)
-          double *tab = tab0;
-          asm volatile("" : "+v"(tab));
-          const double *Bz = tab + S::tBZ, *Bt = tab + S::tBT;
-          int             ic[2];
-          ic[0] = (int)(cell % p.n[0]);
-          ic[1] = (int)(cell / p.n[0]);
-          // ---- A: topology, flags, state
-          if (lane < NFC)
+          const int lf = lane, a = lf >> 1, sd = lf & 1;
+          const long long F = face_of(p, ic[0], ic[1], 0, lf);
+          facev[lf]         = F;
+          blk[lf]           = p.blkptr[F];
+          nbk[lf]           = p.nbk[F];
+          const bool bnd    = sd ? (ic[a] == p.n[a] - 1) : (ic[a] == 0);
+          const int  bid    = bnd ? lf : -1;
+          bidv[lf]          = bid;
+          for (int c = 0; c < 3; ++c)
             {
-              const int lf = lane, a = lf >> 1, sd = lf & 1;
-              const long long F = face_of(p, ic[0], ic[1], 0, lf);
-              facev[lf]         = F;
-              blk[lf]           = p.blkptr[F];
-              nbk[lf]           = p.nbk[F];
-              const bool bnd    = sd ? (ic[a] == p.n[a] - 1) : (ic[a] == 0);
-              const int  bid    = bnd ? lf : -1;
-              bidv[lf]          = bid;
-              for (int c = 0; c < 3; ++c)
-                {
-                  const int bt      = (bid >= 0 && p.en[c]) ? p.bc[bid][c] : 0;
-                  isdir[lf * 3 + c] = bt == 1;
-                  isneu[lf * 3 + c] = bt == 2;
-                }
+              const int bt      = (bid >= 0 && p.en[c]) ? p.bc[bid][c] : 0;
+              isdir[lf * 3 + c] = bt == 1;
+              isneu[lf * 3 + c] = bt == 2;
             }
-          if (lane < NFC * NFC)
-            slot[lane] = p.cslot[cell * NFC * NFC + lane];
-          for (int i = lane; i < S::NCD; i += 64)
-            xs[i] = p.xc[cell * S::NCD + i];
-          for (int i = lane; i < QC; i += 64)
-            qd[5 * QC + i] = p.doping[cell * QC + i];
-          WAVE_SYNC();
-          for (int i = lane; i < NFC * 3 * NF; i += 64)
-            {
-              const int lf = i / (3 * NF), r = i % (3 * NF);
-              xt[i]        = p.xt[facev[lf] * 3 * NF + r];
-            }
-          WAVE_SYNC();
-          // ---- B: state at quadrature points
-          for (int e = lane; e < 3 * 3 * QC; e += 64)
-            {
-              const int c = e / (3 * QC), r = e % (3 * QC);
-              const int blkc = r / QC, q = r % QC;
-              const double *x = xs + c * 3 * NB + blkc * NB;
-              double        v = 0;
+        }
+      if (lane < NFC * NFC)
+        slot[lane] = p.cslot[cell * NFC * NFC + lane];
+      for (int i = lane; i < S::NCD; i += 64)
+        xs[i] = p.xc[cell * S::NCD + i];
+      for (int i = lane; i < QC; i += 64)
+        qd[5 * QC + i] = p.doping[cell * QC + i];
+      WAVE_SYNC();
+      for (int i = lane; i < NFC * 3 * NF; i += 64)
+        {
+          const int lf = i / (3 * NF), r = i % (3 * NF);
+          xt[i]        = p.xt[facev[lf] * 3 * NF + r];
+        }
+      WAVE_SYNC();
+      // ---- B: state at quadrature points
+      for (int e = lane; e < 3 * 3 * QC; e += 64)
+        {
+          const int c = e / (3 * QC), r = e % (3 * QC);
+          const int blkc = r / QC, q = r % QC;
+          const double *x = xs + c * 3 * NB + blkc * NB;
+          double        v = 0;
 #pragma unroll
-              for (int j = 0; j < NB; ++j)
-                v += x[j] * tab[S::tPHIC + q * NB + j];
-              if (blkc == 2)
-                u0[c * QC + q] = v;
-              else
-                W0[(c * 2 + blkc) * QC + q] = v;
-            }
-          for (int e = lane; e < 3 * NFC * QF; e += 64)
+          for (int j = 0; j < NB; ++j)
+            v += x[j] * tab[S::tPHIC + q * NB + j];
+          if (blkc == 2)
+            u0[c * QC + q] = v;
+          else
+            W0[(c * 2 + blkc) * QC + q] = v;
+        }
+      for (int e = lane; e < 3 * NFC * QF; e += 64)
+        {
+          const int c = e / (NFC * QF), g = (e / QF) % NFC, s = e % QF;
+          const int a = g >> 1;
+          const double nu  = (g & 1) ? 1.0 : -1.0;
+          const double *xc = xs + c * 3 * NB;
+          const double *ph = tab + S::tPHIF + (g * QF + s) * NB;
+          double        u = 0, w = 0, t = 0;
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
             {
-              const int c = e / (NFC * QF), g = (e / QF) % NFC, s = e % QF;
+              u += xc[2 * NB + j] * ph[j];
+              w += xc[a * NB + j] * ph[j];
+            }
+#pragma unroll
+          for (int m = 0; m < NF; ++m)
+            t += xt[(g * 3 + c) * NF + m] * tab[S::tXI + s * NF + m];
+          u0f[e] = u;
+          W0f[e] = nu * w;
+          tr0[e] = t;
+        }
+      WAVE_SYNC();
+      // ---- C: point data, flux-row residual, Dirichlet update
+      for (int q = lane; q < QC; q += 64)
+        {
+          double r = 0, a = 0, b = 0;
+          if (!THERMO)
+            recomb(p, u0[QC + q], u0[2 * QC + q], r, a, b);
+          qd[q]          = r;
+          qd[QC + q]     = a;
+          qd[2 * QC + q] = b;
+          if (THERMO)
+            {
+              const double V0 = u0[q], kT = p.kTq;
+              const double e1 = exp((p.ev_r - V0) / kT);
+              const double e2 = exp((V0 - p.ec_r) / kT);
+              qd[3 * QC + q]  = -1 / p.n_rf * (p.nv / kT * e1 + p.nc / kT * e2);
+              qd[4 * QC + q]  = 1 / p.n_rf * (p.nv * e1 - p.nc * e2);
+            }
+        }
+      for (int e = lane; e < NE * 2 * NB; e += 64)
+        {
+          const int c = e / (2 * NB), a = (e / NB) % 2, i = e % NB;
+          double    v = 0;
+#pragma unroll
+          for (int q = 0; q < QC; ++q)
+            v += (u0[c * QC + q] * tab[S::tGRADC + (q * NB + i) * 2 + a] -
+                  W0[(c * 2 + a) * QC + q] * tab[S::tPHIC + q * NB + i]) *
+                 tab[S::tJXWC + q];
+#pragma unroll
+          for (int sd = 0; sd < 2; ++sd)
+            {
+              const int    g  = 2 * a + sd;
+              const double nu = sd ? 1.0 : -1.0;
+#pragma unroll
+              for (int s = 0; s < QF; ++s)
+                v -= tr0[(c * NFC + g) * QF + s] * tab[S::tPHIF + (g * QF + s) * NB + i] *
+                     nu * tab[S::tJXWF + g * QF + s];
+            }
+          rhsq[e] = v;
+        }
+      for (int e = lane; e < NFC * 3 * NF; e += 64)
+        {
+          const int g = e / (3 * NF), c = (e / NF) % 3, m = e % NF;
+          double    v = 0;
+          if (PASS2)
+            v = p.dxt[facev[g] * 3 * NF + c * NF + m];
+          else if (!THERMO && isdir[g * 3 + c])
+            {
               const int a = g >> 1;
-              const double nu  = (g & 1) ? 1.0 : -1.0;
-              const double *xc = xs + c * 3 * NB;
-              const double *ph = tab + S::tPHIF + (g * QF + s) * NB;
-              double        u = 0, w = 0, t = 0;
 #pragma unroll
-              for (int j = 0; j < NB; ++j)
+              for (int n2 = 0; n2 < NF; ++n2)
                 {
-                  u += xc[2 * NB + j] * ph[j];
-                  w += xc[a * NB + j] * ph[j];
-                }
-#pragma unroll
-              for (int m = 0; m < NF; ++m)
-                t += xt[(g * 3 + c) * NF + m] * tab[S::tXI + s * NF + m];
-              u0f[e] = u;
-              W0f[e] = nu * w;
-              tr0[e] = t;
-            }
-          WAVE_SYNC();
-          // ---- C: point data, flux-row residual, Dirichlet update
-          for (int q = lane; q < QC; q += 64)
-            {
-              double r = 0, a = 0, b = 0;
-              if (!THERMO)
-                recomb(p, u0[QC + q], u0[2 * QC + q], r, a, b);
-              qd[q]          = r;
-              qd[QC + q]     = a;
-              qd[2 * QC + q] = b;
-              if (THERMO)
-                {
-                  const double V0 = u0[q], kT = p.kTq;
-                  const double e1 = exp((p.ev_r - V0) / kT);
-                  const double e2 = exp((V0 - p.ec_r) / kT);
-                  qd[3 * QC + q]  = -1 / p.n_rf * (p.nv / kT * e1 + p.nc / kT * e2);
-                  qd[4 * QC + q]  = 1 / p.n_rf * (p.nv * e1 - p.nc * e2);
-                }
-            }
-          for (int e = lane; e < NE * 2 * NB; e += 64)
-            {
-              const int c = e / (2 * NB), a = (e / NB) % 2, i = e % NB;
-              double    v = 0;
-#pragma unroll
-              for (int q = 0; q < QC; ++q)
-                v += (u0[c * QC + q] * tab[S::tGRADC + (q * NB + i) * 2 + a] -
-                      W0[(c * 2 + a) * QC + q] * tab[S::tPHIC + q * NB + i]) *
-                     tab[S::tJXWC + q];
-#pragma unroll
-              for (int sd = 0; sd < 2; ++sd)
-                {
-                  const int    g  = 2 * a + sd;
-                  const double nu = sd ? 1.0 : -1.0;
+                  double b = 0;
 #pragma unroll
                   for (int s = 0; s < QF; ++s)
-                    v -= tr0[(c * NFC + g) * QF + s] * tab[S::tPHIF + (g * QF + s) * NB + i] *
-                         nu * tab[S::tJXWF + g * QF + s];
+                    b += tab[S::tXI + s * NF + n2] * tab[S::tJXWF + g * QF + s] *
+                         (p.dirv[(facev[g] * 3 + c) * QF + s] - tr0[(c * NFC + g) * QF + s]);
+                  v += tab[S::tMFI + (a * NF + m) * NF + n2] * b;
                 }
-              rhsq[e] = v;
             }
-          for (int e = lane; e < NFC * 3 * NF; e += 64)
+          lupd[e] = v;
+        }
+      WAVE_SYNC();
+      // ---- D: lifted residual flux with the Dirichlet update folded in;
+      //         residual forms at the quadrature points
+      for (int e = lane; e < NE * 2 * QC; e += 64)
+        {
+          const int c = e / (2 * QC), a = (e / QC) % 2, q = e % QC;
+          double    v = 0;
+#pragma unroll
+          for (int i = 0; i < NB; ++i)
+            v += tab[S::tPMC + q * NB + i] * rhsq[(c * 2 + a) * NB + i];
+#pragma unroll
+          for (int sd = 0; sd < 2; ++sd)
             {
-              const int g = e / (3 * NF), c = (e / NF) % 3, m = e % NF;
-              double    v = 0;
-              if (PASS2)
-                v = p.dxt[facev[g] * 3 * NF + c * NF + m];
-              else if (!THERMO && isdir[g * 3 + c])
-                {
-                  const int a = g >> 1;
-#pragma unroll
-                  for (int n2 = 0; n2 < NF; ++n2)
-                    {
-                      double b = 0;
-#pragma unroll
-                      for (int s = 0; s < QF; ++s)
-                        b += tab[S::tXI + s * NF + n2] * tab[S::tJXWF + g * QF + s] *
-                             (p.dirv[(facev[g] * 3 + c) * QF + s] - tr0[(c * NFC + g) * QF + s]);
-                      v += tab[S::tMFI + (a * NF + m) * NF + n2] * b;
-                    }
-                }
-              lupd[e] = v;
-            }
-          WAVE_SYNC();
-          // ---- D: lifted residual flux with the Dirichlet update folded in;
-          //         residual forms at the quadrature points
-          for (int e = lane; e < NE * 2 * QC; e += 64)
-            {
-              const int c = e / (2 * QC), a = (e / QC) % 2, q = e % QC;
-              double    v = 0;
-#pragma unroll
-              for (int i = 0; i < NB; ++i)
-                v += tab[S::tPMC + q * NB + i] * rhsq[(c * 2 + a) * NB + i];
-#pragma unroll
-              for (int sd = 0; sd < 2; ++sd)
-                {
-                  const int f = 2 * a + sd;
-#pragma unroll
-                  for (int m = 0; m < NF; ++m)
-                    v += tab[S::tLCC + (f * QC + q) * NF + m] * lupd[(f * 3 + c) * NF + m];
-                }
-              Wrc[e] = v;
-            }
-          for (int e = lane; e < NE * NFC * QF; e += 64)
-            {
-              const int c = e / (NFC * QF), g = (e / QF) % NFC, s = e % QF;
-              const int a = g >> 1;
-              double    v = 0, l = 0;
-#pragma unroll
-              for (int i = 0; i < NB; ++i)
-                v += tab[S::tPMF + (g * QF + s) * NB + i] * rhsq[(c * 2 + a) * NB + i];
-#pragma unroll
-              for (int sd = 0; sd < 2; ++sd)
-                {
-                  const int f = 2 * a + sd;
-#pragma unroll
-                  for (int m = 0; m < NF; ++m)
-                    v += tab[S::tLCF + ((f * NFC + g) * QF + s) * NF + m] *
-                         lupd[(f * 3 + c) * NF + m];
-                }
+              const int f = 2 * a + sd;
 #pragma unroll
               for (int m = 0; m < NF; ++m)
-                l += lupd[(g * 3 + c) * NF + m] * tab[S::tXI + s * NF + m];
-              Wrf[e]  = v;
-              lamf[e] = l;
+                v += tab[S::tLCC + (f * QC + q) * NF + m] * lupd[(f * 3 + c) * NF + m];
             }
-          for (int e = lane; e < QC * NE; e += 64)
+          Wrc[e] = v;
+        }
+      for (int e = lane; e < NE * NFC * QF; e += 64)
+        {
+          const int c = e / (NFC * QF), g = (e / QF) % NFC, s = e % QF;
+          const int a = g >> 1;
+          double    v = 0, l = 0;
+#pragma unroll
+          for (int i = 0; i < NB; ++i)
+            v += tab[S::tPMF + (g * QF + s) * NB + i] * rhsq[(c * 2 + a) * NB + i];
+#pragma unroll
+          for (int sd = 0; sd < 2; ++sd)
             {
-              const int q = e / NE, ci = e % NE, c = NE == 1 ? 0 : ci;
-              const double E0x = W0[0 * QC + q], E0y = W0[1 * QC + q];
-              double       al, bx, by;
-              if (c == 0)
+              const int f = 2 * a + sd;
+#pragma unroll
+              for (int m = 0; m < NF; ++m)
+                v += tab[S::tLCF + ((f * NFC + g) * QF + s) * NF + m] *
+                     lupd[(f * 3 + c) * NF + m];
+            }
+#pragma unroll
+          for (int m = 0; m < NF; ++m)
+            l += lupd[(g * 3 + c) * NF + m] * tab[S::tXI + s * NF + m];
+          Wrf[e]  = v;
+          lamf[e] = l;
+        }
+      for (int e = lane; e < QC * NE; e += 64)
+        {
+          const int q = e / NE, ci = e % NE, c = NE == 1 ? 0 : ci;
+          const double E0x = W0[0 * QC + q], E0y = W0[1 * QC + q];
+          double       al, bx, by;
+          if (c == 0)
+            {
+              al = qd[5 * QC + q] + (THERMO ? qd[4 * QC + q] : u0[2 * QC + q] - u0[QC + q]);
+              bx = eps * E0x;
+              by = eps * E0y;
+            }
+          else if (c == 1)
+            {
+              const double n0 = u0[QC + q];
+              al = qd[q];
+              bx = n0 * mun * E0x - UT * mun * W0[2 * QC + q];
+              by = n0 * mun * E0y - UT * mun * W0[3 * QC + q];
+            }
+          else
+            {
+              const double p0 = u0[2 * QC + q];
+              al = qd[q];
+              bx = -p0 * mup * E0x - UT * mup * W0[4 * QC + q];
+              by = -p0 * mup * E0y - UT * mup * W0[5 * QC + q];
+            }
+          resc[(q * NE + ci) * 3 + 0] = al;
+          resc[(q * NE + ci) * 3 + 1] = bx;
+          resc[(q * NE + ci) * 3 + 2] = by;
+        }
+      for (int e = lane; e < NFC * QF * NE; e += 64)
+        {
+          const int fi = e / NE, ci = e % NE, c = NE == 1 ? 0 : ci;
+          const double E0n = W0f[fi];
+          double       v;
+          if (c == 0)
+            v = -eps * E0n - tV * u0f[fi] + tV * tr0[fi];
+          else if (c == 1)
+            {
+              const double n0 = u0f[NFC * QF + fi];
+              v = -(n0 * mun * E0n - UT * mun * W0f[NFC * QF + fi]) + tN * n0 -
+                  tN * tr0[NFC * QF + fi];
+            }
+          else
+            {
+              const double p0 = u0f[2 * NFC * QF + fi];
+              v = -(-p0 * mup * E0n - UT * mup * W0f[2 * NFC * QF + fi]) + tP * p0 -
+                  tP * tr0[2 * NFC * QF + fi];
+            }
+          resf[e] = v;
+        }
+      for (int e = lane; e < NFC * NE * NF; e += 64)
+        {
+          const int g = e / (NE * NF), ci = (e / NF) % NE, i = e % NF;
+          const int c = NE == 1 ? 0 : ci;
+          double    v = 0;
+          if (isneu[g * 3 + c] && !isdir[g * 3 + c])
+#pragma unroll
+            for (int s = 0; s < QF; ++s)
+              v += Bt[(g * QF + s) * NF + i] * p.neuv[(facev[g] * 3 + c) * QF + s];
+          neu[e] = v;
+        }
+      WAVE_SYNC();
+
+      // ---- E: bundle of this lane's column and the forms it feeds.
+      // Branch-free: every table value is loaded (indices valid for all
+      // lanes) and blended with the lane's 0/1 weights, so no per-lane
+      // masks have to be kept alive.
+      const bool   isU = kind == 0, isL = kind == 1, isR = kind == 2;
+      const double isr = isR ? 1.0 : 0.0;
+      const int    ju = isU ? jj : 0, jl = isL ? jj : 0, fl = isL ? ff : 0;
+      double       mU[NE], mL[NE];
+#pragma unroll
+      for (int ci = 0; ci < NE; ++ci)
+        {
+          mU[ci] = (isU && cc == ci) ? 1.0 : 0.0;
+          mL[ci] = (isL && cc == ci) ? 1.0 : 0.0;
+        }
+      const double mLx = (fl >> 1) == 0 ? 1.0 : 0.0, mLy = 1.0 - mLx;
+      double      *tcl  = ws + S::wTCL;
+      int         *pivs = wi + S::iPIV;
+
+      // ---- 1a: scalar-test rows, cell quadrature points.  q stays a
+      // runtime index: unrolled, the scheduler hoists every (uniform) Bz
+      // load of all quadrature points at once
+      double R[NS];
+#pragma unroll
+      for (int r = 0; r < NS; ++r)
+        R[r] = 0.0;
+#pragma unroll 1
+      for (int q = 0; q < QC; ++q)
+        {
+          double uu[3] = {0, 0, 0}, WX[3] = {0, 0, 0}, WY[3] = {0, 0, 0};
+          const double ph = tab[S::tPHIC + q * NB + ju];
+          const double lx = tab[S::tLDC + (q * 2 + 0) * NB + ju];
+          const double ly = tab[S::tLDC + (q * 2 + 1) * NB + ju];
+          const double lc = tab[S::tLCC + (fl * QC + q) * NF + jl];
+#pragma unroll
+          for (int ci = 0; ci < NE; ++ci)
+            {
+              uu[ci] = mU[ci] * ph;
+              WX[ci] = mU[ci] * lx + mL[ci] * mLx * lc - isr * Wrc[(ci * 2 + 0) * QC + q];
+              WY[ci] = mU[ci] * ly + mL[ci] * mLy * lc - isr * Wrc[(ci * 2 + 1) * QC + q];
+            }
+          const double E0x = W0[q], E0y = W0[QC + q];
+#pragma unroll
+          for (int ci = 0; ci < NE; ++ci)
+            {
+              double al, bx, by;
+              if (ci == 0)
                 {
-                  al = qd[5 * QC + q] + (THERMO ? qd[4 * QC + q] : u0[2 * QC + q] - u0[QC + q]);
-                  bx = eps * E0x;
-                  by = eps * E0y;
+                  if (THERMO)
+                    al = -qd[3 * QC + q] * uu[0];
+                  else
+                    al = uu[NE > 1 ? 1 : 0] - uu[NE > 2 ? 2 : 0];
+                  bx = -eps * WX[0];
+                  by = -eps * WY[0];
                 }
-              else if (c == 1)
+              else if (ci == 1)
                 {
                   const double n0 = u0[QC + q];
-                  al = qd[q];
-                  bx = n0 * mun * E0x - UT * mun * W0[2 * QC + q];
-                  by = n0 * mun * E0y - UT * mun * W0[3 * QC + q];
+                  al = -(qd[QC + q] * uu[1] + qd[2 * QC + q] * uu[NE > 2 ? 2 : 1]);
+                  bx = -mun * E0x * uu[1] + sd * n0 * mun * WX[0] + UT * mun * WX[1];
+                  by = -mun * E0y * uu[1] + sd * n0 * mun * WY[0] + UT * mun * WY[1];
                 }
               else
                 {
                   const double p0 = u0[2 * QC + q];
-                  al = qd[q];
-                  bx = -p0 * mup * E0x - UT * mup * W0[4 * QC + q];
-                  by = -p0 * mup * E0y - UT * mup * W0[5 * QC + q];
+                  al = -(qd[QC + q] * uu[1] + qd[2 * QC + q] * uu[NE > 2 ? 2 : 1]);
+                  bx = mup * E0x * uu[NE > 2 ? 2 : 1] - sd * p0 * mup * WX[0] +
+                       UT * mup * WX[NE > 2 ? 2 : 1];
+                  by = mup * E0y * uu[NE > 2 ? 2 : 1] - sd * p0 * mup * WY[0] +
+                       UT * mup * WY[NE > 2 ? 2 : 1];
                 }
-              resc[(q * NE + ci) * 3 + 0] = al;
-              resc[(q * NE + ci) * 3 + 1] = bx;
-              resc[(q * NE + ci) * 3 + 2] = by;
-            }
-          for (int e = lane; e < NFC * QF * NE; e += 64)
-            {
-              const int fi = e / NE, ci = e % NE, c = NE == 1 ? 0 : ci;
-              const double E0n = W0f[fi];
-              double       v;
-              if (c == 0)
-                v = -eps * E0n - tV * u0f[fi] + tV * tr0[fi];
-              else if (c == 1)
-                {
-                  const double n0 = u0f[NFC * QF + fi];
-                  v = -(n0 * mun * E0n - UT * mun * W0f[NFC * QF + fi]) + tN * n0 -
-                      tN * tr0[NFC * QF + fi];
-                }
-              else
-                {
-                  const double p0 = u0f[2 * NFC * QF + fi];
-                  v = -(-p0 * mup * E0n - UT * mup * W0f[2 * NFC * QF + fi]) + tP * p0 -
-                      tP * tr0[2 * NFC * QF + fi];
-                }
-              resf[e] = v;
-            }
-          for (int e = lane; e < NFC * NE * NF; e += 64)
-            {
-              const int g = e / (NE * NF), ci = (e / NF) % NE, i = e % NF;
-              const int c = NE == 1 ? 0 : ci;
-              double    v = 0;
-              if (isneu[g * 3 + c] && !isdir[g * 3 + c])
+              al += isr * resc[(q * NE + ci) * 3 + 0];
+              bx += isr * resc[(q * NE + ci) * 3 + 1];
+              by += isr * resc[(q * NE + ci) * 3 + 2];
 #pragma unroll
-                for (int s = 0; s < QF; ++s)
-                  v += Bt[(g * QF + s) * NF + i] * p.neuv[(facev[g] * 3 + c) * QF + s];
-              neu[e] = v;
+              for (int i = 0; i < NB; ++i)
+                R[ci * NB + i] += Bz[(q * 3 + 0) * NB + i] * al +
+                                  Bz[(q * 3 + 1) * NB + i] * bx +
+                                  Bz[(q * 3 + 2) * NB + i] * by;
             }
-          WAVE_SYNC();
+        }
 
-          // ---- E: bundle of this lane's column and the forms it feeds.
-          // Branch-free: every table value is loaded (indices valid for all
-          // lanes) and blended with the lane's 0/1 weights, so no per-lane
-          // masks have to be kept alive.
-          const bool   isU = kind == 0, isL = kind == 1, isR = kind == 2;
-          const double isr = isR ? 1.0 : 0.0;
-          const int    ju = isU ? jj : 0, jl = isL ? jj : 0, fl = isL ? ff : 0;
-          double       mU[NE], mL[NE];
+      // ---- 1b: face quadrature points.  The numerical flux F_c of the
+      // column (the HDG flux of component c, use_cell) at face point
+      // (g, s); it feeds the scalar-test rows (phi_i F_c) and the
+      // trace-test rows (xi_i F_c).
+      auto face_flux = [&](int g, int s, double (&F)[NE]) {
+        const int    fi  = g * QF + s;
+        const double ph  = tab[S::tPHIF + fi * NB + ju];
+        const double ld  = tab[S::tLDF + fi * NB + ju];
+        const double lc  = tab[S::tLCF + ((fl * NFC + g) * QF + s) * NF + jl];
+        const double xv  = tab[S::tXI + s * NF + jl] * (fl == g ? 1.0 : 0.0);
+        const double E0n = W0f[fi];
+        const double WN0 = mU[0] * ld + mL[0] * lc - isr * Wrf[fi];
 #pragma unroll
-          for (int ci = 0; ci < NE; ++ci)
-            {
-              mU[ci] = (isU && cc == ci) ? 1.0 : 0.0;
-              mL[ci] = (isL && cc == ci) ? 1.0 : 0.0;
-            }
-          const double mLx = (fl >> 1) == 0 ? 1.0 : 0.0, mLy = 1.0 - mLx;
-          double      *tcl  = ws + S::wTCL;
-          int         *pivs = wi + S::iPIV;
-
-          // ---- 1a: scalar-test rows, cell quadrature points.  q stays a
-          // runtime index: unrolled, the scheduler hoists every (uniform) Bz
-          // load of all quadrature points at once
-          double R[NS];
+        for (int ci = 0; ci < NE; ++ci)
+          {
+            const double uuc = mU[ci] * ph;
+            const double WNc = mU[ci] * ld + mL[ci] * lc - isr * Wrf[ci * NFC * QF + fi];
+            const double LLc = mL[ci] * xv - isr * lamf[ci * NFC * QF + fi];
+            double       Fv;
+            if (ci == 0)
+              Fv = eps * WNc + tV * uuc - tV * LLc;
+            else if (ci == 1)
+              Fv = uuc * mun * E0n + u0f[NFC * QF + fi] * mun * WN0 - UT * mun * WNc -
+                   tN * uuc + tN * LLc;
+            else
+              Fv = -(uuc * mup * E0n + u0f[2 * NFC * QF + fi] * mup * WN0) -
+                   UT * mup * WNc - tP * uuc + tP * LLc;
+            F[ci] = Fv + isr * resf[fi * NE + ci];
+          }
+      };
+      // trace-test rows of face g: xi_i F_c summed over the face points,
+      // Neumann data on the rhs lane, zero rows on Dirichlet faces
+      auto trace_rows = [&](int g, double (&acc)[TDF]) {
 #pragma unroll
-          for (int r = 0; r < NS; ++r)
-            R[r] = 0.0;
+        for (int r = 0; r < TDF; ++r)
+          acc[r] = 0.0;
 #pragma unroll 1
-          for (int q = 0; q < QC; ++q)
-            {
-              double uu[3] = {0, 0, 0}, WX[3] = {0, 0, 0}, WY[3] = {0, 0, 0};
-              const double ph = tab[S::tPHIC + q * NB + ju];
-              const double lx = tab[S::tLDC + (q * 2 + 0) * NB + ju];
-              const double ly = tab[S::tLDC + (q * 2 + 1) * NB + ju];
-              const double lc = tab[S::tLCC + (fl * QC + q) * NF + jl];
-#pragma unroll
-              for (int ci = 0; ci < NE; ++ci)
-                {
-                  uu[ci] = mU[ci] * ph;
-                  WX[ci] = mU[ci] * lx + mL[ci] * mLx * lc - isr * Wrc[(ci * 2 + 0) * QC + q];
-                  WY[ci] = mU[ci] * ly + mL[ci] * mLy * lc - isr * Wrc[(ci * 2 + 1) * QC + q];
-                }
-              const double E0x = W0[q], E0y = W0[QC + q];
-#pragma unroll
-              for (int ci = 0; ci < NE; ++ci)
-                {
-                  double al, bx, by;
-                  if (ci == 0)
-                    {
-                      if (THERMO)
-                        al = -qd[3 * QC + q] * uu[0];
-                      else
-                        al = uu[NE > 1 ? 1 : 0] - uu[NE > 2 ? 2 : 0];
-                      bx = -eps * WX[0];
-                      by = -eps * WY[0];
-                    }
-                  else if (ci == 1)
-                    {
-                      const double n0 = u0[QC + q];
-                      al = -(qd[QC + q] * uu[1] + qd[2 * QC + q] * uu[NE > 2 ? 2 : 1]);
-                      bx = -mun * E0x * uu[1] + sd * n0 * mun * WX[0] + UT * mun * WX[1];
-                      by = -mun * E0y * uu[1] + sd * n0 * mun * WY[0] + UT * mun * WY[1];
-                    }
-                  else
-                    {
-                      const double p0 = u0[2 * QC + q];
-                      al = -(qd[QC + q] * uu[1] + qd[2 * QC + q] * uu[NE > 2 ? 2 : 1]);
-                      bx = mup * E0x * uu[NE > 2 ? 2 : 1] - sd * p0 * mup * WX[0] +
-                           UT * mup * WX[NE > 2 ? 2 : 1];
-                      by = mup * E0y * uu[NE > 2 ? 2 : 1] - sd * p0 * mup * WY[0] +
-                           UT * mup * WY[NE > 2 ? 2 : 1];
-                    }
-                  al += isr * resc[(q * NE + ci) * 3 + 0];
-                  bx += isr * resc[(q * NE + ci) * 3 + 1];
-                  by += isr * resc[(q * NE + ci) * 3 + 2];
-#pragma unroll
-                  for (int i = 0; i < NB; ++i)
-                    R[ci * NB + i] += Bz[(q * 3 + 0) * NB + i] * al +
-                                      Bz[(q * 3 + 1) * NB + i] * bx +
-                                      Bz[(q * 3 + 2) * NB + i] * by;
-                }
-            }
-
-          // ---- 1b: face quadrature points.  The numerical flux F_c of the
-          // column (the HDG flux of component c, use_cell) at face point
-          // (g, s); it feeds the scalar-test rows (phi_i F_c) and the
-          // trace-test rows (xi_i F_c).
-          auto face_flux = [&](int g, int s, double (&F)[NE]) {
-            const int    fi  = g * QF + s;
-            const double ph  = tab[S::tPHIF + fi * NB + ju];
-            const double ld  = tab[S::tLDF + fi * NB + ju];
-            const double lc  = tab[S::tLCF + ((fl * NFC + g) * QF + s) * NF + jl];
-            const double xv  = tab[S::tXI + s * NF + jl] * (fl == g ? 1.0 : 0.0);
-            const double E0n = W0f[fi];
-            const double WN0 = mU[0] * ld + mL[0] * lc - isr * Wrf[fi];
+        for (int s = 0; s < QF; ++s)
+          {
+            double F[NE];
+            face_flux(g, s, F);
 #pragma unroll
             for (int ci = 0; ci < NE; ++ci)
-              {
-                const double uuc = mU[ci] * ph;
-                const double WNc = mU[ci] * ld + mL[ci] * lc - isr * Wrf[ci * NFC * QF + fi];
-                const double LLc = mL[ci] * xv - isr * lamf[ci * NFC * QF + fi];
-                double       Fv;
-                if (ci == 0)
-                  Fv = eps * WNc + tV * uuc - tV * LLc;
-                else if (ci == 1)
-                  Fv = uuc * mun * E0n + u0f[NFC * QF + fi] * mun * WN0 - UT * mun * WNc -
-                       tN * uuc + tN * LLc;
-                else
-                  Fv = -(uuc * mup * E0n + u0f[2 * NFC * QF + fi] * mup * WN0) -
-                       UT * mup * WNc - tP * uuc + tP * LLc;
-                F[ci] = Fv + isr * resf[fi * NE + ci];
-              }
-          };
-          // trace-test rows of face g: xi_i F_c summed over the face points,
-          // Neumann data on the rhs lane, zero rows on Dirichlet faces
-          auto trace_rows = [&](int g, double (&acc)[TDF]) {
 #pragma unroll
-            for (int r = 0; r < TDF; ++r)
-              acc[r] = 0.0;
+              for (int i = 0; i < NF; ++i)
+                acc[ci * NF + i] += Bt[(g * QF + s) * NF + i] * F[ci];
+          }
+#pragma unroll
+        for (int ci = 0; ci < NE; ++ci)
+          {
+            const double keepr = isdir[g * 3 + (NE == 1 ? 0 : ci)] ? 0.0 : 1.0;
+#pragma unroll
+            for (int i = 0; i < NF; ++i)
+              acc[ci * NF + i] = keepr * (acc[ci * NF + i] + isr * neu[(g * NE + ci) * NF + i]);
+          }
+      };
+#pragma unroll
+      for (int g = 0; g < NFC; ++g)
+        {
+          const int a = g >> 1, side = g & 1;
+          double    acc[TDF];
+#pragma unroll
+          for (int r = 0; r < TDF; ++r)
+            acc[r] = 0.0;
 #pragma unroll 1
-            for (int s = 0; s < QF; ++s)
-              {
-                double F[NE];
-                face_flux(g, s, F);
+          for (int s = 0; s < QF; ++s)
+            {
+              const int fi = g * QF + s;
+              double    F[NE];
+              face_flux(g, s, F);
 #pragma unroll
-                for (int ci = 0; ci < NE; ++ci)
+              for (int ci = 0; ci < NE; ++ci)
+                {
 #pragma unroll
-                  for (int i = 0; i < NF; ++i)
-                    acc[ci * NF + i] += Bt[(g * QF + s) * NF + i] * F[ci];
-              }
+                  for (int t = 0; t < K1; ++t)
+                    {
+                      const int i = a == 0 ? side * DEG + K1 * t : t + K1 * side * DEG;
+                      R[ci * NB + i] += Bz[(QC * 3 + fi) * NB + i] * F[ci];
+                    }
+                  if (!PASS2)
+#pragma unroll
+                    for (int i = 0; i < NF; ++i)
+                      acc[ci * NF + i] += Bt[fi * NF + i] * F[ci];
+                }
+            }
+          // ---- 2: TC' (trace-test rows of the scalar columns) into LDS
+          if (!PASS2 && isU)
 #pragma unroll
             for (int ci = 0; ci < NE; ++ci)
               {
                 const double keepr = isdir[g * 3 + (NE == 1 ? 0 : ci)] ? 0.0 : 1.0;
 #pragma unroll
                 for (int i = 0; i < NF; ++i)
-                  acc[ci * NF + i] = keepr * (acc[ci * NF + i] + isr * neu[(g * NE + ci) * NF + i]);
+                  tcl[lane * T + g * TDF + ci * NF + i] = keepr * acc[ci * NF + i];
               }
-          };
+        }
+      // LocalCondenser::condense_ct_matrix: the Dirichlet update is in the
+      // rhs bundle already; the Dirichlet columns are dropped
+      const bool   dircol = isL && isdir[ff * 3 + (NE == 1 ? 0 : cc)];
+      const double keep   = dircol ? 0.0 : 1.0;
 #pragma unroll
-          for (int g = 0; g < NFC; ++g)
-            {
-              const int a = g >> 1, side = g & 1;
-              double    acc[TDF];
-#pragma unroll
-              for (int r = 0; r < TDF; ++r)
-                acc[r] = 0.0;
-#pragma unroll 1
-              for (int s = 0; s < QF; ++s)
-                {
-                  const int fi = g * QF + s;
-                  double    F[NE];
-                  face_flux(g, s, F);
-#pragma unroll
-                  for (int ci = 0; ci < NE; ++ci)
-                    {
-#pragma unroll
-                      for (int t = 0; t < K1; ++t)
-                        {
-                          const int i = a == 0 ? side * DEG + K1 * t : t + K1 * side * DEG;
-                          R[ci * NB + i] += Bz[(QC * 3 + fi) * NB + i] * F[ci];
-                        }
-                      if (!PASS2)
-#pragma unroll
-                        for (int i = 0; i < NF; ++i)
-                          acc[ci * NF + i] += Bt[fi * NF + i] * F[ci];
-                    }
-                }
-              // ---- 2: TC' (trace-test rows of the scalar columns) into LDS
-              if (!PASS2 && isU)
-#pragma unroll
-                for (int ci = 0; ci < NE; ++ci)
-                  {
-                    const double keepr = isdir[g * 3 + (NE == 1 ? 0 : ci)] ? 0.0 : 1.0;
-#pragma unroll
-                    for (int i = 0; i < NF; ++i)
-                      tcl[lane * T + g * TDF + ci * NF + i] = keepr * acc[ci * NF + i];
-                  }
-            }
-          // LocalCondenser::condense_ct_matrix: the Dirichlet update is in the
-          // rhs bundle already; the Dirichlet columns are dropped
-          const bool   dircol = isL && isdir[ff * 3 + (NE == 1 ? 0 : cc)];
-          const double keep   = dircol ? 0.0 : 1.0;
-#pragma unroll
-          for (int r = 0; r < NS; ++r)
-            R[r] *= keep;
+      for (int r = 0; r < NS; ++r)
+        R[r] *= keep;
 
-          // ---- 3: Gauss-Jordan, scaled column pivoting
-          if constexpr (!MIXED)
-            {
-            double cs = 0;
+      // ---- 3: Gauss-Jordan, scaled column pivoting
+      if constexpr (!MIXED)
+        {
+        double cs = 0;
   #pragma unroll
-            for (int r = 0; r < NS; ++r)
-              cs = fmax(cs, fabs(R[r]));
-            cs           = cs > 0 ? 1.0 / cs : 0.0;
-            bool pivoted = !isU;
+        for (int r = 0; r < NS; ++r)
+          cs = fmax(cs, fabs(R[r]));
+        cs           = cs > 0 ? 1.0 / cs : 0.0;
+        bool pivoted = !isU;
   #pragma unroll
-            for (int k = 0; k < NS; ++k)
+        for (int k = 0; k < NS; ++k)
+          {
+            // order-preserving key: non-negative doubles compare as integers
+            const double       sc  = pivoted ? 0.0 : fabs(R[k]) * cs;
+            unsigned long long key = pivoted ? 0ull
+                                             : (((unsigned long long)__double_as_longlong(sc) &
+                                                 ~63ull) | (unsigned long long)(63 - lane)) + 1ull;
+  #pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
               {
-                // order-preserving key: non-negative doubles compare as integers
-                const double       sc  = pivoted ? 0.0 : fabs(R[k]) * cs;
-                unsigned long long key = pivoted ? 0ull
-                                                 : (((unsigned long long)__double_as_longlong(sc) &
-                                                     ~63ull) | (unsigned long long)(63 - lane)) + 1ull;
-  #pragma unroll
-                for (int off = 32; off > 0; off >>= 1)
-                  {
-                    const unsigned long long o = __shfl_xor(key, off);
-                    key                        = o > key ? o : key;
-                  }
-                const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane((unsigned)(key - 1ull))) & 63u);
-                if (lane == pl)
-                  {
-                    pivoted = true;
-  #pragma unroll
-                    for (int r = 0; r < NS; ++r)
-                      bc[r] = R[r];
-                  }
-                if (lane == 0)
-                  pivs[k] = pl;
-                WAVE_SYNC();
-                const double f = R[k] / bc[k];
+                const unsigned long long o = __shfl_xor(key, off);
+                key                        = o > key ? o : key;
+              }
+            const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane((unsigned)(key - 1ull))) & 63u);
+            if (lane == pl)
+              {
+                pivoted = true;
   #pragma unroll
                 for (int r = 0; r < NS; ++r)
-                  if (r != k)
-                    R[r] -= bc[r] * f;
-                R[k] = f;
-                WAVE_SYNC();
+                  bc[r] = R[r];
               }
+            if (lane == 0)
+              pivs[k] = pl;
+            WAVE_SYNC();
+            const double f = R[k] / bc[k];
+  #pragma unroll
+            for (int r = 0; r < NS; ++r)
+              if (r != k)
+                R[r] -= bc[r] * f;
+            R[k] = f;
+            WAVE_SYNC();
+          }
 
-            }
-          else
+        }
+      else
+        {
+          // FP32 elimination with one FP64 refinement step: x0 from the
+          // FP32 elimination, r = b - A' x0 in FP64 (the scalar lanes keep
+          // their FP64 columns of A' untouched in R), the correction from
+          // a second FP32 elimination with the same pivots, x = x0 + dx
+          float *bcf = reinterpret_cast<float *>(bc);
+          float  R32[NS], X0[NS];
+          float  cs32 = 0.f;
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
             {
-              // FP32 elimination with one FP64 refinement step: x0 from the
-              // FP32 elimination, r = b - A' x0 in FP64 (the scalar lanes keep
-              // their FP64 columns of A' untouched in R), the correction from
-              // a second FP32 elimination with the same pivots, x = x0 + dx
-              float *bcf = reinterpret_cast<float *>(bc);
-              float  R32[NS], X0[NS];
-              float  cs32 = 0.f;
-#pragma unroll
-              for (int r = 0; r < NS; ++r)
-                {
-                  R32[r] = (float)R[r];
-                  cs32   = fmaxf(cs32, fabsf(R32[r]));
-                }
-              cs32         = cs32 > 0.f ? 1.f / cs32 : 0.f;
-              bool pivoted = !isU;
-#pragma unroll
-              for (int k = 0; k < NS; ++k)
-                {
-                  const float sc  = pivoted ? 0.f : fabsf(R32[k]) * cs32;
-                  unsigned    key = pivoted ? 0u
-                                            : ((__float_as_uint(sc) & ~63u) | (unsigned)(63 - lane)) + 1u;
-#pragma unroll
-                  for (int off = 32; off > 0; off >>= 1)
-                    {
-                      const unsigned o = __shfl_xor(key, off);
-                      key              = o > key ? o : key;
-                    }
-                  const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane(key - 1u)) & 63u);
-                  if (lane == pl)
-                    {
-                      pivoted = true;
-#pragma unroll
-                      for (int r = 0; r < NS; ++r)
-                        bcf[r] = R32[r];
-                    }
-                  if (lane == 0)
-                    pivs[k] = pl;
-                  WAVE_SYNC();
-                  const float f = R32[k] / bcf[k];
-#pragma unroll
-                  for (int r = 0; r < NS; ++r)
-                    if (r != k)
-                      R32[r] -= bcf[r] * f;
-                  R32[k] = f;
-                  WAVE_SYNC();
-                }
-#pragma unroll
-              for (int k = 0; k < NS; ++k)
-                X0[k] = R32[k];
-              const double mres = isU ? 0.0 : 1.0;
-#pragma unroll
-              for (int k = 0; k < NS; ++k)
-                {
-                  if (lane == pivs[k])
-#pragma unroll
-                    for (int r = 0; r < NS; ++r)
-                      bc[r] = R[r];
-                  WAVE_SYNC();
-                  const double xk = mres * (double)X0[k];
-#pragma unroll
-                  for (int r = 0; r < NS; ++r)
-                    R[r] -= bc[r] * xk;
-                  WAVE_SYNC();
-                }
-#pragma unroll
-              for (int r = 0; r < NS; ++r)
-                R32[r] = (float)R[r];
-#pragma unroll
-              for (int k = 0; k < NS; ++k)
-                {
-                  if (lane == pivs[k])
-#pragma unroll
-                    for (int r = 0; r < NS; ++r)
-                      bcf[r] = R32[r];
-                  WAVE_SYNC();
-                  const float f = R32[k] / bcf[k];
-#pragma unroll
-                  for (int r = 0; r < NS; ++r)
-                    if (r != k)
-                      R32[r] -= bcf[r] * f;
-                  R32[k] = f;
-                  WAVE_SYNC();
-                }
-#pragma unroll
-              for (int k = 0; k < NS; ++k)
-                R[k] = (double)X0[k] + (double)R32[k];
+              R32[r] = (float)R[r];
+              cs32   = fmaxf(cs32, fabsf(R32[r]));
             }
-
-          if (PASS2)
-            {
-              // the rhs lane holds u(p_k) in row k: publish u, then
-              // W = M^-1 (rhs_q + D_a u - sum_f C_{a,f} lambda_upd,f) and u
-              // go out lane-parallel (reference template.h:2688-2697)
-              double *sol = bc;
-              if (isR)
+          cs32         = cs32 > 0.f ? 1.f / cs32 : 0.f;
+          bool pivoted = !isU;
 #pragma unroll
-                for (int k = 0; k < NS; ++k)
-                  sol[pivs[k]] = R[k];
+          for (int k = 0; k < NS; ++k)
+            {
+              const float sc  = pivoted ? 0.f : fabsf(R32[k]) * cs32;
+              unsigned    key = pivoted ? 0u
+                                        : ((__float_as_uint(sc) & ~63u) | (unsigned)(63 - lane)) + 1u;
+#pragma unroll
+              for (int off = 32; off > 0; off >>= 1)
+                {
+                  const unsigned o = __shfl_xor(key, off);
+                  key              = o > key ? o : key;
+                }
+              const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane(key - 1u)) & 63u);
+              if (lane == pl)
+                {
+                  pivoted = true;
+#pragma unroll
+                  for (int r = 0; r < NS; ++r)
+                    bcf[r] = R32[r];
+                }
+              if (lane == 0)
+                pivs[k] = pl;
               WAVE_SYNC();
-              for (int e = lane; e < S::NCD; e += 64)
+              const float f = R32[k] / bcf[k];
+#pragma unroll
+              for (int r = 0; r < NS; ++r)
+                if (r != k)
+                  R32[r] -= bcf[r] * f;
+              R32[k] = f;
+              WAVE_SYNC();
+            }
+#pragma unroll
+          for (int k = 0; k < NS; ++k)
+            X0[k] = R32[k];
+          const double mres = isU ? 0.0 : 1.0;
+#pragma unroll
+          for (int k = 0; k < NS; ++k)
+            {
+              if (lane == pivs[k])
+#pragma unroll
+                for (int r = 0; r < NS; ++r)
+                  bc[r] = R[r];
+              WAVE_SYNC();
+              const double xk = mres * (double)X0[k];
+#pragma unroll
+              for (int r = 0; r < NS; ++r)
+                R[r] -= bc[r] * xk;
+              WAVE_SYNC();
+            }
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            R32[r] = (float)R[r];
+#pragma unroll
+          for (int k = 0; k < NS; ++k)
+            {
+              if (lane == pivs[k])
+#pragma unroll
+                for (int r = 0; r < NS; ++r)
+                  bcf[r] = R32[r];
+              WAVE_SYNC();
+              const float f = R32[k] / bcf[k];
+#pragma unroll
+              for (int r = 0; r < NS; ++r)
+                if (r != k)
+                  R32[r] -= bcf[r] * f;
+              R32[k] = f;
+              WAVE_SYNC();
+            }
+#pragma unroll
+          for (int k = 0; k < NS; ++k)
+            R[k] = (double)X0[k] + (double)R32[k];
+        }
+
+      if (PASS2)
+        {
+          // the rhs lane holds u(p_k) in row k: publish u, then
+          // W = M^-1 (rhs_q + D_a u - sum_f C_{a,f} lambda_upd,f) and u
+          // go out lane-parallel (reference template.h:2688-2697)
+          double *sol = bc;
+          if (isR)
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+              sol[pivs[k]] = R[k];
+          WAVE_SYNC();
+          for (int e = lane; e < S::NCD; e += 64)
+            {
+              const int c = e / (3 * NB), r = e % (3 * NB), blkc = r / NB, i = r % NB;
+              double    v = 0;
+              if (c < NE)
                 {
-                  const int c = e / (3 * NB), r = e % (3 * NB), blkc = r / NB, i = r % NB;
-                  double    v = 0;
-                  if (c < NE)
+                  const double *u = sol + c * NB;
+                  if (blkc == 2)
+                    v = u[i];
+                  else
                     {
-                      const double *u = sol + c * NB;
-                      if (blkc == 2)
-                        v = u[i];
-                      else
+                      const int a = blkc;
+#pragma unroll
+                      for (int l = 0; l < NB; ++l)
+                        v += tb.Minv[i * NB + l] * rhsq[(c * 2 + a) * NB + l] +
+                             tb.MinvD[(a * NB + i) * NB + l] * u[l];
+#pragma unroll
+                      for (int sd2 = 0; sd2 < 2; ++sd2)
                         {
-                          const int a = blkc;
+                          const int f = 2 * a + sd2;
 #pragma unroll
-                          for (int l = 0; l < NB; ++l)
-                            v += tb.Minv[i * NB + l] * rhsq[(c * 2 + a) * NB + l] +
-                                 tb.MinvD[(a * NB + i) * NB + l] * u[l];
-#pragma unroll
-                          for (int sd2 = 0; sd2 < 2; ++sd2)
-                            {
-                              const int f = 2 * a + sd2;
-#pragma unroll
-                              for (int m = 0; m < NF; ++m)
-                                v -= tb.MinvC[(f * NB + i) * NF + m] * lupd[(f * 3 + c) * NF + m];
-                            }
+                          for (int m = 0; m < NF; ++m)
+                            v -= tb.MinvC[(f * NB + i) * NF + m] * lupd[(f * 3 + c) * NF + m];
                         }
                     }
-                  p.dxc[cell * S::NCD + e] = v;
                 }
-              WAVE_SYNC();
-              continue;
-            }
-          // ---- 4: [S | g] = [TT' | t'] - TC' X, face by face
-#pragma unroll 1
-          for (int g = 0; g < NFC; ++g)
-            {
-              double acc[TDF];
-              trace_rows(g, acc);
-#pragma unroll
-              for (int k = 0; k < NS; ++k)
-                {
-                  const double *tc = tcl + pivs[k] * T + g * TDF;
-                  const double  x  = R[k];
-#pragma unroll
-                  for (int r = 0; r < TDF; ++r)
-                    acc[r] -= tc[r] * x;
-                }
-              if (mode == 1)
-                {
-                  if (isL)
-#pragma unroll
-                    for (int r = 0; r < TDF; ++r)
-                      p.Sout[(long long)idx * T * T + (g * TDF + r) * T + (lane - NS)] =
-                        dircol ? 0.0 : acc[r];
-                  if (isR)
-#pragma unroll
-                    for (int r = 0; r < TDF; ++r)
-                      p.gout[(long long)idx * T + g * TDF + r] = acc[r];
-                  continue;
-                }
-              if (isL)
-                {
-                  const int lt = lane - NS, gj = lt / TDF, rj = lt % TDF;
-                  const long long base = blk[g] * TDF * TDF + (long long)slot[g * NFC + gj] * TDF + rj;
-                  const long long rs   = (long long)nbk[g] * TDF;
-                  const bool      addd = color == 1 && g == gj && bidv[g] < 0;
-#pragma unroll
-                  for (int r = 0; r < TDF; ++r)
-                    {
-                      const int  i   = g * TDF + r, ci = r / NF;
-                      const bool dir = isdir[g * 3 + (NE == 1 ? 0 : ci)];
-                      const double v = dir ? (i == lt ? 1.0 : 0.0) : (dircol ? 0.0 : acc[r]);
-                      const long long pos = base + r * rs;
-                      if (addd)
-                        p.vals[pos] += v;
-                      else
-                        p.vals[pos] = v;
-                    }
-                }
-              if (isR)
-                {
-                  const bool addr = color == 1 && bidv[g] < 0;
-#pragma unroll
-                  for (int r = 0; r < TDF; ++r)
-                    {
-                      const int  ci   = r / NF;
-                      const int  comp = NE == 1 ? 0 : ci;
-                      const bool dir  = isdir[g * 3 + comp];
-                      const long long row = facev[g] * TDF + r;
-                      const double v = dir ? 0.0 : acc[r];
-                      if (addr)
-                        p.rhs[row] += v;
-                      else
-                        p.rhs[row] = v;
-                      if (dir)
-                        p.dirout[row] = lupd[(g * 3 + comp) * NF + (r % NF)];
-                    }
-                }
+              p.dxc[cell * S::NCD + e] = v;
             }
           WAVE_SYNC();
+          return;
         }
+      // ---- 4: [S | g] = [TT' | t'] - TC' X, face by face
+#pragma unroll 1
+      for (int g = 0; g < NFC; ++g)
+        {
+          double acc[TDF];
+          trace_rows(g, acc);
+#pragma unroll
+          for (int k = 0; k < NS; ++k)
+            {
+              const double *tc = tcl + pivs[k] * T + g * TDF;
+              const double  x  = R[k];
+#pragma unroll
+              for (int r = 0; r < TDF; ++r)
+                acc[r] -= tc[r] * x;
+            }
+          if (mode == 1)
+            {
+              if (isL)
+#pragma unroll
+                for (int r = 0; r < TDF; ++r)
+                  p.Sout[(long long)idx * T * T + (g * TDF + r) * T + (lane - NS)] =
+                    dircol ? 0.0 : acc[r];
+              if (isR)
+#pragma unroll
+                for (int r = 0; r < TDF; ++r)
+                  p.gout[(long long)idx * T + g * TDF + r] = acc[r];
+              continue;
+            }
+          if (isL)
+            {
+              const int lt = lane - NS, gj = lt / TDF, rj = lt % TDF;
+              const long long base = blk[g] * TDF * TDF + (long long)slot[g * NFC + gj] * TDF + rj;
+              const long long rs   = (long long)nbk[g] * TDF;
+              const bool      addd = color == 1 && g == gj && bidv[g] < 0;
+#pragma unroll
+              for (int r = 0; r < TDF; ++r)
+                {
+                  const int  i   = g * TDF + r, ci = r / NF;
+                  const bool dir = isdir[g * 3 + (NE == 1 ? 0 : ci)];
+                  const double v = dir ? (i == lt ? 1.0 : 0.0) : (dircol ? 0.0 : acc[r]);
+                  const long long pos = base + r * rs;
+                  if (addd)
+                    p.vals[pos] += v;
+                  else
+                    p.vals[pos] = v;
+                }
+            }
+          if (isR)
+            {
+              const bool addr = color == 1 && bidv[g] < 0;
+#pragma unroll
+              for (int r = 0; r < TDF; ++r)
+                {
+                  const int  ci   = r / NF;
+                  const int  comp = NE == 1 ? 0 : ci;
+                  const bool dir  = isdir[g * 3 + comp];
+                  const long long row = facev[g] * TDF + r;
+                  const double v = dir ? 0.0 : acc[r];
+                  if (addr)
+                    p.rhs[row] += v;
+                  else
+                    p.rhs[row] = v;
+                  if (dir)
+                    p.dirout[row] = lupd[(g * 3 + comp) * NF + (r % NF)];
+                }
+            }
+        }
+      WAVE_SYNC();
+    }
+
+    // PASS2: the reconstruction pass (cells = all cells in order, columns =
+    // scalar dofs + rhs with the trace update folded in, output dx_cell)
+    // MIXED (pass 1 only): the elimination of the scalar block in FP32 with one
+    // FP64 refinement step; assembly, residual, refinement residual and Schur
+    // product in FP64
+    template <int DEG, int NE, bool THERMO, bool PASS2, bool MIXED = false>
+    __global__ void __launch_bounds__(64 * WPB)
+      __attribute__((amdgpu_waves_per_eu(PASS2 ? WPE2 : WPE)))
+      fast_condense(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
+    {
+      using S            = F2<DEG, NE>;
+      constexpr int WSTR = PASS2 ? S::WDBL2 : S::WDBL;
+      extern __shared__ double smem[];
+      stage_tables<DEG, NE>(p.tb, smem, threadIdx.x);
+      __syncthreads();
+      const int wv = threadIdx.x >> 6;
+      for (int idx = blockIdx.x * WPB + wv; idx < ncell; idx += gridDim.x * WPB)
+        {
+          const long long cell = PASS2 ? (long long)idx : (long long)cells[idx];
+          cell_body<DEG, NE, THERMO, PASS2, MIXED>(p, smem, WSTR, cell, idx, color, mode);
+        }
+    }
+
+
+    // Pass 1 over the cells of several meshes (devices) of degree 1 or 2 in
+    // one persistent launch per colour: the work list holds chunks of WPB
+    // cells of one device (padded with -1 at device boundaries); a workgroup
+    // restages the reference-cell tables when its chunk's device changes.
+    // LDS is sized for degree 2; a degree-1 device uses the first part.
+    template <int NE, bool THERMO, bool MIXED>
+    __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(WPE)))
+      batch_condense(const KParams *__restrict__ plist, const int *__restrict__ chunk_dev,
+                     const int *__restrict__ items, int nchunks, int color)
+    {
+      extern __shared__ double smem[];
+      const int wv  = threadIdx.x >> 6;
+      int       cur = -1, deg = 0;
+      for (int c = blockIdx.x; c < nchunks; c += gridDim.x)
+        {
+          const int      dev = chunk_dev[c];
+          const KParams &p   = plist[dev];
+          if (dev != cur)
+            {
+              __syncthreads();
+              deg = p.deg;
+              if (deg == 1)
+                stage_tables<1, NE>(p.tb, smem, threadIdx.x);
+              else
+                stage_tables<2, NE>(p.tb, smem, threadIdx.x);
+              __syncthreads();
+              cur = dev;
+            }
+          const int cell = items[c * WPB + wv];
+          if (cell < 0)
+            continue;
+          if (deg == 1)
+            cell_body<1, NE, THERMO, false, MIXED>(p, smem, F2<1, NE>::WDBL, cell, cell, color, 0);
+          else
+            cell_body<2, NE, THERMO, false, MIXED>(p, smem, F2<2, NE>::WDBL, cell, cell, color, 0);
+        }
+    }
+
+    template <int NE, bool THERMO, bool MIXED>
+    hipError_t launch_batch_t(const KParams *plist, const int *chunk_dev, const int *items,
+                              int nchunks, int color, int grid, hipStream_t s)
+    {
+      constexpr size_t bytes = F2<2, NE>::bytes_for(false);
+      static bool      attr  = false;
+      if (!attr)
+        {
+          hipError_t e = hipFuncSetAttribute((const void *)batch_condense<NE, THERMO, MIXED>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)bytes);
+          if (e != hipSuccess)
+            return e;
+          attr = true;
+        }
+      if (nchunks <= 0)
+        return hipSuccess;
+      hipLaunchKernelGGL((batch_condense<NE, THERMO, MIXED>), dim3(grid), dim3(64 * WPB), bytes,
+                         s, plist, chunk_dev, items, nchunks, color);
+      return hipGetLastError();
     }
 
     template <int DEG, int NE, bool THERMO, bool PASS2 = false, bool MIXED = false>
@@ -933,6 +1014,24 @@ namespace hdg
                       : launch_fast<2, 1, true>(p, cells, ncell, color, mode, s);
     return deg == 1 ? launch_fast<1, 3, false>(p, cells, ncell, color, mode, s)
                     : launch_fast<2, 3, false>(p, cells, ncell, color, mode, s);
+  }
+
+  int batch_waves_per_group()
+  {
+    return WPB;
+  }
+
+  hipError_t launch_batch_condense(int ne, int thermo, int mixed, const KParams *plist,
+                                   const int *chunk_dev, const int *items, int nchunks,
+                                   int color, int grid, hipStream_t s)
+  {
+    if (thermo)
+      return mixed ? launch_batch_t<1, true, true>(plist, chunk_dev, items, nchunks, color, grid, s)
+                   : launch_batch_t<1, true, false>(plist, chunk_dev, items, nchunks, color, grid, s);
+    if (ne != 3)
+      return hipErrorInvalidValue;
+    return mixed ? launch_batch_t<3, false, true>(plist, chunk_dev, items, nchunks, color, grid, s)
+                 : launch_batch_t<3, false, false>(plist, chunk_dev, items, nchunks, color, grid, s);
   }
 
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s)

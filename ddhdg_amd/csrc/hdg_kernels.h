@@ -31,6 +31,7 @@ namespace hdg
     int    bc[6][3];
     double sdrift; // +1 reference linearisation, -1 consistent
     int    mixed;  // 1: FP32 elimination + FP64 refinement in the register kernel's pass 1
+    int    deg;    // polynomial degree (batched launches mix degrees)
     DevTables tb;
     // per-mesh data
     const double *doping;   // ncells*QC
@@ -61,6 +62,14 @@ namespace hdg
   // numerical flux of n, out[1] of p (device, dimensionless)
   hipError_t launch_current(int dim, int nb, int nf, int qf, const KParams &p, int bid,
                             double *out, hipStream_t s);
+  // pass 1 of several meshes in one persistent launch per colour (register
+  // kernel, degree 1 or 2): plist (device) holds one parameter block per
+  // mesh, chunk_dev / items (device) the work list of nchunks chunks of
+  // batch_waves_per_group() cells
+  int        batch_waves_per_group();
+  hipError_t launch_batch_condense(int ne, int thermo, int mixed, const KParams *plist,
+                                   const int *chunk_dev, const int *items, int nchunks,
+                                   int color, int grid, hipStream_t s);
   // register-resident pass 2 (same shapes as launch_fast_condense)
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s);
 } // namespace hdg

@@ -28,6 +28,7 @@ EXPORTS = [
     "hdg_trace_solver_create", "hdg_trace_solver_destroy", "hdg_trace_solver_info",
     "hdg_trace_factor", "hdg_trace_factor_status", "hdg_trace_solve",
     "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
+    "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_condense", "hdg_batch_info",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -86,6 +87,10 @@ def lib():
             "hdg_trace_plan_selftest": (i, [i, ip, i, ct.c_uint, dp, dp]),
             "hdg_terminal_current": (i, [v, v, v, i, v, v]),
             "hdg_set_precision": (i, [v, i]),
+            "hdg_batch_create": (i, [P(v), i, P(v)]),
+            "hdg_batch_destroy": (None, [v]),
+            "hdg_batch_condense": (i, [v, i, P(v), P(v), P(v), P(v), P(v), v]),
+            "hdg_batch_info": (i, [v, dp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -263,3 +268,38 @@ def trace_plan_selftest(dim, n_cells, dofs_per_face, seed=1, csr_vals=None):
     _check(lib().hdg_trace_plan_selftest(int(dim), n, int(dofs_per_face), int(seed),
                                          _dp(vals) if vals is not None else None, _dp(out)))
     return float(out[0]), int(out[1]), int(out[2])
+
+
+class HdgBatch:
+    """pass 1 of several contexts (2D, degree 1/2, same components) in one
+    persistent launch per colour; the grid is autotuned on the first call."""
+
+    def __init__(self, contexts):
+        self.L = lib()
+        self.ctxs = list(contexts)
+        arr = (ct.c_void_p * len(self.ctxs))(*[c.ctx.value for c in self.ctxs])
+        self.h = ct.c_void_p()
+        _check(self.L.hdg_batch_create(arr, len(self.ctxs), ct.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            self.L.hdg_batch_destroy(self.h)
+            self.h = ct.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def condense(self, thermodyn, x_cells, x_traces, vals, rhs, dirv):
+        n = len(self.ctxs)
+        arr = lambda ts: (ct.c_void_p * n)(*[t.data_ptr() for t in ts])
+        _check(self.L.hdg_batch_condense(self.h, int(thermodyn), arr(x_cells), arr(x_traces),
+                                         arr(vals), arr(rhs), arr(dirv), _stream()))
+
+    def info(self):
+        out = np.zeros(6)
+        _check(self.L.hdg_batch_info(self.h, _dp(out)))
+        return {"cells": int(out[0]), "grid": int(out[1]), "cus": int(out[2]),
+                "tuned_ms": [float(x) for x in out[3:]]}

@@ -193,6 +193,27 @@ int hdg_terminal_current(hdg_ctx *ctx, const double *x_cell,
                          const double *x_trace, int boundary_id, double *out,
                          void *stream);
 
+/* ---- Batched pass 1 over several meshes --------------------------------
+ * Pass 1 (hdg_condense) of several contexts (2D, degree 1 or 2 mixed, the
+ * same enabled components) in one persistent launch per colour; each
+ * context's outputs are exactly those of its own hdg_condense.  The
+ * persistent grid is picked by timing 1, 2 and 4 workgroups per CU on the
+ * first call (hdg_batch_info reports the choice).  Pointer arguments are
+ * host arrays of n device pointers, in the order of ctxs.  No counterpart in
+ * the reference (one mesh per NPSolver). */
+typedef struct hdg_batch hdg_batch;
+
+int  hdg_batch_create(hdg_ctx *const *ctxs, int n, hdg_batch **out);
+void hdg_batch_destroy(hdg_batch *batch);
+int  hdg_batch_condense(hdg_batch *batch, int thermodyn,
+                        const double *const *x_cell,
+                        const double *const *x_trace, double *const *csr_vals,
+                        double *const *rhs, double *const *dir_vals,
+                        void *stream);
+/* out = {cells, persistent workgroups chosen, CUs, ms of the three timed
+ * candidates} */
+int  hdg_batch_info(const hdg_batch *batch, double out[6]);
+
 /* ---- Global trace solve ------------------------------------------------
  * Sparse direct LU of the restricted trace system, on the GPU: a nested-
  * dissection multifrontal factorisation over the structured cell grid
