@@ -269,6 +269,24 @@ def main():
         newton_ms = (time.perf_counter() - w0) * 1e3 / n_newton
         phase_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in phases.items()}
 
+    graph_step_ms = None
+    if n_newton > 0:
+        # the same Newton iteration replayed from a captured hipGraph
+        from ddhdg_amd.np_solver import NewtonIteration
+        x0 = (s.x_cell.clone(), s.x_trace.clone())
+        it = NewtonIteration(s, False, 1e-10, 1e-10, 100, graph=True)
+        for _ in range(2):                      # eager, then capture + replay
+            it.launch()
+        it.sync()
+        gw = time.perf_counter()
+        for _ in range(n_newton):
+            it.launch()
+        it.sync()
+        graph_step_ms = (time.perf_counter() - gw) * 1e3 / n_newton
+        s.x_cell.copy_(x0[0])
+        s.x_trace.copy_(x0[1])
+        del it
+
     launch_ms = kernel_ms / 2.0
     cells_per_launch = cells / 2.0
     achieved_tflops = flops_per_cell(N, T) * cells_per_launch / (launch_ms * 1e-3) / 1e12
@@ -301,6 +319,7 @@ def main():
     if n_newton > 0:
         line["newton_step"] = {
             "ms": newton_ms, "steps": n_newton, "phase_ms": phase_ms,
+            "graph_replay_ms": graph_step_ms,
             "trace_solver": {"levels": ts.levels, "max_front": ts.max_front,
                              "front_GB": ts.bytes / 1e9, "factor_GFLOP": ts.flops / 1e9,
                              "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12}}
@@ -313,9 +332,15 @@ def main():
         from ddhdg_amd.serving import IVServer, summarize, synthetic_requests
         reqs = synthetic_requests(args.serve, seed=20261204, rate=args.serve_rate,
                                   sizes=(32, 96), degrees=(1, 2), points=13, vmax=0.6)
-        srv = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots)
+        srv = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots, graph=True)
         res, swall = srv.run(reqs)
+        eager = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots, graph=False)
+        res_e, wall_e = eager.run(reqs)
         line["serving"] = {**summarize(res, swall), "max_concurrent": srv.max_active,
+                           "graph_newton": True,
+                           "without_graphs": {k: v for k, v in summarize(res_e, wall_e).items()
+                                              if k in ("solves_per_s", "p50_latency_s",
+                                                       "p99_latency_s")},
                            "workload": f"{args.serve} I-V sweeps (13 points 0-0.6 V), 2D, "
                                        f"{32}^2-{96}^2 cells log-uniform, degree 1-2, Poisson "
                                        f"arrivals {args.serve_rate}/s, FP64",

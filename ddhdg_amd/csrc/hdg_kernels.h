@@ -58,6 +58,9 @@ namespace hdg
   bool       fast_supported(int dim, int deg, int ne, int thermo, const int en[3]);
   hipError_t launch_fast_condense(int deg, const KParams &p, const int *cells, int ncell,
                                   int color, int mode, hipStream_t s);
+  // out[v] = max |a[v]_k| (device); scratch: 4*256 doubles
+  hipError_t launch_norms(const double *const a[4], const long long n[4], double *scratch,
+                          double *out, hipStream_t s);
   // terminal current through boundary id bid: out[0] = integral of the
   // numerical flux of n, out[1] of p (device, dimensionless)
   hipError_t launch_current(int dim, int nb, int nf, int qf, const KParams &p, int bid,

@@ -807,6 +807,69 @@ namespace hdg
     }
   } // namespace
 
+  namespace
+  {
+    // max |a_k| of four device vectors: block partial maxima, then one
+    // workgroup (max is exact in any order: deterministic)
+    constexpr int NRM_T = 256, NRM_B = 256;
+
+    __global__ void __launch_bounds__(NRM_T)
+      norms_partial(const double *a0, long long n0, const double *a1, long long n1,
+                    const double *a2, long long n2, const double *a3, long long n3,
+                    double *part)
+    {
+      __shared__ double red[NRM_T];
+      const double   *a[4] = {a0, a1, a2, a3};
+      const long long n[4] = {n0, n1, n2, n3};
+      for (int v = 0; v < 4; ++v)
+        {
+          double m = 0;
+          for (long long i = blockIdx.x * (long long)NRM_T + threadIdx.x; i < n[v];
+               i += (long long)gridDim.x * NRM_T)
+            m = fmax(m, fabs(a[v][i]));
+          red[threadIdx.x] = m;
+          __syncthreads();
+          for (int off = NRM_T / 2; off > 0; off >>= 1)
+            {
+              if ((int)threadIdx.x < off)
+                red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+              __syncthreads();
+            }
+          if (threadIdx.x == 0)
+            part[v * NRM_B + blockIdx.x] = red[0];
+          __syncthreads();
+        }
+    }
+
+    __global__ void __launch_bounds__(NRM_T) norms_final(const double *part, double *out)
+    {
+      __shared__ double red[NRM_T];
+      for (int v = 0; v < 4; ++v)
+        {
+          red[threadIdx.x] = threadIdx.x < NRM_B ? part[v * NRM_B + threadIdx.x] : 0.0;
+          __syncthreads();
+          for (int off = NRM_T / 2; off > 0; off >>= 1)
+            {
+              if ((int)threadIdx.x < off)
+                red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+              __syncthreads();
+            }
+          if (threadIdx.x == 0)
+            out[v] = red[0];
+          __syncthreads();
+        }
+    }
+  } // namespace
+
+  hipError_t launch_norms(const double *const a[4], const long long n[4], double *scratch,
+                          double *out, hipStream_t s)
+  {
+    hipLaunchKernelGGL(norms_partial, dim3(NRM_B), dim3(NRM_T), 0, s, a[0], n[0], a[1], n[1],
+                       a[2], n[2], a[3], n[3], scratch);
+    hipLaunchKernelGGL(norms_final, dim3(1), dim3(NRM_T), 0, s, scratch, out);
+    return hipGetLastError();
+  }
+
   hipError_t launch_current(int dim, int nb, int nf, int qf, const KParams &p, int bid,
                             double *out, hipStream_t s)
   {

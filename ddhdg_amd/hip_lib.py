@@ -29,6 +29,7 @@ EXPORTS = [
     "hdg_trace_factor", "hdg_trace_factor_status", "hdg_trace_solve",
     "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
     "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_condense", "hdg_batch_info",
+    "hdg_max_abs4",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -91,6 +92,7 @@ def lib():
             "hdg_batch_destroy": (None, [v]),
             "hdg_batch_condense": (i, [v, i, P(v), P(v), P(v), P(v), P(v), v]),
             "hdg_batch_info": (i, [v, dp]),
+            "hdg_max_abs4": (i, [P(v), lp, v, v, v]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -303,3 +305,10 @@ class HdgBatch:
         _check(self.L.hdg_batch_info(self.h, _dp(out)))
         return {"cells": int(out[0]), "grid": int(out[1]), "cus": int(out[2]),
                 "tuned_ms": [float(x) for x in out[3:]]}
+
+
+def max_abs4(vectors, scratch, out):
+    """out[v] = max |vectors[v]| (four device tensors; scratch: 1024 doubles)"""
+    arr = (ct.c_void_p * 4)(*[t.data_ptr() for t in vectors])
+    lens = (ct.c_longlong * 4)(*[t.numel() for t in vectors])
+    _check(lib().hdg_max_abs4(arr, lens, _ptr(scratch), _ptr(out), _stream()))

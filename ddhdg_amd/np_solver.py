@@ -22,7 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import fe
-from .hip_lib import HdgContext, MeshDesc, Phys, TraceSolver
+from .hip_lib import HdgContext, MeshDesc, Phys, TraceSolver, max_abs4
 from .problem import (DIRICHLET, EV, KB, NEUMANN, Q, Adimensionalizer,
                       FixedTauNPSolverParameters, LinearRecombinationTerm,
                       ShockleyReadHallFixedTemperature, compute_density, evaluate,
@@ -301,7 +301,7 @@ class NPSolver:
         it = NewtonIteration(self, thermodyn, abs_tol, rel_tol, max_it, log)
         while True:
             it.launch()
-            self.torch.cuda.current_stream().synchronize()
+            it.sync()
             if it.finish() != "continue":
                 break
         return it.result()
@@ -360,8 +360,18 @@ class NewtonIteration:
     1521-1560) and returns "converged", "continue" or "failed" (iteration
     limit)."""
 
-    def __init__(self, solver, thermodyn, abs_tol, rel_tol, max_it, log=None):
+    def __init__(self, solver, thermodyn, abs_tol, rel_tol, max_it, log=None, graph=None):
         t = solver.torch
+        # hipGraph: the first iteration runs eagerly (it also creates the
+        # trace solver and the BLAS workspaces), the second is captured and
+        # every later one replays it; all buffers below are fixed for the
+        # object's lifetime and boundary data are updated in place
+        self.use_graph = solver.parameters.graph_newton if graph is None else graph
+        self.graph = None
+        # capture needs a stream other than the default one
+        cur = t.cuda.current_stream()
+        self.stream = (t.cuda.Stream() if self.use_graph and cur == t.cuda.default_stream()
+                       else cur)
         self.s, self.t = solver, t
         self.thermodyn, self.abs_tol, self.rel_tol = thermodyn, abs_tol, rel_tol
         self.max_it = max_it if max_it >= 0 else 1 << 30
@@ -376,10 +386,36 @@ class NewtonIteration:
         self.upd_cell = t.zeros_like(solver.x_cell)
         self.dirs = t.as_tensor(dirs, device=dev)
         self.norms = t.zeros(4, dtype=t.float64).pin_memory()
+        self.dnorms = t.zeros(4, dtype=t.float64, device=dev)
+        self.nscratch = t.empty(1024, dtype=t.float64, device=dev)
         self.alpha = solver.parameters.nonlinear_parameters.alpha
         self.step, self.converged, self.upd_cell_norm = 0, False, 0.0
 
+    def reset(self):
+        """a new Newton solve with the same buffers (and captured graph): the
+        boundary data may have changed in place, the components and the
+        operator not"""
+        self.step, self.converged, self.upd_cell_norm = 0, False, 0.0
+
     def launch(self):
+        t = self.t
+        if self.stream != t.cuda.current_stream():
+            self.stream.wait_stream(t.cuda.current_stream())
+        with t.cuda.stream(self.stream):
+            if not self.use_graph or (self.step == 0 and self.graph is None):
+                self._iteration()
+            else:
+                if self.graph is None:
+                    self.graph = t.cuda.CUDAGraph()
+                    with t.cuda.graph(self.graph, stream=self.stream):
+                        self._iteration()
+                self.graph.replay()
+        self.step += 1
+
+    def sync(self):
+        self.stream.synchronize()
+
+    def _iteration(self):
         s, t = self.s, self.t
         s.ctx.condense(s.x_cell, s.x_trace, self.thermodyn, self.vals, self.rhs, self.dirv)
         s._solve_trace(self.vals, self.rhs, self.sol)
@@ -387,12 +423,11 @@ class NewtonIteration:
         self.upd_trace.zero_()
         self.upd_trace[self.r2f] = self.sol
         s.ctx.reconstruct(s.x_cell, s.x_trace, self.upd_trace, self.thermodyn, self.upd_cell)
-        dn = t.stack([self.upd_cell.abs().max(), self.upd_trace.abs().max(),
-                      s.x_cell.abs().max(), s.x_trace.abs().max()])
-        self.norms.copy_(dn, non_blocking=True)
+        max_abs4([self.upd_cell, self.upd_trace, s.x_cell, s.x_trace], self.nscratch,
+                 self.dnorms)
+        self.norms.copy_(self.dnorms, non_blocking=True)
         s.x_cell.add_(self.upd_cell, alpha=self.alpha)
         s.x_trace.add_(self.upd_trace, alpha=self.alpha)
-        self.step += 1
 
     def finish(self):
         n_ = self.norms.numpy()

@@ -180,6 +180,9 @@ class FixedTauNPSolverParameters:
     # device (the default), "host" = SciPy's SuperLU on the host (the
     # reference's plain sparse direct form, UMFPACK, src/np_solver.cpp:1347-1351)
     trace_solver: str = "gpu"
+    # replay each Newton iteration from a captured hipGraph (one launch per
+    # iteration instead of hundreds); needs trace_solver "gpu"
+    graph_newton: bool = False
     # False: the reference's linearisation of the volume drift term
     # (include/assembling_methods.template.h:500, 521-522), whose sign is
     # opposite to the derivative of its own residual (:656, :673), so that

@@ -7,6 +7,10 @@ system; examples/residual.ipynb cells 4-17) and then one coupled Newton solve
 per bias point, the Newton state kept on the device from one point to the
 next; each converged point records the terminal current at both contacts.
 
+Every Newton iteration after a request's first is replayed from a captured
+hipGraph (one launch instead of hundreds); the graph is kept across the bias
+points, which change only boundary data in place.
+
 Scheduling: requests wait in arrival order and are admitted while a slot is
 free and their device-memory estimate (context tables, trace-solver fronts,
 Newton buffers) fits the HBM budget, so requests arriving while others run
@@ -79,8 +83,8 @@ def memory_estimate(req: IVRequest) -> int:
 class _Slot:
     """one admitted request: its solver, stream and stage machine"""
 
-    def __init__(self, req: IVRequest, t0: float, now: float, torch):
-        self.req, self.torch = req, torch
+    def __init__(self, req: IVRequest, t0: float, now: float, torch, graph=True):
+        self.req, self.torch, self.graph = req, torch, graph
         self.stream = torch.cuda.Stream()
         with torch.cuda.stream(self.stream):
             prob, params, adim = configs.diode(
@@ -111,8 +115,14 @@ class _Slot:
             # a request holds the trace solver of its current stage only
             for key in [k for k in s._solver_cache if k != s.enabled]:
                 s._solver_cache.pop(key).close()
-            self.it = NewtonIteration(s, self.stage == 0, nl.absolute_tolerance,
-                                      nl.relative_tolerance, nl.max_number_of_iterations)
+            if self.stage >= 3:
+                # bias points: same operator and buffers, boundary data
+                # changed in place; the captured iteration is replayed
+                self.it.reset()
+            else:
+                self.it = NewtonIteration(s, self.stage == 0, nl.absolute_tolerance,
+                                          nl.relative_tolerance, nl.max_number_of_iterations,
+                                          graph=self.graph)
 
     def launch(self):
         with self.torch.cuda.stream(self.stream):
@@ -140,11 +150,11 @@ class IVServer:
     """Admits I-V requests by HBM budget into at most max_slots concurrent
     slots and runs them with continuous batching (see the module doc)."""
 
-    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, keep_state=False):
+    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, keep_state=False, graph=True):
         import torch
         self.torch = torch
         self.budget, self.max_slots = int(hbm_budget_bytes), int(max_slots)
-        self.keep_state = keep_state
+        self.keep_state, self.graph = keep_state, graph
 
     def run(self, requests, clock=None):
         """serve requests (arrival times in seconds from the start); returns
@@ -163,7 +173,7 @@ class IVServer:
                     break
                 req = pending.pop(0)
                 used += need
-                active.append((_Slot(req, t0, now(), torch), need))
+                active.append((_Slot(req, t0, now(), torch, self.graph), need))
             if not active:
                 wait = pending[0].arrival - now()
                 if wait > 0:

@@ -193,6 +193,13 @@ int hdg_terminal_current(hdg_ctx *ctx, const double *x_cell,
                          const double *x_trace, int boundary_id, double *out,
                          void *stream);
 
+/* Newton norms: out[v] = max_k |vectors[v][k]| for four device vectors of
+ * the given lengths (out and scratch in device memory, scratch 1024
+ * doubles); two kernels, no memset, deterministic.  The L-infinity norms of
+ * the reference's convergence test (src/np_solver.cpp:1521-1560). */
+int hdg_max_abs4(const double *const *vectors, const long long *lengths,
+                 double *scratch, double *out, void *stream);
+
 /* ---- Batched pass 1 over several meshes --------------------------------
  * Pass 1 (hdg_condense) of several contexts (2D, degree 1 or 2 mixed, the
  * same enabled components) in one persistent launch per colour; each
