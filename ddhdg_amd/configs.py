@@ -15,7 +15,7 @@ import numpy as np
 
 from .fe import BoxMesh
 from .problem import (DIRICHLET, EPSILON0, Adimensionalizer, BoundaryConditionHandler,
-                      FixedTauNPSolverParameters, HomogeneousMobility, HomogeneousPermittivity,
+                      CellFaceTauNPSolverParameters, FixedTauNPSolverParameters, HomogeneousMobility, HomogeneousPermittivity,
                       HomogeneousProblem, NonlinearSolverParameters,
                       ShockleyReadHallFixedTemperature, compute_density,
                       local_charge_neutrality_potential, n, p, V)
@@ -48,14 +48,18 @@ def scaled_unit_tau(adim):
 
 
 def diode(dim, n_cells, degree, doping=None, bias=0.0, tau=None, nonlinear=None,
-          consistent_drift_jacobian=True):
+          consistent_drift_jacobian=True, dd_flux_type="use_cell", phi_linearize=False,
+          cell_face_tau=None):
     """(problem, parameters, adimensionalizer) of a box diode.
 
     The diode runs use the consistent linearisation of the volume drift term
     (the reference's own converges at best linearly and diverges on these
     1e23 m^-3 junctions); pass consistent_drift_jacobian=False for the
     reference's.  n_cells: cells per axis (one number for all axes, or one
-    per axis)."""
+    per axis).  dd_flux_type and phi_linearize as the reference's
+    NPSolverParameters; cell_face_tau: physical tau per (cell, face,
+    component), shape (ncells, 2*dim, 3), selects
+    CellFaceTauNPSolverParameters."""
     ncs = list(n_cells) if hasattr(n_cells, "__len__") else [n_cells] * dim
     mesh = BoxMesh(dim, ncs, [0.0] * dim, [1.0] * dim)
     doping = doping or step_doping()
@@ -74,11 +78,21 @@ def diode(dim, n_cells, degree, doping=None, bias=0.0, tau=None, nonlinear=None,
     )
     set_contacts(prob, bias)
     adim = Adimensionalizer(1e-6, 1.602176634e-19 / 1.380649e-23, 1e23, 1.0)
-    tau = tau or scaled_unit_tau(adim)
-    params = FixedTauNPSolverParameters(
-        degree, degree, degree,
-        nonlinear or NonlinearSolverParameters(1e-10, 1e-10, 100, 1.0),
-        tau[0], tau[1], tau[2])
+    nl = nonlinear or NonlinearSolverParameters(1e-10, 1e-10, 100, 1.0)
+    if cell_face_tau is not None:
+        tab = np.asarray(cell_face_tau, dtype=np.float64).reshape(mesh.ncells, 2 * dim, 3)
+        params = CellFaceTauNPSolverParameters(degree, degree, degree, nl,
+                                               dd_flux_type=dd_flux_type,
+                                               phi_linearize=phi_linearize)
+        m = mesh.ncells * 2 * dim
+        params.set_multiple_faces(dim, [0] * m, np.repeat(np.arange(mesh.ncells), 2 * dim),
+                                  np.tile(np.arange(2 * dim), mesh.ncells), tab[:, :, 0].ravel(),
+                                  tab[:, :, 1].ravel(), tab[:, :, 2].ravel())
+    else:
+        tau = tau or scaled_unit_tau(adim)
+        params = FixedTauNPSolverParameters(degree, degree, degree, nl, tau[0], tau[1], tau[2],
+                                            dd_flux_type=dd_flux_type,
+                                            phi_linearize=phi_linearize)
     params.consistent_drift_jacobian = consistent_drift_jacobian
     return prob, params, adim
 

@@ -76,9 +76,18 @@ namespace
     return 0;
   }
 
+  // the default operator: use_cell flux, fixed tau, no phi_linearize (the
+  // equilibrium operator ignores the flux and phi_linearize)
+  bool default_variant(const hdg_ctx *C, int thermodyn)
+  {
+    if (C->d_taucf)
+      return false;
+    return thermodyn || (C->phys.flux_type == 0 && !C->phys.phi_linearize);
+  }
+
   bool use_fast(const hdg_ctx *C, int thermodyn)
   {
-    if (C->variant == 1)
+    if (C->variant == 1 || !default_variant(C, thermodyn))
       return false;
     return hdg::fast_supported(C->d, C->k, C->ne, thermodyn, C->en);
   }
@@ -122,6 +131,9 @@ namespace
     p.sdrift = C->phys.drift_jacobian ? -1.0 : 1.0;
     p.mixed  = C->precision;
     p.deg    = C->k;
+    p.flux_type = C->phys.flux_type;
+    p.phi_lin   = C->phys.phi_linearize ? 1 : 0;
+    p.tau_cf    = C->d_taucf;
     p.tb     = C->tb;
     p.doping = C->d_doping;
     p.dirv   = C->d_dirv;
@@ -146,8 +158,9 @@ int hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys, hdg_ctx **ou
     return fail("hdg_ctx_create: null argument");
   if (!hdg::supported(mesh->dim, mesh->degree))
     return fail("hdg_ctx_create: (dim, degree) not supported: 2D k=1..3, 3D k=1", -2);
-  if (phys->flux_type != 0)
-    return fail("hdg_ctx_create: only the use_cell flux is supported", -2);
+  if (phys->flux_type < 0 || phys->flux_type > 2)
+    return fail("hdg_ctx_create: flux_type must be 0 (use_cell), 1 (use_trace) or 2 "
+                "(qiu_shi_stabilization)", -2);
   for (int a = 0; a < mesh->dim; ++a)
     if (mesh->n_cells[a] < 1 || !(mesh->upper[a] > mesh->lower[a]))
       return fail("hdg_ctx_create: invalid mesh extent", -2);
@@ -258,7 +271,7 @@ void hdg_ctx_destroy(hdg_ctx *C)
   if (!C)
     return;
   for (void *p : {(void *)C->d_tab, (void *)C->d_doping, (void *)C->d_dirv,
-                  (void *)C->d_neuv, (void *)C->d_blkptr, (void *)C->d_nbk,
+                  (void *)C->d_neuv, (void *)C->d_taucf, (void *)C->d_blkptr, (void *)C->d_nbk,
                   (void *)C->d_cslot, (void *)C->d_cells[0], (void *)C->d_cells[1]})
     if (p)
       hipFree(p);
@@ -304,6 +317,9 @@ int hdg_set_kernel_variant(hdg_ctx *C, int variant)
   if (variant == 2 && !hdg::fast_supported(C->d, C->k, C->ne, 0, C->en) &&
       !hdg::fast_supported(C->d, C->k, C->ne, 1, C->en))
     return fail("hdg_set_kernel_variant: no register-resident kernel for this shape", -2);
+  if (variant == 2 && !default_variant(C, 1))
+    return fail("hdg_set_kernel_variant: the register-resident kernel runs the default "
+                "operator only (use_cell flux, fixed tau, no phi_linearize)", -2);
   C->variant = variant;
   return 0;
 }
@@ -425,6 +441,35 @@ int hdg_set_doping(hdg_ctx *C, const double *doping_qp)
   hipError_t e = hipMemcpy(C->d_doping, doping_qp, C->ncells * C->qc * sizeof(double),
                            hipMemcpyHostToDevice);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_set_doping");
+}
+
+int hdg_set_cell_face_tau(hdg_ctx *C, const double *tau)
+{
+  if (!C)
+    return fail("hdg_set_cell_face_tau: null context");
+  if (!tau)
+    {
+      if (C->d_taucf)
+        hipFree(C->d_taucf);
+      C->d_taucf = nullptr;
+      return 0;
+    }
+  const long long n = C->ncells * C->nfc * 3;
+  for (long long i = 0; i < n; ++i)
+    if (!(tau[i] >= 0) || tau[i] > 1e300)
+      return fail("hdg_set_cell_face_tau: tau values must be finite and >= 0 (cell " +
+                      std::to_string(i / (C->nfc * 3)) + ", face " +
+                      std::to_string((i / 3) % C->nfc) + ")",
+                  -2);
+  // values change in place: launches recorded in a graph keep the address
+  if (!C->d_taucf)
+    {
+      hipError_t e = hipMalloc((void **)&C->d_taucf, n * sizeof(double));
+      if (e != hipSuccess)
+        return hip_fail(e, "hdg_set_cell_face_tau");
+    }
+  hipError_t e = hipMemcpy(C->d_taucf, tau, n * sizeof(double), hipMemcpyHostToDevice);
+  return e == hipSuccess ? 0 : hip_fail(e, "hdg_set_cell_face_tau");
 }
 
 int hdg_set_bc_values(hdg_ctx *C, const double *dir, const double *neu)
@@ -660,6 +705,10 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
     {
       if (!x_cell[i] || !x_trace[i] || !csr_vals[i] || !rhs[i] || !dir_vals[i])
         return fail("hdg_batch_condense: null device pointer");
+      if (!default_variant(B->ctx[i], thermodyn))
+        return fail("hdg_batch_condense: context " + std::to_string(i) +
+                        " uses a non-default flux, tau or phi_linearize; condense it alone",
+                    -2);
       hdg::KParams p = params(B->ctx[i], thermodyn);
       p.xc           = x_cell[i];
       p.xt           = x_trace[i];

@@ -28,6 +28,7 @@ struct hdg_ctx
   double         *d_tab = nullptr;
   hdg::DevTables  tb{};
   double         *d_doping = nullptr, *d_dirv = nullptr, *d_neuv = nullptr;
+  double         *d_taucf  = nullptr; // per (cell, face, comp) tau, or null
   long long      *d_blkptr = nullptr;
   int            *d_nbk    = nullptr;
   unsigned char  *d_cslot  = nullptr;

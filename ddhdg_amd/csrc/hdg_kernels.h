@@ -32,11 +32,14 @@ namespace hdg
     double sdrift; // +1 reference linearisation, -1 consistent
     int    mixed;  // 1: FP32 elimination + FP64 refinement in the register kernel's pass 1
     int    deg;    // polynomial degree (batched launches mix degrees)
+    int    flux_type; // 0 use_cell, 1 use_trace, 2 qiu_shi_stabilization
+    int    phi_lin;   // phi_linearize: Boltzmann V rows at frozen quasi-Fermi potentials
     DevTables tb;
     // per-mesh data
     const double *doping;   // ncells*QC
     const double *dirv;     // nfaces*3*QF
     const double *neuv;     // nfaces*3*QF
+    const double *tau_cf;   // ncells*2d*3 per-(cell, face, comp) tau, or null: tau[]
     // scatter map
     const long long     *blkptr;   // nfaces
     const int           *nbk;      // nfaces

@@ -21,7 +21,12 @@
 //        :1737-2118)
 //   trace rows:       xi_i * F_c       assemble_tc_matrix, assemble_tt_matrix,
 //        add_tc/tt_matrix_terms_to_tt_rhs (:1145-1588), apply_nbc_on_face
-// where F_c is the HDG numerical flux of component c (use_cell).
+// where F_c is the HDG numerical flux of component c: its drift part uses
+// the cell value (use_cell), the trace value (use_trace) or, for
+// qiu_shi_stabilization, the cell value where E.nu > 0 and the trace value
+// elsewhere (template.h:983-1009, 1249-1263, 1417-1435, 1523-1536,
+// 1865-1903, 2036-2071).  tau is per face: FixedTauComputer or
+// CellFaceTauComputer (include/np_solver_parameters.h:92-115, 157-236).
 #include "hdg_device.h"
 #include "hdg_kernels.h"
 
@@ -66,7 +71,8 @@ namespace hdg
       static constexpr int oRS    = oLUPD + NFC * 3 * NF;
       static constexpr int oMUL   = oRS + MAXR;
       static constexpr int oSOL   = oMUL + MAXR;
-      static constexpr int NDBL   = oSOL + MAXNS;
+      static constexpr int oTAU   = oSOL + MAXNS;   // tau per (face, comp)
+      static constexpr int NDBL   = oTAU + NFC * 3;
       // ints after the doubles
       static constexpr int iUSED  = 0;
       static constexpr int iPIV   = iUSED + MAXR;
@@ -96,6 +102,7 @@ namespace hdg
       double *W0f = sm + S::oW0F, *tr0 = sm + S::oTR0, *qd = sm + S::oQD;
       double *rhsq = sm + S::oRHSQ, *Wrc = sm + S::oWRC, *Wrf = sm + S::oWRF;
       double *lamf = sm + S::oLAMF, *del = sm + S::oDEL, *lupd = sm + S::oLUPD;
+      double *tauv = sm + S::oTAU;
       int    *isdir = si + S::iDIR, *isneu = si + S::iNEU, *bidv = si + S::iBID;
       long long *facev = reinterpret_cast<long long *>(si + S::iFACE);
       // point data: r, drn, drp, phi'_V, phi_rhs, doping
@@ -136,6 +143,8 @@ namespace hdg
         xs[i] = p.xc[cell * S::NCD + i];
       for (int i = tid; i < QC; i += NT)
         qdop[i] = p.doping[cell * QC + i];
+      if (tid < NFC * 3)
+        tauv[tid] = p.tau_cf ? p.tau_cf[cell * NFC * 3 + tid] : p.tau[tid % 3];
       __syncthreads();
       for (int i = tid; i < NFC * 3 * NF; i += NT)
         {
@@ -201,12 +210,24 @@ namespace hdg
           qr[q]   = r;
           qdrn[q] = a;
           qdrp[q] = b;
-          if (p.thermo)
+          if (p.thermo || p.phi_lin)
             {
-              // template.h:442-459 and 677-693 with phi_n = phi_p = 0
+              // template.h:442-459 and 677-693: phi_n = phi_p = 0 in
+              // thermodynamic equilibrium; with phi_linearize the
+              // quasi-Fermi potentials of the current n and p
+              // (template.h:229-264, include/solver.h:343-375), zero for a
+              // disabled carrier (the scratch's zero-initialised vectors)
               const double V0 = u0[q], kT = p.kTq;
-              const double e1 = exp((p.ev_r - V0) / kT);
-              const double e2 = exp((V0 - p.ec_r) / kT);
+              double       pn = 0, pq = 0;
+              if (!p.thermo)
+                {
+                  if (p.en[1])
+                    pn = V0 - p.ec_r - kT * log(u0[QC + q] * p.n_rf / p.nc);
+                  if (p.en[2])
+                    pq = V0 - p.ev_r + kT * log(u0[2 * QC + q] * p.n_rf / p.nv);
+                }
+              const double e1 = exp((p.ev_r - V0 + pq) / kT);
+              const double e2 = exp((V0 - pn - p.ec_r) / kT);
               qphd[q] = -1 / p.n_rf * (p.nv / kT * e1 + p.nc / kT * e2);
               qphr[q] = 1 / p.n_rf * (p.nv * e1 - p.nc * e2);
             }
@@ -304,7 +325,8 @@ namespace hdg
 
       // ---- E: reduced system, one row block (component) at a time
       const double eps = p.eps, mun = p.mun, mup = p.mup, UT = p.UT;
-      const double tV = p.tau[0], tN = p.tau[1], tP = p.tau[2];
+      const bool   phim = p.thermo || p.phi_lin;
+      const int    ft   = p.flux_type;
       for (int ci = 0; ci < ne; ++ci)
         {
           const int c = p.encomp[ci];
@@ -339,14 +361,14 @@ namespace hdg
                   if (cs == 0)
                     {
                       if (c == 0)
-                        val = p.thermo ? -qphd[q] * uu[0] : uu[1] - uu[2];
+                        val = phim ? -qphd[q] * uu[0] : uu[1] - uu[2];
                       else
                         val = -(qdrn[q] * uu[1] + qdrp[q] * uu[2]);
                       if (kind == 2)
                         {
                           if (c == 0)
                             val += qdop[q] +
-                                   (p.thermo ? qphr[q] : u0[2 * QC + q] - u0[QC + q]);
+                                   (phim ? qphr[q] : u0[2 * QC + q] - u0[QC + q]);
                           else
                             val += qr[q];
                         }
@@ -409,29 +431,37 @@ namespace hdg
                       }
                   const int    fi  = g * QF + s;
                   const double E0n = W0f[(0 * NFC) * QF + fi];
+                  const double tc  = tauv[g * 3 + c];
+                  // drift of the numerical flux on the trace value
+                  const bool up = ft == 1 || (ft == 2 && E0n <= 0);
                   if (c == 0)
                     {
-                      val = eps * WN[0] + tV * uu[0] - tV * LL[0];
+                      val = eps * WN[0] + tc * uu[0] - tc * LL[0];
                       if (kind == 2)
-                        val += -eps * E0n - tV * u0f[fi] + tV * tr0[fi];
+                        val += -eps * E0n - tc * u0f[fi] + tc * tr0[fi];
                     }
                   else if (c == 1)
                     {
-                      const double n0 = u0f[NFC * QF + fi];
-                      val = uu[1] * mun * E0n + n0 * mun * WN[0] - UT * mun * WN[1] -
-                            tN * uu[1] + tN * LL[1];
+                      const double n0 = u0f[NFC * QF + fi], l0 = tr0[NFC * QF + fi];
+                      // the rhs column carries no upwind trace term: zero
+                      // trace update in pass 1, and the reconstruct pass
+                      // omits it (template.h:2122-2160)
+                      const double dr = up ? (kind == 2 ? 0.0 : LL[1] * mun * E0n) + l0 * mun * WN[0]
+                                           : uu[1] * mun * E0n + n0 * mun * WN[0];
+                      val = dr - UT * mun * WN[1] - tc * uu[1] + tc * LL[1];
                       if (kind == 2)
-                        val += -(n0 * mun * E0n - UT * mun * W0f[NFC * QF + fi]) +
-                               tN * n0 - tN * tr0[NFC * QF + fi];
+                        val += -((up ? l0 : n0) * mun * E0n - UT * mun * W0f[NFC * QF + fi]) +
+                               tc * n0 - tc * l0;
                     }
                   else
                     {
-                      const double p0 = u0f[2 * NFC * QF + fi];
-                      val = -(uu[2] * mup * E0n + p0 * mup * WN[0]) - UT * mup * WN[2] -
-                            tP * uu[2] + tP * LL[2];
+                      const double p0 = u0f[2 * NFC * QF + fi], l0 = tr0[2 * NFC * QF + fi];
+                      const double dr = up ? (kind == 2 ? 0.0 : LL[2] * mup * E0n) + l0 * mup * WN[0]
+                                           : uu[2] * mup * E0n + p0 * mup * WN[0];
+                      val = -dr - UT * mup * WN[2] - tc * uu[2] + tc * LL[2];
                       if (kind == 2)
-                        val += -(-p0 * mup * E0n - UT * mup * W0f[2 * NFC * QF + fi]) +
-                               tP * p0 - tP * tr0[2 * NFC * QF + fi];
+                        val += -(-(up ? l0 : p0) * mup * E0n - UT * mup * W0f[2 * NFC * QF + fi]) +
+                               tc * p0 - tc * l0;
                     }
                 }
               G[slot * LD + m] = val;
@@ -739,9 +769,11 @@ namespace hdg
     // Terminal current: the HDG numerical fluxes of n and p integrated over
     // the faces of one boundary id (outward normal).  The fluxes are the ones
     // the trace equations make single-valued (add_tc_matrix_terms_to_tt_rhs,
-    // include/assembling_methods.template.h:1311-1447, use_cell):
-    //   F_n =  n mu_n E.nu - U_T mu_n W_n.nu - tau_n (n - lambda_n)
-    //   F_p = -p mu_p E.nu - U_T mu_p W_p.nu - tau_p (p - lambda_p)
+    // include/assembling_methods.template.h:1311-1447):
+    //   F_n =  n^ mu_n E.nu - U_T mu_n W_n.nu - tau_n (n - lambda_n)
+    //   F_p = -p^ mu_p E.nu - U_T mu_p W_p.nu - tau_p (p - lambda_p)
+    // n^ = n (use_cell), lambda_n (use_trace), or n where E.nu > 0 and
+    // lambda_n elsewhere (qiu_shi_stabilization, template.h:1417, 1435).
     // One workgroup; fixed-order reduction (deterministic).
     constexpr int CUR_T = 256;
 
@@ -783,8 +815,11 @@ namespace hdg
                     lam[c] += xt[c * nf + m] * tb.xi[s * nf + m];
                 }
               const double JW = tb.JxWf[bid * qf + s];
-              sn += JW * (u[1] * p.mun * w[0] - p.UT * p.mun * w[1] - p.tau[1] * (u[1] - lam[1]));
-              sp += JW * (-u[2] * p.mup * w[0] - p.UT * p.mup * w[2] - p.tau[2] * (u[2] - lam[2]));
+              const double *tf = p.tau_cf ? p.tau_cf + (cell * 2 * dim + bid) * 3 : p.tau;
+              const bool    cl = p.flux_type == 0 || (p.flux_type == 2 && w[0] > 0);
+              const double  nn = cl ? u[1] : lam[1], pp = cl ? u[2] : lam[2];
+              sn += JW * (nn * p.mun * w[0] - p.UT * p.mun * w[1] - tf[1] * (u[1] - lam[1]));
+              sp += JW * (-pp * p.mup * w[0] - p.UT * p.mup * w[2] - tf[2] * (u[2] - lam[2]));
             }
         }
       red[0][threadIdx.x] = sn;

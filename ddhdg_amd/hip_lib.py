@@ -29,7 +29,7 @@ EXPORTS = [
     "hdg_trace_factor", "hdg_trace_factor_status", "hdg_trace_solve",
     "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
     "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_condense", "hdg_batch_info",
-    "hdg_max_abs4",
+    "hdg_max_abs4", "hdg_set_cell_face_tau",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -51,7 +51,7 @@ class Phys(ct.Structure):
                 ("rec", ct.c_double * 3), ("n_rescale", ct.c_double),
                 ("rec_rescale", ct.c_double), ("kT_over_q", ct.c_double),
                 ("ec_r", ct.c_double), ("ev_r", ct.c_double), ("nc", ct.c_double),
-                ("nv", ct.c_double), ("flux_type", ct.c_int),
+                ("nv", ct.c_double), ("flux_type", ct.c_int), ("phi_linearize", ct.c_int),
                 ("bc_type", (ct.c_int * 3) * 6), ("drift_jacobian", ct.c_int)]
 
 
@@ -93,6 +93,7 @@ def lib():
             "hdg_batch_condense": (i, [v, i, P(v), P(v), P(v), P(v), P(v), v]),
             "hdg_batch_info": (i, [v, dp]),
             "hdg_max_abs4": (i, [P(v), lp, v, v, v]),
+            "hdg_set_cell_face_tau": (i, [v, dp]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -190,6 +191,16 @@ class HdgContext:
         a = np.ascontiguousarray(rescaled_doping_qp, dtype=np.float64)
         assert a.size == self.ncells * self.Qc
         _check(self.L.hdg_set_doping(self.ctx, _dp(a)))
+
+    def set_cell_face_tau(self, rescaled_tau):
+        """per-(cell, face, component) tau, rescaled, shape (ncells, 2*dim,
+        3); None returns to the fixed tau"""
+        if rescaled_tau is None:
+            _check(self.L.hdg_set_cell_face_tau(self.ctx, None))
+            return
+        a = np.ascontiguousarray(rescaled_tau, dtype=np.float64)
+        assert a.size == self.ncells * 2 * self.dim * 3
+        _check(self.L.hdg_set_cell_face_tau(self.ctx, _dp(a)))
 
     def set_bc_values(self, dirichlet=None, neumann=None):
         n = self.nfaces * 3 * self.Qf

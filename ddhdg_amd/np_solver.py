@@ -24,7 +24,8 @@ import numpy as np
 from . import fe
 from .hip_lib import HdgContext, MeshDesc, Phys, TraceSolver, max_abs4
 from .problem import (DIRICHLET, EV, KB, NEUMANN, Q, Adimensionalizer,
-                      FixedTauNPSolverParameters, LinearRecombinationTerm,
+                      CellFaceTauNPSolverParameters, FixedTauNPSolverParameters,
+                      LinearRecombinationTerm, flux_type_code,
                       ShockleyReadHallFixedTemperature, compute_density, evaluate,
                       local_charge_neutrality_potential, n, p, V)
 
@@ -47,7 +48,10 @@ def fold_physics(problem, parameters, adim) -> Phys:
     ph.mu_p = problem.p_mobility.mu / adim.get_mobility_rescaling_factor()
     T = float(problem.temperature)
     ph.U_T = (KB * T / Q) / adim.get_thermal_voltage_rescaling_factor()
-    taus = (parameters.V_tau, parameters.n_tau, parameters.p_tau)
+    if isinstance(parameters, CellFaceTauNPSolverParameters):
+        taus = (0.0, 0.0, 0.0)    # per (cell, face): hdg_set_cell_face_tau
+    else:
+        taus = (parameters.V_tau, parameters.n_tau, parameters.p_tau)
     for c in range(3):
         ph.tau[c] = taus[c] / adim.get_tau_rescaling_factor(c)
     r = problem.recombination_term
@@ -72,11 +76,8 @@ def fold_physics(problem, parameters, adim) -> Phys:
     ph.ev_r = problem.valence_band_edge_energy * EV / (Q * V_rf)
     ph.nc = problem.conduction_band_density
     ph.nv = problem.valence_band_density
-    if parameters.dd_flux_type != "use_cell":
-        raise NotImplementedError("only the use_cell drift-diffusion flux is implemented")
-    if parameters.phi_linearize:
-        raise NotImplementedError("phi_linearize is not implemented")
-    ph.flux_type = 0
+    ph.flux_type = flux_type_code(parameters.dd_flux_type)
+    ph.phi_linearize = 1 if parameters.phi_linearize else 0
     ph.drift_jacobian = 1 if parameters.consistent_drift_jacobian else 0
     tab = problem.boundary_handler.bc_type_table()
     for b in range(6):
@@ -117,6 +118,8 @@ class NPSolver:
                                    device=self.device)
         self._pattern_cache = {}
         self._solver_cache = {}
+        self._tau_version = None
+        self._sync_tau()
 
     # ------------------------------------------------------------ set-up
     def _upload_problem_data(self):
@@ -143,6 +146,18 @@ class NPSolver:
                 else:
                     neuv[faces, c, :] = vals / self.adim.get_neumann_boundary_condition_rescaling_factor(c)
         ctx.set_bc_values(dirv.ravel(), neuv.ravel())
+
+    def _sync_tau(self):
+        """CellFaceTauComputer reads the parameters' tau maps at every
+        assembly (include/np_solver_parameters.h:157-236): upload them when
+        they changed since the last upload"""
+        prm = self.parameters
+        if not isinstance(prm, CellFaceTauNPSolverParameters) or prm.version == self._tau_version:
+            return
+        tab = prm.tau_table(self.ctx.ncells, 2 * self.mesh.dim)
+        rf = np.array([self.adim.get_tau_rescaling_factor(c) for c in range(3)])
+        self.ctx.set_cell_face_tau(tab / rf)
+        self._tau_version = prm.version
 
     def replace_boundary_condition(self, boundary_id, bc_type, component, f):
         """NPSolver::replace_boundary_condition (src/np_solver.cpp:2160-2193)."""
@@ -259,6 +274,7 @@ class NPSolver:
         pyddhdg's assemble_system/get_residual/get_jacobian expose them."""
         import scipy.sparse as sp
         t = self.torch
+        self._sync_tau()
         rp, ci, dirs, _ = self._pattern()
         vals = t.empty(self.ctx.nnz, dtype=t.float64, device=self.device)
         rhs = t.empty(self.ctx.ntrace, dtype=t.float64, device=self.device)
@@ -377,6 +393,7 @@ class NewtonIteration:
         self.max_it = max_it if max_it >= 0 else 1 << 30
         self.log = log
         ctx, dev = solver.ctx, solver.device
+        solver._sync_tau()
         _, _, dirs, self.r2f = solver._pattern()
         self.vals = t.empty(ctx.nnz, dtype=t.float64, device=dev)
         self.rhs = t.empty(ctx.ntrace, dtype=t.float64, device=dev)
@@ -394,7 +411,8 @@ class NewtonIteration:
     def reset(self):
         """a new Newton solve with the same buffers (and captured graph): the
         boundary data may have changed in place, the components and the
-        operator not"""
+        operator not (per-face tau values are updated in place)"""
+        self.s._sync_tau()
         self.step, self.converged, self.upd_cell_norm = 0, False, 0.0
 
     def launch(self):

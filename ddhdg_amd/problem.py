@@ -190,6 +190,94 @@ class FixedTauNPSolverParameters:
     consistent_drift_jacobian: bool = False
 
 
+# DDFluxType (include/np_solver_parameters.h:34-39)
+DD_FLUX_TYPES = {"use_cell": 0, "use_trace": 1, "qiu_shi_stabilization": 2}
+
+
+def flux_type_code(name):
+    if name not in DD_FLUX_TYPES:
+        raise ValueError(f"dd_flux_type must be one of {sorted(DD_FLUX_TYPES)}, not {name!r}")
+    return DD_FLUX_TYPES[name]
+
+
+@dataclass
+class CellFaceTauNPSolverParameters:
+    """CellFaceTauNPSolverParameters (include/np_solver_parameters.h:335-404,
+    src/np_solver_parameters.cpp:239-356; Python binding
+    src/pyddhdg/pyddhdg_common.cpp:149-250): tau given per (cell, face) and
+    component by set_face / set_multiple_faces, rescaled by the same factors
+    as the fixed tau (CellFaceTauComputer, include/np_solver_parameters.h:
+    139-259).  A cell is its index on the structured mesh (level 0, x
+    fastest, the numbering of GridGenerator::subdivided_hyper_rectangle), or
+    a (level, index) pair, or an object with level() and index().  As in the
+    reference, a cell that was never set is an error when the solver
+    assembles, and unset faces of a set cell hold tau = 0."""
+    V_degree: int = 1
+    n_degree: int = 1
+    p_degree: int = 1
+    nonlinear_parameters: NonlinearSolverParameters = field(default_factory=NonlinearSolverParameters)
+    iterative_linear_solver: bool = False
+    multithreading: bool = True
+    dd_flux_type: str = "use_cell"
+    phi_linearize: bool = False
+    trace_solver: str = "gpu"
+    graph_newton: bool = False
+    consistent_drift_jacobian: bool = False
+    version: int = field(default=0, init=False)
+
+    def __post_init__(self):
+        self._tau = {}
+
+    @staticmethod
+    def _cell_id(cell):
+        if hasattr(cell, "level") and hasattr(cell, "index"):
+            lv, ix = (cell.level() if callable(cell.level) else cell.level,
+                      cell.index() if callable(cell.index) else cell.index)
+        elif isinstance(cell, tuple):
+            lv, ix = cell
+        else:
+            lv, ix = 0, cell
+        if int(lv) != 0:
+            raise ValueError("the structured meshes have level-0 cells only")
+        return int(ix)
+
+    def set_face(self, cell, face, V_tau_value, n_tau_value, p_tau_value, faces_per_cell=None):
+        cid = self._cell_id(cell)
+        fpc = faces_per_cell or 6
+        if not 0 <= int(face) < fpc:
+            raise ValueError("face index out of range")
+        row = self._tau.setdefault(cid, {})
+        row[int(face)] = (float(V_tau_value), float(n_tau_value), float(p_tau_value))
+        self.version += 1
+
+    def set_multiple_faces(self, dim, cell_levels, cell_indices, faces, V_tau_value,
+                           n_tau_value, p_tau_value):
+        m = len(cell_levels)
+        for lst in (cell_indices, faces, V_tau_value, n_tau_value, p_tau_value):
+            if len(lst) != m:
+                raise ValueError("The list passed to this function must have the same size")
+        for i in range(m):
+            self.set_face((cell_levels[i], cell_indices[i]), faces[i], V_tau_value[i],
+                          n_tau_value[i], p_tau_value[i], 2 * dim)
+
+    def clear(self):
+        self._tau.clear()
+        self.version += 1
+
+    def tau_table(self, ncells, faces_per_cell):
+        """physical tau, shape (ncells, faces_per_cell, 3)"""
+        out = np.zeros((ncells, faces_per_cell, 3))
+        for c in range(ncells):
+            row = self._tau.get(c)
+            if row is None:
+                raise KeyError(f"CellFaceTauNPSolverParameters: no tau set for cell {c}")
+            for f, v in row.items():
+                if f >= faces_per_cell:
+                    raise ValueError(f"cell {c}: face {f} out of range")
+                out[c, f] = v
+        return out
+
+
 def compute_density(problem, c, qf_potential, potential, temperature):
     """Solver::compute_density (include/solver.h:378-412), physical units."""
     kbt = KB * temperature

@@ -69,7 +69,14 @@ typedef struct
   double kT_over_q;      /* k_B T / (q V_rescale) */
   double ec_r, ev_r;     /* band edges / (q V_rescale) */
   double nc, nv;         /* band densities, physical */
-  int    flux_type;      /* 0 use_cell (only one supported) */
+  int    flux_type;      /* DDFluxType (include/np_solver_parameters.h:
+                            34-39): 0 use_cell, 1 use_trace,
+                            2 qiu_shi_stabilization */
+  int    phi_linearize;  /* NPSolverParameters::phi_linearize: the V rows
+                            use Boltzmann densities at the quasi-Fermi
+                            potentials of the current n and p
+                            (include/assembling_methods.template.h:229-264,
+                            442-459, 677-693) */
   int    bc_type[6][3];  /* per boundary id and component:
                             0 none, 1 Dirichlet, 2 Neumann */
   int    drift_jacobian; /* 0: the reference's linearisation of the volume
@@ -140,6 +147,15 @@ int hdg_face_qpoints(const hdg_ctx *ctx, double *xyz);
  * i.e. the reference's rescaled_doping evaluated at the quadrature points
  * (src/np_solver.cpp:341-343). */
 int hdg_set_doping(hdg_ctx *ctx, const double *doping_qp);
+/* CellFaceTauComputer (reference include/np_solver_parameters.h:139-259,
+ * CellFaceTauNPSolverParameters::set_face, src/np_solver_parameters.cpp:
+ * 328-356): tau per cell, face and component, rescaled like hdg_phys.tau,
+ * host array tau[(cell*2*dim + face)*3 + comp].  NULL returns to the fixed
+ * tau of hdg_phys.  Values must be finite and >= 0.  Later calls overwrite
+ * the values in place (synchronous copy), so launches recorded in a HIP
+ * graph see them; a NULL call frees the device copy and must not be made
+ * while such a graph may still run. */
+int hdg_set_cell_face_tau(hdg_ctx *ctx, const double *tau);
 /* Boundary data at face quadrature points (host arrays, nfaces*3*QF):
  * Dirichlet values divided by the component rescaling factor and Neumann
  * values divided by the Neumann rescaling factor; entries of interior

@@ -273,6 +273,9 @@ struct orc_ctx
   double V_rf, n_rf, perm_rf, UT_rf, mu_rf, rec_rf;
   double tau_rf[3], neu_rf[3];
   double eps_r, mun_r, mup_r, tau_r[3];
+  // CellFaceTauComputer values, rescaled, ncells*2d*3 (cell, face, comp);
+  // empty: FixedTauComputer (tau_r)
+  std::vector<double> tau_cf;
 
   // reference shape tables on [0,1]^d
   std::vector<double>         qx1, qw1;  // 1D gauss
@@ -457,6 +460,7 @@ namespace
     std::vector<Vec3>   fE0, fWn0, fWp0;
     std::vector<double> upd_tr[3];
     Vec3                normal;
+    long long           cur = 0; // cell being assembled
     // dirichlet condenser
     int                 n_constr;
     std::vector<int>    constr_dofs;
@@ -495,6 +499,16 @@ namespace
         u.resize(Qf);
       constr_dofs.resize(T);
       constr_vals.resize(T);
+    }
+
+    // tau of component comp on local face lf of the current cell:
+    // FixedTauComputer::compute_tau or CellFaceTauComputer::compute_tau
+    // (include/np_solver_parameters.h:92-115, 157-236), rescaled
+    inline double tau(int lf, int comp) const
+    {
+      if (C->tau_cf.empty())
+        return C->tau_r[comp];
+      return C->tau_cf[(cur * C->nfaces_cell + lf) * 3 + comp];
     }
 
     // dof decode: component, flux axis (-1 scalar), basis index
@@ -652,12 +666,16 @@ namespace
             r[q] = drn[q] = drp[q] = 0;
           if (C->P.phi_linearize && !thermo)
             {
-              // compute_quasi_fermi_potential (include/solver.h:343-375)
-              const double UTp = C->P.temperature * kKB / kQ;
-              phin[q] = V0[q] * C->V_rf - C->P.ec * kEV / kQ -
-                        UTp * std::log(n0[q] * C->n_rf / C->P.nc);
-              phip[q] = V0[q] * C->V_rf - C->P.ev * kEV / kQ +
-                        UTp * std::log(p0[q] * C->n_rf / C->P.nv);
+              // compute_quasi_fermi_potential (include/solver.h:343-375) of
+              // the enabled carriers (template.h:229-264); a disabled
+              // carrier keeps the scratch's zero (src/np_solver.cpp:138)
+              const double UTp = C->P.temperature * (kKB / kQ);
+              phin[q] = C->en[CN] ? V0[q] * C->V_rf - C->P.ec * kEV / kQ -
+                                      UTp * std::log(n0[q] * C->n_rf / C->P.nc)
+                                  : 0.0;
+              phip[q] = C->en[CP] ? V0[q] * C->V_rf - C->P.ev * kEV / kQ +
+                                      UTp * std::log(p0[q] * C->n_rf / C->P.nv)
+                                  : 0.0;
             }
         }
     }
@@ -865,15 +883,15 @@ namespace
               const int ii = sup[i];
               if (C->en[CV])
                 c[ii] += (-trV0[s] * dot(q1[i], normal, d) +
-                          C->tau_r[CV] * trV0[s] * z1[i]) *
+                          tau(lf, CV) * trV0[s] * z1[i]) *
                          JW;
               if (C->en[CN])
                 c[ii] += (-trn0[s] * dot(q2[i], normal, d) -
-                          C->tau_r[CN] * trn0[s] * z2[i]) *
+                          tau(lf, CN) * trn0[s] * z2[i]) *
                          JW;
               if (C->en[CP])
                 c[ii] += (-trp0[s] * dot(q3[i], normal, d) -
-                          C->tau_r[CP] * trp0[s] * z3[i]) *
+                          tau(lf, CP) * trp0[s] * z3[i]) *
                          JW;
             }
         }
@@ -903,14 +921,14 @@ namespace
                   double      &e   = CT[ii * T + jj];
                   if (C->en[CV])
                     e += (trV * dot(q1[i], normal, d) -
-                          C->tau_r[CV] * trV * z1[i]) *
+                          tau(lf, CV) * trV * z1[i]) *
                          JW;
                   if (C->en[CN])
                     {
                       if (ft == 1 || (ft == 2 && E0n <= 0))
                         e += munE * trN * z2[i] * JW;
                       e += (trN * dot(q2[i], normal, d) +
-                            C->tau_r[CN] * (trN * z2[i])) *
+                            tau(lf, CN) * (trN * z2[i])) *
                            JW;
                     }
                   if (C->en[CP])
@@ -918,7 +936,7 @@ namespace
                       if (ft == 1 || (ft == 2 && E0n <= 0))
                         e += -mupE * trP * z3[i] * JW;
                       e += (trP * dot(q3[i], normal, d) +
-                            C->tau_r[CP] * (trP * z3[i])) *
+                            tau(lf, CP) * (trP * z3[i])) *
                            JW;
                     }
                 }
@@ -931,7 +949,7 @@ namespace
     {
       const auto &sup = C->cell_support_on_face[lf];
       const int   tdf = C->ne * C->nf;
-      const double tau = C->tau_r[comp];
+      const double tf = tau(lf, comp);
       for (int s = 0; s < C->Qf; ++s)
         {
           const double JW = fJxW[s];
@@ -949,7 +967,7 @@ namespace
                     {
                       const double epsEn =
                         C->eps_r * dot(q1[j], normal, d);
-                      e -= (epsEn + tau * z1[j]) * xiv * JW;
+                      e -= (epsEn + tf * z1[j]) * xiv * JW;
                     }
                   else if (comp == CN)
                     {
@@ -961,7 +979,7 @@ namespace
                         cE = z2[j] * C->mun_r * E0n + fn0[s] * mE;
                       else
                         cE = trn0[s] * mE;
-                      e -= (cE - Df - tau * z2[j]) * xiv * JW;
+                      e -= (cE - Df - tf * z2[j]) * xiv * JW;
                     }
                   else
                     {
@@ -973,7 +991,7 @@ namespace
                         cE = -z3[j] * C->mup_r * E0n - fp0[s] * mE;
                       else
                         cE = -trp0[s] * mE;
-                      e -= (cE - Df - tau * z3[j]) * xiv * JW;
+                      e -= (cE - Df - tf * z3[j]) * xiv * JW;
                     }
                 }
             }
@@ -984,7 +1002,7 @@ namespace
     void tc_rhs(int lf, int comp, int ft)
     {
       const int    tdf = C->ne * C->nf;
-      const double tau = C->tau_r[comp];
+      const double tf = tau(lf, comp);
       for (int s = 0; s < C->Qf; ++s)
         {
           const double JW  = fJxW[s];
@@ -994,14 +1012,14 @@ namespace
               const int    ii  = lf * tdf + i;
               const double xiv = tr_value(ii, lf, s, comp);
               if (comp == CV)
-                t[ii] += (-C->eps_r * E0n - tau * fV0[s]) * xiv * JW;
+                t[ii] += (-C->eps_r * E0n - tf * fV0[s]) * xiv * JW;
               else if (comp == CN)
                 {
                   const double nn =
                     (ft == 0 || (ft == 2 && E0n > 0)) ? fn0[s] : trn0[s];
                   const double Dn =
                     C->mun_r * UTf[s] * dot(fWn0[s], normal, d);
-                  t[ii] += (-nn * C->mun_r * E0n + Dn + tau * fn0[s]) *
+                  t[ii] += (-nn * C->mun_r * E0n + Dn + tf * fn0[s]) *
                            xiv * JW;
                 }
               else
@@ -1010,7 +1028,7 @@ namespace
                     (ft == 0 || (ft == 2 && E0n > 0)) ? fp0[s] : trp0[s];
                   const double Dp =
                     C->mup_r * UTf[s] * dot(fWp0[s], normal, d);
-                  t[ii] += (pp * C->mup_r * E0n + Dp + tau * fp0[s]) *
+                  t[ii] += (pp * C->mup_r * E0n + Dp + tf * fp0[s]) *
                            xiv * JW;
                 }
             }
@@ -1021,7 +1039,7 @@ namespace
     void tt_matrix(int lf, int comp, int ft)
     {
       const int    tdf  = C->ne * C->nf;
-      const double tau  = C->tau_r[comp];
+      const double tf   = tau(lf, comp);
       const double sign = comp == CV ? -1 : 1;
       for (int s = 0; s < C->Qf; ++s)
         {
@@ -1040,7 +1058,7 @@ namespace
                     e += C->mun_r * E0n * trj * xii * JW;
                   if (comp == CP && (ft == 1 || (ft == 2 && E0n <= 0)))
                     e += -C->mup_r * E0n * trj * xii * JW;
-                  e += sign * tau * trj * xii * JW;
+                  e += sign * tf * trj * xii * JW;
                 }
             }
         }
@@ -1050,7 +1068,7 @@ namespace
     void tt_rhs(int lf, int comp)
     {
       const int    tdf  = C->ne * C->nf;
-      const double tau  = C->tau_r[comp];
+      const double tf   = tau(lf, comp);
       const double sign = comp == CV ? -1 : 1;
       const std::vector<double> &tr0 =
         comp == CV ? trV0 : (comp == CN ? trn0 : trp0);
@@ -1058,7 +1076,7 @@ namespace
         for (int i = 0; i < tdf; ++i)
           {
             const int ii = lf * tdf + i;
-            t[ii] += -sign * tau * tr0[s] * tr_value(ii, lf, s, comp) *
+            t[ii] += -sign * tf * tr0[s] * tr_value(ii, lf, s, comp) *
                      fJxW[s];
           }
     }
@@ -1131,7 +1149,7 @@ namespace
                   double   &a  = A[ii * N + jj];
                   if (C->en[CV])
                     a += (C->eps_r * dot(q1[j], normal, d) * z1[i] +
-                          C->tau_r[CV] * z1[j] * z1[i]) *
+                          tau(lf, CV) * z1[j] * z1[i]) *
                          JW;
                   if (C->en[CN])
                     {
@@ -1144,7 +1162,7 @@ namespace
                       else
                         cE = mEn * trn0[s];
                       a += (cE * z2[i] - Dn * z2[i] -
-                            C->tau_r[CN] * z2[j] * z2[i]) *
+                            tau(lf, CN) * z2[j] * z2[i]) *
                            JW;
                     }
                   if (C->en[CP])
@@ -1158,7 +1176,7 @@ namespace
                       else
                         cE = -mEp * trp0[s];
                       a += (cE * z3[i] - Dp * z3[i] -
-                            C->tau_r[CP] * z3[j] * z3[i]) *
+                            tau(lf, CP) * z3[j] * z3[i]) *
                            JW;
                     }
                 }
@@ -1196,12 +1214,12 @@ namespace
               const int ii = sup[i];
               if (C->en[CV])
                 c[ii] += (-C->eps_r * E0n * z1[i] -
-                          C->tau_r[CV] * fV0[s] * z1[i]) *
+                          tau(lf, CV) * fV0[s] * z1[i]) *
                          JW;
               if (C->en[CN])
-                c[ii] += (-Jn * z2[i] + C->tau_r[CN] * fn0[s] * z2[i]) * JW;
+                c[ii] += (-Jn * z2[i] + tau(lf, CN) * fn0[s] * z2[i]) * JW;
               if (C->en[CP])
-                c[ii] += (-Jp * z3[i] + C->tau_r[CP] * fp0[s] * z3[i]) * JW;
+                c[ii] += (-Jp * z3[i] + tau(lf, CP) * fp0[s] * z3[i]) * JW;
             }
         }
     }
@@ -1223,7 +1241,7 @@ namespace
               const double sign = comp == CV ? 1 : -1;
               for (size_t i = 0; i < sup.size(); ++i)
                 c[sup[i]] += (-dot(fq[i], normal, d) +
-                              sign * C->tau_r[comp] * fz[i]) *
+                              sign * tau(lf, comp) * fz[i]) *
                              upd_tr[comp][s] * JW;
             }
         }
@@ -1234,6 +1252,7 @@ namespace
     void assemble(long long cell, bool thermo, bool reconstruct,
                   const double *upd_trace)
     {
+      cur = cell;
       std::fill(A.begin(), A.end(), 0.0);
       std::fill(c.begin(), c.end(), 0.0);
       if (!reconstruct)
@@ -1817,6 +1836,23 @@ void orc_set_bc_values(orc_ctx *C, const double *dir, const double *neu)
     std::copy(neu, neu + n, C->neu_vals.begin());
 }
 
+// CellFaceTauNPSolverParameters::set_face values for every (cell, face):
+// tau[(cell*2d + face)*3 + comp], physical; rescaled as CellFaceTauComputer
+// does (src/np_solver_parameters.cpp:79-95, include/np_solver_parameters.h:
+// 157-236).  NULL returns to the fixed tau of the problem.
+void orc_set_cell_face_tau(orc_ctx *C, const double *tau)
+{
+  if (!tau)
+    {
+      C->tau_cf.clear();
+      return;
+    }
+  const long long n = C->ncells * C->nfaces_cell * 3;
+  C->tau_cf.resize(n);
+  for (long long i = 0; i < n; ++i)
+    C->tau_cf[i] = tau[i] / C->tau_rf[i % 3];
+}
+
 void orc_set_state(orc_ctx *C, const double *cell, const double *trace)
 {
   if (cell)
@@ -1969,10 +2005,12 @@ double orc_pass2(orc_ctx *C, int thermo, int nthreads,
 
 // Terminal current (a quantity this build defines; the reference has no
 // contact-current integral): the numerical fluxes of its trace equations
-// (add_tc_matrix_terms_to_tt_rhs, template.h:1311-1447, use_cell)
-//   F_n =  n mu_n E.nu - U_T mu_n W_n.nu - tau_n (n - lambda_n)
-//   F_p = -p mu_p E.nu - U_T mu_p W_p.nu - tau_p (p - lambda_p)
-// integrated over the faces of one boundary id, outward normal.
+// (add_tc_matrix_terms_to_tt_rhs, template.h:1311-1447)
+//   F_n =  n^ mu_n E.nu - U_T mu_n W_n.nu - tau_n (n - lambda_n)
+//   F_p = -p^ mu_p E.nu - U_T mu_p W_p.nu - tau_p (p - lambda_p)
+// with n^ = n (use_cell), lambda_n (use_trace) or, for qiu_shi, n where
+// E.nu > 0 and lambda_n elsewhere (template.h:1417, 1435); integrated over
+// the faces of one boundary id, outward normal.
 int orc_terminal_current(orc_ctx *C, int bid, double out[2])
 {
   if (bid < 0 || bid >= 2 * C->P.dim)
@@ -1986,16 +2024,21 @@ int orc_terminal_current(orc_ctx *C, int bid, double out[2])
     {
       if (boundary_id(C, cell, bid) != bid)
         continue;
+      W.cur = cell;
       W.prepare_face(cell, bid, false, nullptr);
+      const int ft = C->P.flux_type;
       for (int s = 0; s < C->Qf; ++s)
         {
           const double En  = dot(W.fE0[s], W.normal, C->P.dim);
           const double Wnn = dot(W.fWn0[s], W.normal, C->P.dim);
           const double Wpn = dot(W.fWp0[s], W.normal, C->P.dim);
-          sn += W.fJxW[s] * (W.fn0[s] * C->mun_r * En - W.UTf[s] * C->mun_r * Wnn -
-                             C->tau_r[CN] * (W.fn0[s] - W.trn0[s]));
-          sp += W.fJxW[s] * (-W.fp0[s] * C->mup_r * En - W.UTf[s] * C->mup_r * Wpn -
-                             C->tau_r[CP] * (W.fp0[s] - W.trp0[s]));
+          const bool   cl  = ft == 0 || (ft == 2 && En > 0);
+          const double nn = cl ? W.fn0[s] : W.trn0[s];
+          const double pp = cl ? W.fp0[s] : W.trp0[s];
+          sn += W.fJxW[s] * (nn * C->mun_r * En - W.UTf[s] * C->mun_r * Wnn -
+                             W.tau(bid, CN) * (W.fn0[s] - W.trn0[s]));
+          sp += W.fJxW[s] * (-pp * C->mup_r * En - W.UTf[s] * C->mup_r * Wpn -
+                             W.tau(bid, CP) * (W.fp0[s] - W.trp0[s]));
         }
     }
   out[0] = sn;

@@ -84,6 +84,9 @@ void orc_set_doping(orc_ctx *ctx, const double *doping_qp);
  * (entries of interior faces are ignored); NULL leaves zeros */
 void orc_set_bc_values(orc_ctx *ctx, const double *dirichlet,
                        const double *neumann);
+/* CellFaceTauComputer: tau per (cell, face, component), physical,
+ * tau[(cell*2d + face)*3 + comp]; NULL returns to the problem's fixed tau */
+void orc_set_cell_face_tau(orc_ctx *ctx, const double *tau);
 void orc_set_state(orc_ctx *ctx, const double *cell, const double *trace);
 void orc_get_state(const orc_ctx *ctx, double *cell, double *trace);
 

@@ -66,6 +66,7 @@ def lib():
             "orc_set_doping": (None, [v, dp]),
             "orc_set_bc_values": (None, [v, dp, dp]),
             "orc_set_state": (None, [v, dp, dp]),
+            "orc_set_cell_face_tau": (None, [v, dp]),
             "orc_get_state": (None, [v, dp, dp]),
             "orc_local_system": (i, [v, i, i, dp, dp, dp, dp, dp, dp]),
             "orc_condensed_cell": (i, [v, i, i, dp, dp, lp, ip, lp, dp]),
@@ -156,6 +157,17 @@ class Oracle:
         d = np.ascontiguousarray(dirichlet if dirichlet is not None else np.zeros(n), dtype=np.float64)
         m = np.ascontiguousarray(neumann if neumann is not None else np.zeros(n), dtype=np.float64)
         self.L.orc_set_bc_values(self.ctx, _dp(d), _dp(m))
+
+    def set_cell_face_tau(self, tau):
+        """CellFaceTauComputer values, physical, shape (ncells, 2*dim, 3)
+        (cell, face, V/n/p); None returns to the problem's fixed tau"""
+        if tau is None:
+            self.L.orc_set_cell_face_tau(self.ctx, None)
+            return
+        a = np.ascontiguousarray(tau, dtype=np.float64)
+        assert a.size == self.ncells * 2 * self.dim * 3
+        self._tau_cf = a
+        self.L.orc_set_cell_face_tau(self.ctx, _dp(a))
 
     def set_state(self, cell, trace):
         c = np.ascontiguousarray(cell, dtype=np.float64)

@@ -14,7 +14,10 @@ def oracle_for(problem, params, adim, enabled=(True, True, True)):
     """Oracle instance of a ddhdg_amd problem (physical data at its own
     quadrature points)."""
     rec = problem.recombination_term
-    from ddhdg_amd.problem import LinearRecombinationTerm, ShockleyReadHallFixedTemperature
+    from ddhdg_amd.problem import (CellFaceTauNPSolverParameters, LinearRecombinationTerm,
+                                   ShockleyReadHallFixedTemperature, flux_type_code)
+    cellface = isinstance(params, CellFaceTauNPSolverParameters)
+    tau = (0.0, 0.0, 0.0) if cellface else (params.V_tau, params.n_tau, params.p_tau)
     if isinstance(rec, ShockleyReadHallFixedTemperature):
         rtype, rp = 1, (rec.intrinsic_carrier_concentration, rec.electron_life_time,
                         rec.hole_life_time)
@@ -29,12 +32,15 @@ def oracle_for(problem, params, adim, enabled=(True, True, True)):
         mu_p=problem.p_mobility.mu, temperature=float(problem.temperature),
         nc=problem.conduction_band_density, nv=problem.valence_band_density,
         ec=problem.conduction_band_edge_energy, ev=problem.valence_band_edge_energy,
-        recombination=rtype, rec_params=rp, tau=(params.V_tau, params.n_tau, params.p_tau),
+        recombination=rtype, rec_params=rp, tau=tau,
+        flux_type=flux_type_code(params.dd_flux_type), phi_linearize=params.phi_linearize,
         adim=(adim.scale_length, adim.temperature_magnitude, adim.doping_magnitude,
               adim.mobility_magnitude),
         bc_type=problem.boundary_handler.bc_type_table(),
         drift_jacobian=int(params.consistent_drift_jacobian))
     o = O.Oracle(P)
+    if cellface:
+        o.set_cell_face_tau(params.tau_table(o.ncells, 2 * mesh.dim))
     o.set_enabled(*enabled)
     o.set_doping(evaluate(problem.doping, o.cell_qpoints()))
     fq = o.face_qpoints()
