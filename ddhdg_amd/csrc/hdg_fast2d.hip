@@ -12,14 +12,17 @@
 //   1. scalar-test rows [A' | CT' | c'] in registers
 //   2. trace-test rows of the scalar columns, TC', staged in LDS
 //   3. Gauss-Jordan on the scalar rows with scaled column pivoting: equation k
-//      picks the unknown (lane) with the largest scaled |entry|, the pivot
-//      column goes through LDS; afterwards row k of a trace/rhs lane holds
-//      X(p_k, m), X = A'^-1 [CT' | c']
+//      picks the unknown (lane) with the largest scaled |entry| (a DPP
+//      max-reduction), the pivot column reaches the other lanes by readlane;
+//      afterwards row k of a trace/rhs lane holds X(p_k, m),
+//      X = A'^-1 [CT' | c']
 //   4. trace/rhs lanes form their trace-test rows TT', t' face by face and
 //      subtract TC' X: [S | g] = [TT' - TC' A'^-1 CT' | t' - TC' A'^-1 c'],
 //      written straight into the trace CSR (two colour launches,
 //      deterministic)
-// Workgroups of 4 waves are persistent; shared tables are staged in LDS once.
+// Workgroups of 4 waves are persistent.  Tables read at lane-dependent
+// indices are staged in LDS once per workgroup; the test tables read at
+// uniform indices (Bz, Bt) come through scalar loads.
 #include "hdg_device.h"
 #include "hdg_kernels.h"
 
@@ -67,9 +70,7 @@ namespace hdg
       static constexpr int tPMC   = tJXWF + NFC * QF;
       static constexpr int tPMF   = tPMC + QC * NB;
       static constexpr int tMFI   = tPMF + NFC * QF * NB;
-      static constexpr int tBZ    = tMFI + 2 * NF * NF;              // (QC*3+NFC*QF)*NB
-      static constexpr int tBT    = tBZ + (QC * 3 + NFC * QF) * NB;  // NFC*QF*NF
-      static constexpr int TAB    = tBT + NFC * QF * NF;
+      static constexpr int TAB    = tMFI + 2 * NF * NF;
       // per-wave scratch (doubles)
       static constexpr int wXS   = 0;
       static constexpr int wXT   = wXS + NCD;
@@ -108,6 +109,116 @@ namespace hdg
       }
     };
 
+    // read-only tables addressed uniformly by the whole wave: loads through
+    // the constant address space become scalar loads (no LDS traffic, no
+    // VGPRs for the values)
+    typedef const __attribute__((address_space(4))) double ctab;
+
+    // the value v of lane l (l uniform), in every lane: two readlanes, no
+    // LDS round trip
+    __device__ __forceinline__ double lane_value(double v, int l)
+    {
+      const long long b  = __double_as_longlong(v);
+      const int       lo = __builtin_amdgcn_readlane((int)b, l);
+      const int       hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+      return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+    }
+    __device__ __forceinline__ float lane_value(float v, int l)
+    {
+      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+    }
+
+    // wave-wide max of an unsigned key with DPP row shifts and row
+    // broadcasts: lane 15 of every row of 16 ends with its row's max, rows
+    // 1 and 3 then take rows 0 and 2, rows 2 and 3 take lane 31; lane 63
+    // holds the max of the wave
+    template <int CTRL, int ROWS>
+    __device__ __forceinline__ unsigned dpp_max_step(unsigned v)
+    {
+      const unsigned o = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+      return o > v ? o : v;
+    }
+    __device__ __forceinline__ unsigned wave_max(unsigned v)
+    {
+      v = dpp_max_step<0x111, 0xf>(v); // row_shr:1
+      v = dpp_max_step<0x112, 0xf>(v); // row_shr:2
+      v = dpp_max_step<0x114, 0xf>(v); // row_shr:4
+      v = dpp_max_step<0x118, 0xf>(v); // row_shr:8
+      v = dpp_max_step<0x142, 0xa>(v); // row_bcast:15
+      v = dpp_max_step<0x143, 0xc>(v); // row_bcast:31
+      return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+    }
+
+    // pivot-search key: the leading bits of the non-negative scaled
+    // candidate (they order as integers) with 63 - lane in the low 6 bits
+    // (ties go to the lowest lane); 0 for a lane that is not a candidate
+    __device__ __forceinline__ unsigned pivot_key(double sc, bool cand, int lane)
+    {
+      return cand ? ((unsigned)(__double_as_longlong(sc) >> 32) & ~63u) +
+                      (unsigned)(63 - lane) + 1u
+                  : 0u;
+    }
+    __device__ __forceinline__ unsigned pivot_key(float sc, bool cand, int lane)
+    {
+      return cand ? (__float_as_uint(sc) & ~63u) + (unsigned)(63 - lane) + 1u : 0u;
+    }
+
+    // Gauss-Jordan over the NS scalar rows of the lanes' columns with
+    // scaled column pivoting: equation k takes the unpivoted scalar lane
+    // with the largest |entry| / column max; that column reaches every lane
+    // by readlane.  Lane 0 records the pivot lanes in pivs (LDS).
+    template <int NS, typename Real>
+    __device__ __forceinline__ void gauss_jordan(Real (&R)[NS], bool isU, int lane, int *pivs)
+    {
+      Real cs = 0;
+#pragma unroll
+      for (int r = 0; r < NS; ++r)
+        cs = fmax(cs, fabs(R[r]));
+      cs           = cs > Real(0) ? Real(1) / cs : Real(0);
+      bool pivoted = !isU;
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        {
+          const unsigned key = wave_max(pivot_key(fabs(R[k]) * cs, !pivoted, lane));
+          const int      pl  = 63 - (int)((key - 1u) & 63u);
+          if (lane == pl)
+            pivoted = true;
+          if (lane == 0)
+            pivs[k] = pl;
+          Real bq[NS];
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            bq[r] = lane_value(R[r], pl);
+          const Real f = R[k] / bq[k];
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            if (r != k)
+              R[r] -= bq[r] * f;
+          R[k] = f;
+        }
+    }
+
+    // the same elimination with the pivot lanes of an earlier one
+    template <int NS, typename Real>
+    __device__ __forceinline__ void gauss_jordan_fixed(Real (&R)[NS], const int *pivs)
+    {
+#pragma unroll
+      for (int k = 0; k < NS; ++k)
+        {
+          const int pl = __builtin_amdgcn_readfirstlane(pivs[k]);
+          Real      bq[NS];
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            bq[r] = lane_value(R[r], pl);
+          const Real f = R[k] / bq[k];
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            if (r != k)
+              R[r] -= bq[r] * f;
+          R[k] = f;
+        }
+    }
+
     // the reference-cell tables of one mesh into LDS (one workgroup)
     template <int DEG, int NE>
     __device__ __forceinline__ void stage_tables(const DevTables &tb, double *tab0, int tid)
@@ -142,10 +253,6 @@ namespace hdg
         tab0[S::tJXWF + i] = tb.JxWf[i];
       for (int i = tid; i < 2 * NF * NF; i += 64 * WPB)
         tab0[S::tMFI + i] = tb.Mfinv[i];
-      for (int i = tid; i < (QC * 3 + NFC * QF) * NB; i += 64 * WPB)
-        tab0[S::tBZ + i] = tb.Bz[i];
-      for (int i = tid; i < NFC * QF * NF; i += 64 * WPB)
-        tab0[S::tBT + i] = tb.Bt[i];
     }
 
     // one cell (one wave): assembly, condensation and output of pass 1, or
@@ -206,7 +313,7 @@ namespace hdg
       // of the cell loop (hundreds of live values, all spilled)
       double *tab = tab0;
       asm volatile("" : "+v"(tab));
-      const double *Bz = tab + S::tBZ, *Bt = tab + S::tBT;
+      ctab *Bz = (ctab *)tb.Bz, *Bt = (ctab *)tb.Bt;
       int             ic[2];
       ic[0] = (int)(cell % p.n[0]);
       ic[1] = (int)(cell / p.n[0]);
@@ -632,137 +739,42 @@ namespace hdg
 
       // ---- 3: Gauss-Jordan, scaled column pivoting
       if constexpr (!MIXED)
-        {
-        double cs = 0;
-  #pragma unroll
-        for (int r = 0; r < NS; ++r)
-          cs = fmax(cs, fabs(R[r]));
-        cs           = cs > 0 ? 1.0 / cs : 0.0;
-        bool pivoted = !isU;
-  #pragma unroll
-        for (int k = 0; k < NS; ++k)
-          {
-            // order-preserving key: non-negative doubles compare as integers
-            const double       sc  = pivoted ? 0.0 : fabs(R[k]) * cs;
-            unsigned long long key = pivoted ? 0ull
-                                             : (((unsigned long long)__double_as_longlong(sc) &
-                                                 ~63ull) | (unsigned long long)(63 - lane)) + 1ull;
-  #pragma unroll
-            for (int off = 32; off > 0; off >>= 1)
-              {
-                const unsigned long long o = __shfl_xor(key, off);
-                key                        = o > key ? o : key;
-              }
-            const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane((unsigned)(key - 1ull))) & 63u);
-            if (lane == pl)
-              {
-                pivoted = true;
-  #pragma unroll
-                for (int r = 0; r < NS; ++r)
-                  bc[r] = R[r];
-              }
-            if (lane == 0)
-              pivs[k] = pl;
-            WAVE_SYNC();
-            const double f = R[k] / bc[k];
-  #pragma unroll
-            for (int r = 0; r < NS; ++r)
-              if (r != k)
-                R[r] -= bc[r] * f;
-            R[k] = f;
-            WAVE_SYNC();
-          }
-
-        }
+        gauss_jordan<NS>(R, isU, lane, pivs);
       else
         {
           // FP32 elimination with one FP64 refinement step: x0 from the
           // FP32 elimination, r = b - A' x0 in FP64 (the scalar lanes keep
           // their FP64 columns of A' untouched in R), the correction from
           // a second FP32 elimination with the same pivots, x = x0 + dx
-          float *bcf = reinterpret_cast<float *>(bc);
-          float  R32[NS], X0[NS];
-          float  cs32 = 0.f;
+          float R32[NS], X0[NS];
 #pragma unroll
           for (int r = 0; r < NS; ++r)
-            {
-              R32[r] = (float)R[r];
-              cs32   = fmaxf(cs32, fabsf(R32[r]));
-            }
-          cs32         = cs32 > 0.f ? 1.f / cs32 : 0.f;
-          bool pivoted = !isU;
-#pragma unroll
-          for (int k = 0; k < NS; ++k)
-            {
-              const float sc  = pivoted ? 0.f : fabsf(R32[k]) * cs32;
-              unsigned    key = pivoted ? 0u
-                                        : ((__float_as_uint(sc) & ~63u) | (unsigned)(63 - lane)) + 1u;
-#pragma unroll
-              for (int off = 32; off > 0; off >>= 1)
-                {
-                  const unsigned o = __shfl_xor(key, off);
-                  key              = o > key ? o : key;
-                }
-              const int pl = 63 - (int)((__builtin_amdgcn_readfirstlane(key - 1u)) & 63u);
-              if (lane == pl)
-                {
-                  pivoted = true;
-#pragma unroll
-                  for (int r = 0; r < NS; ++r)
-                    bcf[r] = R32[r];
-                }
-              if (lane == 0)
-                pivs[k] = pl;
-              WAVE_SYNC();
-              const float f = R32[k] / bcf[k];
-#pragma unroll
-              for (int r = 0; r < NS; ++r)
-                if (r != k)
-                  R32[r] -= bcf[r] * f;
-              R32[k] = f;
-              WAVE_SYNC();
-            }
+            R32[r] = (float)R[r];
+          gauss_jordan<NS>(R32, isU, lane, pivs);
 #pragma unroll
           for (int k = 0; k < NS; ++k)
             X0[k] = R32[k];
+          WAVE_SYNC();
           const double mres = isU ? 0.0 : 1.0;
 #pragma unroll
           for (int k = 0; k < NS; ++k)
             {
-              if (lane == pivs[k])
-#pragma unroll
-                for (int r = 0; r < NS; ++r)
-                  bc[r] = R[r];
-              WAVE_SYNC();
+              const int    pl = __builtin_amdgcn_readfirstlane(pivs[k]);
               const double xk = mres * (double)X0[k];
 #pragma unroll
               for (int r = 0; r < NS; ++r)
-                R[r] -= bc[r] * xk;
-              WAVE_SYNC();
+                R[r] -= lane_value(R[r], pl) * xk;
             }
 #pragma unroll
           for (int r = 0; r < NS; ++r)
             R32[r] = (float)R[r];
-#pragma unroll
-          for (int k = 0; k < NS; ++k)
-            {
-              if (lane == pivs[k])
-#pragma unroll
-                for (int r = 0; r < NS; ++r)
-                  bcf[r] = R32[r];
-              WAVE_SYNC();
-              const float f = R32[k] / bcf[k];
-#pragma unroll
-              for (int r = 0; r < NS; ++r)
-                if (r != k)
-                  R32[r] -= bcf[r] * f;
-              R32[k] = f;
-              WAVE_SYNC();
-            }
+          gauss_jordan_fixed<NS>(R32, pivs);
 #pragma unroll
           for (int k = 0; k < NS; ++k)
             R[k] = (double)X0[k] + (double)R32[k];
         }
+      // the pivot lanes (LDS, lane 0) are read below
+      WAVE_SYNC();
 
       if (PASS2)
         {
