@@ -25,6 +25,7 @@
 // uniform indices (Bz, Bt) come through scalar loads.
 #include "hdg_device.h"
 #include "hdg_kernels.h"
+#include "hdg_wave.h"
 
 #define WAVE_SYNC()                                          \
   do                                                         \
@@ -114,49 +115,12 @@ namespace hdg
     // VGPRs for the values)
     typedef const __attribute__((address_space(4))) double ctab;
 
-    // the value v of lane l (l uniform), in every lane: two readlanes, no
-    // LDS round trip
-    __device__ __forceinline__ double lane_value(double v, int l)
-    {
-      const long long b  = __double_as_longlong(v);
-      const int       lo = __builtin_amdgcn_readlane((int)b, l);
-      const int       hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-      return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
-    }
-    __device__ __forceinline__ float lane_value(float v, int l)
-    {
-      return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-    }
-
-    // wave-wide max of an unsigned key with DPP row shifts and row
-    // broadcasts: lane 15 of every row of 16 ends with its row's max, rows
-    // 1 and 3 then take rows 0 and 2, rows 2 and 3 take lane 31; lane 63
-    // holds the max of the wave
-    template <int CTRL, int ROWS>
-    __device__ __forceinline__ unsigned dpp_max_step(unsigned v)
-    {
-      const unsigned o = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
-      return o > v ? o : v;
-    }
-    __device__ __forceinline__ unsigned wave_max(unsigned v)
-    {
-      v = dpp_max_step<0x111, 0xf>(v); // row_shr:1
-      v = dpp_max_step<0x112, 0xf>(v); // row_shr:2
-      v = dpp_max_step<0x114, 0xf>(v); // row_shr:4
-      v = dpp_max_step<0x118, 0xf>(v); // row_shr:8
-      v = dpp_max_step<0x142, 0xa>(v); // row_bcast:15
-      v = dpp_max_step<0x143, 0xc>(v); // row_bcast:31
-      return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
-    }
-
     // pivot-search key: the leading bits of the non-negative scaled
     // candidate (they order as integers) with 63 - lane in the low 6 bits
     // (ties go to the lowest lane); 0 for a lane that is not a candidate
     __device__ __forceinline__ unsigned pivot_key(double sc, bool cand, int lane)
     {
-      return cand ? ((unsigned)(__double_as_longlong(sc) >> 32) & ~63u) +
-                      (unsigned)(63 - lane) + 1u
-                  : 0u;
+      return cand ? (leading_bits(sc) & ~63u) + (unsigned)(63 - lane) + 1u : 0u;
     }
     __device__ __forceinline__ unsigned pivot_key(float sc, bool cand, int lane)
     {

@@ -23,9 +23,14 @@
 
 #include "hdg_ctx.h"
 #include "hdg_nd_plan.h"
+#include "hdg_wave.h"
 
 namespace
 {
+  using hdg::lane_value;
+  using hdg::leading_bits;
+  using hdg::wave_max;
+
   struct DevGroup
   {
     long long *d_pf = nullptr, *d_pw = nullptr, *d_edof = nullptr, *d_idof = nullptr;
@@ -175,13 +180,12 @@ namespace
   constexpr int FT      = 256; // threads of the small-front kernels
 
   __global__ void __launch_bounds__(FT)
-    k_front_lu_small(double *F, long long fbase, int n, int S, int I, int *ipiv,
+    k_front_lu_small(double *F, long long fbase, int n, int S, int I, int *ipiv, int *perm,
                      long long pbase, int *info, int batch, int yoff)
   {
     __shared__ double A[SMALL_S * SMALL_S]; // F11, column-major, ld S
-    __shared__ double red[FT];
-    __shared__ int    redi[FT];
-    __shared__ int    piv[SMALL_S];
+    __shared__ int    piv[SMALL_S], pm[SMALL_S];
+    __shared__ int    pk;
     const int b = blockIdx.x + yoff;
     if (b >= batch)
       return;
@@ -189,47 +193,38 @@ namespace
     double   *Fb = F + fbase + (long long)b * n * n;
     for (int e = t; e < S * S; e += FT)
       A[e] = Fb[(e % S) + (long long)(e / S) * n];
+    for (int k = t; k < S; k += FT)
+      pm[k] = k;
     __syncthreads();
     int bad = 0;
     for (int k = 0; k < S; ++k)
       {
-        // pivot: largest |A(i,k)|, i >= k (lowest row on ties)
-        double best = -1.0;
-        int    bi   = k;
-        for (int i = k + t; i < S; i += FT)
+        // pivot: largest |A(i,k)|, i >= k, to the leading bits of the
+        // value (lowest row on ties), found by wave 0 with a DPP max
+        if (t < 64)
           {
-            const double v = fabs(A[i + k * S]);
-            if (v > best)
+            unsigned key = 0;
+            for (int i = k + t; i < S; i += 64)
               {
-                best = v;
-                bi   = i;
+                const unsigned c = (leading_bits(fabs(A[i + k * S])) & ~127u) | (unsigned)(127 - i);
+                key              = c > key ? c : key;
               }
+            key = wave_max(key);
+            if (t == 0)
+              pk = 127 - (int)(key & 127u);
           }
-        red[t]  = best;
-        redi[t] = bi;
         __syncthreads();
-        for (int off = FT / 2; off > 0; off >>= 1)
-          {
-            if (t < off)
-              {
-                const double o = red[t + off];
-                const int    oi = redi[t + off];
-                if (o > red[t] || (o == red[t] && oi < redi[t]))
-                  {
-                    red[t]  = o;
-                    redi[t] = oi;
-                  }
-              }
-            __syncthreads();
-          }
-        const int p = redi[0];
+        const int p = pk;
         if (t == 0)
           {
             piv[k] = p;
-            if (red[0] == 0.0 && !bad)
-              bad = k + 1;
+            if (p != k)
+              {
+                const int tmp = pm[k];
+                pm[k]         = pm[p];
+                pm[p]         = tmp;
+              }
           }
-        __syncthreads();
         if (p != k)
           for (int j = t; j < S; j += FT)
             {
@@ -239,6 +234,8 @@ namespace
             }
         __syncthreads();
         const double d = A[k + k * S];
+        if (t == 0 && d == 0.0 && !bad)
+          bad = k + 1;
         for (int i = k + 1 + t; i < S; i += FT)
           A[i + k * S] = d != 0.0 ? A[i + k * S] / d : 0.0;
         __syncthreads();
@@ -255,86 +252,110 @@ namespace
     for (int e = t; e < S * S; e += FT)
       Fb[(e % S) + (long long)(e / S) * n] = A[e];
     for (int k = t; k < S; k += FT)
-      ipiv[pbase + (long long)b * S + k] = piv[k] + 1;
+      {
+        ipiv[pbase + (long long)b * S + k] = piv[k] + 1;
+        perm[pbase + (long long)b * S + k] = pm[k];
+      }
   }
 
-  // forward sweep of one small front: P, L11^-1, then w_i -= L21 z
-  __global__ void __launch_bounds__(FT)
+  // forward sweep of one small front, one wave: z = L11^-1 P w_s with L11
+  // staged in LDS and z in registers (rows lane and lane + 64; each
+  // solved entry reaches the other lanes by readlane), then w_i -= L21 z.
+  // perm: row k of the factored block is row perm[k] of the front.
+  __global__ void __launch_bounds__(64)
     k_fwd_small(const double *F, double *w, long long fbase, long long wbase, int n, int S,
-                int I, const int *ipiv, long long pbase, int batch, int yoff)
+                int I, const int *perm, long long pbase, int batch, int yoff)
   {
-    __shared__ double z[SMALL_S];
+    extern __shared__ double L[]; // S x S, column-major
     const int b = blockIdx.x + yoff;
     if (b >= batch)
       return;
     const int     t  = threadIdx.x;
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
-    const int    *pv = ipiv + pbase + (long long)b * S;
-    if (t == 0)
+    const int    *pv = perm + pbase + (long long)b * S;
+    for (int e = t; e < S * S; e += 64)
+      L[e] = Fb[(e % S) + (long long)(e / S) * n];
+    const int i0 = t, i1 = t + 64;
+    double    z0 = i0 < S ? wb[pv[i0]] : 0.0;
+    double    z1 = i1 < S ? wb[pv[i1]] : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int j = 0; j + 1 < S; ++j)
       {
-        for (int k = 0; k < S; ++k)
-          z[k] = wb[k];
-        for (int k = 0; k < S; ++k)
-          {
-            const int p = pv[k] - 1;
-            if (p != k)
-              {
-                const double tmp = z[k];
-                z[k]             = z[p];
-                z[p]             = tmp;
-              }
-          }
-        for (int j = 0; j < S; ++j)
-          {
-            const double y = z[j];
-            for (int i = j + 1; i < S; ++i)
-              z[i] -= Fb[i + (long long)j * n] * y;
-          }
+        const double y = j < 64 ? lane_value(z0, j) : lane_value(z1, j - 64);
+        if (i0 > j && i0 < S)
+          z0 -= L[i0 + j * S] * y;
+        if (i1 > j && i1 < S)
+          z1 -= L[i1 + j * S] * y;
       }
-    __syncthreads();
-    for (int k = t; k < S; k += FT)
-      wb[k] = z[k];
-    for (int r = t; r < I; r += FT)
+    // every lane has read its pivoted entries of w_s: overwrite them
+    __builtin_amdgcn_wave_barrier();
+    if (i0 < S)
+      wb[i0] = z0;
+    if (i1 < S)
+      wb[i1] = z1;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int r = t; r < I; r += 64)
       {
         double acc = wb[S + r];
         for (int j = 0; j < S; ++j)
-          acc -= Fb[S + r + (long long)j * n] * z[j];
+          acc -= Fb[S + r + (long long)j * n] * wb[j];
         wb[S + r] = acc;
       }
   }
 
-  // backward sweep of one small front: z -= U12 x_i, then U11^-1
-  __global__ void __launch_bounds__(FT)
+  // backward sweep of one small front, one wave: z = w_s - U12 x_i, then
+  // U11^-1 z with U11 staged in LDS and z in registers
+  __global__ void __launch_bounds__(64)
     k_bwd_small(const double *F, double *w, long long fbase, long long wbase, int n, int S,
                 int I, int batch, int yoff)
   {
-    __shared__ double z[SMALL_S];
+    extern __shared__ double U[]; // S x S, column-major
     const int b = blockIdx.x + yoff;
     if (b >= batch)
       return;
     const int     t  = threadIdx.x;
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
-    for (int j = t; j < S; j += FT)
+    for (int e = t; e < S * S; e += 64)
+      U[e] = Fb[(e % S) + (long long)(e / S) * n];
+    const int i0 = t, i1 = t + 64;
+    double    z0 = 0.0, z1 = 0.0;
+    if (i0 < S)
       {
-        double acc = wb[j];
+        z0 = wb[i0];
         for (int c = 0; c < I; ++c)
-          acc -= Fb[j + (long long)(S + c) * n] * wb[S + c];
-        z[j] = acc;
+          z0 -= Fb[i0 + (long long)(S + c) * n] * wb[S + c];
       }
-    __syncthreads();
-    if (t == 0)
-      for (int j = S - 1; j >= 0; --j)
-        {
-          double acc = z[j];
-          for (int l = j + 1; l < S; ++l)
-            acc -= Fb[j + (long long)l * n] * z[l];
-          z[j] = acc / Fb[j + (long long)j * n];
-        }
-    __syncthreads();
-    for (int j = t; j < S; j += FT)
-      wb[j] = z[j];
+    if (i1 < S)
+      {
+        z1 = wb[i1];
+        for (int c = 0; c < I; ++c)
+          z1 -= Fb[i1 + (long long)(S + c) * n] * wb[S + c];
+      }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int j = S - 1; j >= 0; --j)
+      {
+        const double y = (j < 64 ? lane_value(z0, j) : lane_value(z1, j - 64)) / U[j + j * S];
+        if (i0 == j)
+          z0 = y;
+        else if (i0 < j)
+          z0 -= U[i0 + j * S] * y;
+        if (i1 == j)
+          z1 = y;
+        else if (i1 < j)
+          z1 -= U[i1 + j * S] * y;
+      }
+    if (i0 < S)
+      wb[i0] = z0;
+    if (i1 < S)
+      wb[i1] = z1;
   }
 
   __global__ void k_gather_rhs(const double *__restrict__ b, const long long *__restrict__ wmap,
@@ -438,6 +459,7 @@ struct hdg_trace_solver
   rocblas_handle        h      = nullptr;
   double               *d_F    = nullptr, *d_w = nullptr;
   int                  *d_ipiv = nullptr, *d_info = nullptr;
+  int                  *d_perm = nullptr; // row order of the small fronts' factored blocks
   long long            *d_map = nullptr, *d_pad = nullptr, *d_wmap = nullptr;
 };
 
@@ -491,6 +513,8 @@ int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
       if (e == hipSuccess)
         e = hipMalloc((void **)&T->d_ipiv, (size_t)std::max<long long>(1, P.ptot) * sizeof(int));
       if (e == hipSuccess)
+        e = hipMalloc((void **)&T->d_perm, (size_t)std::max<long long>(1, P.ptot) * sizeof(int));
+      if (e == hipSuccess)
         e = hipMalloc((void **)&T->d_info, (size_t)P.nnodes * sizeof(int));
       if (e != hipSuccess)
         rc = hdg::set_error(std::string("hdg_trace_solver_create: fronts: ") +
@@ -528,7 +552,8 @@ void hdg_trace_solver_destroy(hdg_trace_solver *T)
                     (void *)D.d_pos, (void *)D.d_edof, (void *)D.d_idof})
       if (p)
         (void)hipFree(p);
-  for (void *p : {(void *)T->d_F, (void *)T->d_w, (void *)T->d_ipiv, (void *)T->d_info,
+  for (void *p : {(void *)T->d_F, (void *)T->d_w, (void *)T->d_ipiv, (void *)T->d_perm,
+                  (void *)T->d_info,
                   (void *)T->d_map, (void *)T->d_pad, (void *)T->d_wmap})
     if (p)
       hipFree(p);
@@ -579,7 +604,7 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
             for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
               hipLaunchKernelGGL(k_front_lu_small, dim3(std::min(YCHUNK, G.batch - y0)),
                                  dim3(FT), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, T->d_ipiv,
-                                 G.pbase, T->d_info + info_off, G.batch, y0);
+                                 T->d_perm, G.pbase, T->d_info + info_off, G.batch, y0);
           else
             st = rocsolver_dgetrf_strided_batched(
               T->h, G.S, G.S, F, G.n, (rocblas_stride)G.n * G.n, T->d_ipiv + G.pbase,
@@ -689,9 +714,9 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
       if (G.S > 0 && G.S <= SMALL_S)
         {
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(k_fwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(FT), 0, s,
-                               T->d_F, T->d_w, G.fbase, G.wbase, G.n, G.S, G.I, T->d_ipiv,
-                               G.pbase, G.batch, y0);
+            hipLaunchKernelGGL(k_fwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
+                               (size_t)G.S * G.S * sizeof(double), s, T->d_F, T->d_w, G.fbase,
+                               G.wbase, G.n, G.S, G.I, T->d_perm, G.pbase, G.batch, y0);
         }
       else if (G.S > 0)
         {
@@ -732,8 +757,9 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
       if (G.S > 0 && G.S <= SMALL_S)
         {
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(k_bwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(FT), 0, s,
-                               T->d_F, T->d_w, G.fbase, G.wbase, G.n, G.S, G.I, G.batch, y0);
+            hipLaunchKernelGGL(k_bwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
+                               (size_t)G.S * G.S * sizeof(double), s, T->d_F, T->d_w, G.fbase,
+                               G.wbase, G.n, G.S, G.I, G.batch, y0);
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
             hipLaunchKernelGGL(k_scatter_x,
                                dim3((G.S + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),

@@ -1,0 +1,48 @@
+// hdg_wave.h -- wave64 helpers for gfx950: lane broadcasts by readlane and
+// wave-wide max reductions by DPP (no LDS round trips).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace hdg
+{
+  // the value v of lane l (l uniform), in every lane
+  __device__ __forceinline__ double lane_value(double v, int l)
+  {
+    const long long b  = __double_as_longlong(v);
+    const int       lo = __builtin_amdgcn_readlane((int)b, l);
+    const int       hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+  }
+  __device__ __forceinline__ float lane_value(float v, int l)
+  {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  }
+
+  // wave-wide max of an unsigned key with DPP row shifts and row
+  // broadcasts: lane 15 of every row of 16 ends with its row's max, rows 1
+  // and 3 then take rows 0 and 2, rows 2 and 3 take lane 31; lane 63 holds
+  // the max of the wave, returned to every lane
+  template <int CTRL, int ROWS>
+  __device__ __forceinline__ unsigned dpp_max_step(unsigned v)
+  {
+    const unsigned o = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+    return o > v ? o : v;
+  }
+  __device__ __forceinline__ unsigned wave_max(unsigned v)
+  {
+    v = dpp_max_step<0x111, 0xf>(v); // row_shr:1
+    v = dpp_max_step<0x112, 0xf>(v); // row_shr:2
+    v = dpp_max_step<0x114, 0xf>(v); // row_shr:4
+    v = dpp_max_step<0x118, 0xf>(v); // row_shr:8
+    v = dpp_max_step<0x142, 0xa>(v); // row_bcast:15
+    v = dpp_max_step<0x143, 0xc>(v); // row_bcast:31
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+  }
+
+  // leading 32 bits of a non-negative double: they order as unsigned
+  // integers, as the values do (to about 5 significant digits)
+  __device__ __forceinline__ unsigned leading_bits(double v)
+  {
+    return (unsigned)(__double_as_longlong(v) >> 32);
+  }
+} // namespace hdg
