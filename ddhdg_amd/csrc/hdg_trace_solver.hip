@@ -118,9 +118,12 @@ namespace
   // Batched triangular solves for fronts with a small eliminated block
   // (S <= TRSM_SMAX), one launch per level.  A lane owns one row (right solve)
   // or one column (left solve) of the off-diagonal block; the values it has
-  // solved stay in LDS, the triangular factor is read as wave-uniform
-  // broadcasts.
+  // solved stay in LDS.  The triangular factor is read at wave-uniform
+  // addresses through the constant address space (scalar loads into
+  // SGPRs): this kernel writes only the off-diagonal block, never the
+  // factor it reads.
   constexpr int TRSM_SMAX = 160;
+  typedef const __attribute__((address_space(4))) double cdouble;
 
   // B := B U^-1, B = F21 (I x S, rows S.. of columns 0..S-1), U = upper(F11)
   __global__ void __launch_bounds__(64)
@@ -131,15 +134,25 @@ namespace
     const int i = blockIdx.x * 64 + threadIdx.x;
     if (b >= batch)
       return;
-    const double *U   = F + fbase + (long long)b * n * n;
-    double       *B   = F + fbase + (long long)b * n * n + S;
-    const bool    act = i < I;
+    cdouble   *U   = (cdouble *)(F + fbase + (long long)b * n * n);
+    double    *B   = F + fbase + (long long)b * n * n + S;
+    const bool act = i < I;
     for (int j = 0; j < S; ++j)
       {
-        const double *Uj  = U + (long long)j * n;
-        double        acc = act ? B[i + (long long)j * n] : 0.0;
-        for (int l = 0; l < j; ++l)
-          acc -= xs[l * 64 + threadIdx.x] * Uj[l];
+        cdouble *Uj = U + (long long)j * n;
+        // four partial sums: the FMA chain is the latency bound
+        double a0 = act ? B[i + (long long)j * n] : 0.0, a1 = 0, a2 = 0, a3 = 0;
+        int    l  = 0;
+        for (; l + 3 < j; l += 4)
+          {
+            a0 -= xs[l * 64 + threadIdx.x] * Uj[l];
+            a1 -= xs[(l + 1) * 64 + threadIdx.x] * Uj[l + 1];
+            a2 -= xs[(l + 2) * 64 + threadIdx.x] * Uj[l + 2];
+            a3 -= xs[(l + 3) * 64 + threadIdx.x] * Uj[l + 3];
+          }
+        for (; l < j; ++l)
+          a0 -= xs[l * 64 + threadIdx.x] * Uj[l];
+        const double acc            = (a0 + a1) + (a2 + a3);
         const double x              = acc / Uj[j];
         xs[j * 64 + threadIdx.x]    = x;
         if (act)
@@ -156,15 +169,24 @@ namespace
     const int c = blockIdx.x * 64 + threadIdx.x;
     if (b >= batch)
       return;
-    const double *L   = F + fbase + (long long)b * n * n;
+    cdouble      *L   = (cdouble *)(F + fbase + (long long)b * n * n);
     double       *B   = F + fbase + (long long)b * n * n + (long long)S * n;
     const bool    act = c < I;
     double       *Bc  = B + (long long)(act ? c : 0) * n;
     for (int j = 0; j < S; ++j)
       {
-        double acc = act ? Bc[j] : 0.0;
-        for (int l = 0; l < j; ++l)
-          acc -= L[j + (long long)l * n] * ys[l * 64 + threadIdx.x];
+        double a0 = act ? Bc[j] : 0.0, a1 = 0, a2 = 0, a3 = 0;
+        int    l  = 0;
+        for (; l + 3 < j; l += 4)
+          {
+            a0 -= L[j + (long long)l * n] * ys[l * 64 + threadIdx.x];
+            a1 -= L[j + (long long)(l + 1) * n] * ys[(l + 1) * 64 + threadIdx.x];
+            a2 -= L[j + (long long)(l + 2) * n] * ys[(l + 2) * 64 + threadIdx.x];
+            a3 -= L[j + (long long)(l + 3) * n] * ys[(l + 3) * 64 + threadIdx.x];
+          }
+        for (; l < j; ++l)
+          a0 -= L[j + (long long)l * n] * ys[l * 64 + threadIdx.x];
+        const double acc         = (a0 + a1) + (a2 + a3);
         ys[j * 64 + threadIdx.x] = acc;
         if (act)
           Bc[j] = acc;
@@ -301,10 +323,18 @@ namespace
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (int r = t; r < I; r += 64)
       {
-        double acc = wb[S + r];
-        for (int j = 0; j < S; ++j)
-          acc -= Fb[S + r + (long long)j * n] * wb[j];
-        wb[S + r] = acc;
+        double a0 = wb[S + r], a1 = 0, a2 = 0, a3 = 0;
+        int    j  = 0;
+        for (; j + 3 < S; j += 4)
+          {
+            a0 -= Fb[S + r + (long long)j * n] * wb[j];
+            a1 -= Fb[S + r + (long long)(j + 1) * n] * wb[j + 1];
+            a2 -= Fb[S + r + (long long)(j + 2) * n] * wb[j + 2];
+            a3 -= Fb[S + r + (long long)(j + 3) * n] * wb[j + 3];
+          }
+        for (; j < S; ++j)
+          a0 -= Fb[S + r + (long long)j * n] * wb[j];
+        wb[S + r] = (a0 + a1) + (a2 + a3);
       }
   }
 
@@ -325,17 +355,23 @@ namespace
       U[e] = Fb[(e % S) + (long long)(e / S) * n];
     const int i0 = t, i1 = t + 64;
     double    z0 = 0.0, z1 = 0.0;
-    if (i0 < S)
+    for (int h = 0; h < 2; ++h)
       {
-        z0 = wb[i0];
-        for (int c = 0; c < I; ++c)
-          z0 -= Fb[i0 + (long long)(S + c) * n] * wb[S + c];
-      }
-    if (i1 < S)
-      {
-        z1 = wb[i1];
-        for (int c = 0; c < I; ++c)
-          z1 -= Fb[i1 + (long long)(S + c) * n] * wb[S + c];
+        const int i = h ? i1 : i0;
+        if (i >= S)
+          continue;
+        double a0 = wb[i], a1 = 0, a2 = 0, a3 = 0;
+        int    c  = 0;
+        for (; c + 3 < I; c += 4)
+          {
+            a0 -= Fb[i + (long long)(S + c) * n] * wb[S + c];
+            a1 -= Fb[i + (long long)(S + c + 1) * n] * wb[S + c + 1];
+            a2 -= Fb[i + (long long)(S + c + 2) * n] * wb[S + c + 2];
+            a3 -= Fb[i + (long long)(S + c + 3) * n] * wb[S + c + 3];
+          }
+        for (; c < I; ++c)
+          a0 -= Fb[i + (long long)(S + c) * n] * wb[S + c];
+        (h ? z1 : z0) = (a0 + a1) + (a2 + a3);
       }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
