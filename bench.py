@@ -62,9 +62,11 @@ def cpu_baseline(n_sample, degree, threads, bias):
 
 
 def batched_sample(args, torch, stream):
-    """pass 1 of many devices of uneven sizes and degrees in one persistent
-    launch per colour (autotuned grid) against the same devices condensed one
-    after the other (two launches each)"""
+    """pass 1 of many devices of uneven sizes and degrees: the degree-1 and
+    degree-2 devices in one persistent launch per colour (autotuned grid),
+    the degree-3 devices (LDS kernel, FP64) one after the other; against all
+    devices condensed one after the other (two launches each).  FP64 and
+    FP32 + refinement."""
     import numpy as np
 
     from ddhdg_amd import configs
@@ -75,10 +77,10 @@ def batched_sample(args, torch, stream):
     sv = []
     for _ in range(args.batch):
         n = int(rng.integers(16, 193))
-        k = int(rng.choice([1, 2]))
+        k = int(rng.choice([1, 2, 3]))
         dop = configs.step_doping(10 ** rng.uniform(21, 24), 10 ** rng.uniform(21, 24),
                                   rng.uniform(0.3, 0.7))
-        prob, params, adim = configs.diode(2, n, k, doping=dop)
+        prob, params, adim = configs.diode(2, n, k, doping=dop, bias=float(rng.uniform(0, 0.6)))
         s = NPSolver(prob, params, adim, device="cuda")
         s.compute_local_charge_neutrality()
         s.project_densities_from_potential()
@@ -86,11 +88,20 @@ def batched_sample(args, torch, stream):
     bufs = [(torch.empty(s.ctx.nnz, dtype=torch.float64, device="cuda"),
              torch.empty(s.ctx.ntrace, dtype=torch.float64, device="cuda"),
              torch.zeros(s.ctx.ntrace, dtype=torch.float64, device="cuda")) for s in sv]
-    b = HdgBatch([s.ctx for s in sv])
-    xc, xt = [s.x_cell for s in sv], [s.x_trace for s in sv]
-    v, r, d = [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs]
-    b.condense(False, xc, xt, v, r, d)            # autotunes
-    torch.cuda.synchronize()
+    inb = [i for i, s in enumerate(sv) if s.degree <= 2]
+    rest = [i for i, s in enumerate(sv) if s.degree > 2]
+    b = HdgBatch([sv[i].ctx for i in inb])
+    xc, xt = [sv[i].x_cell for i in inb], [sv[i].x_trace for i in inb]
+    v, r, d = ([bufs[i][j] for i in inb] for j in range(3))
+
+    def batched():
+        b.condense(False, xc, xt, v, r, d)
+        for i in rest:
+            sv[i].ctx.condense(sv[i].x_cell, sv[i].x_trace, False, *bufs[i])
+
+    def sequential():
+        for s, (vv, rr, dd) in zip(sv, bufs):
+            s.ctx.condense(s.x_cell, s.x_trace, False, vv, rr, dd)
 
     def timed(fn, reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -105,20 +116,30 @@ def batched_sample(args, torch, stream):
         return (time.perf_counter() - t0) / reps, e0.elapsed_time(e1) / reps
 
     reps = max(3, min(args.steps, 20))
-    wb, gb = timed(lambda: b.condense(False, xc, xt, v, r, d), reps)
-
-    def sequential():
-        for s, (vv, rr, dd) in zip(sv, bufs):
-            s.ctx.condense(s.x_cell, s.x_trace, False, vv, rr, dd)
-    ws, gs = timed(sequential, reps)
     cells = sum(s.ctx.ncells for s in sv)
-    return {"devices": len(sv), "cells": int(cells),
-            "degrees": {str(k): sum(1 for s in sv if s.degree == k) for k in (1, 2)},
-            "value": cells / wb, "unit": "cells/s", "gpu_ms": gb,
-            "sequential_value": cells / ws, "sequential_gpu_ms": gs,
-            "autotune": b.info(),
-            "workload": f"{len(sv)} 2D diodes, sides uniform in 16..192 cells, degree 1-2, "
-                        "doping log-uniform 1e21-1e24, junction in [0.3, 0.7]; pass 1, FP64"}
+    out = {"devices": len(sv), "cells": int(cells),
+           "degrees": {str(k): sum(1 for s in sv if s.degree == k) for k in (1, 2, 3)}}
+    for prec in (0, 1):
+        for s in sv:
+            s.ctx.set_precision(prec)
+        batched()                      # first call autotunes the grid
+        torch.cuda.synchronize()
+        wb, gb = timed(batched, reps)
+        ws, gs = timed(sequential, reps)
+        key = "fp64" if prec == 0 else "fp32_refine"
+        out[key] = {"value": cells / wb, "unit": "cells/s", "gpu_ms": gb,
+                    "sequential_value": cells / ws, "sequential_gpu_ms": gs}
+    for s in sv:
+        s.ctx.set_precision(0)
+    out["value"] = out["fp64"]["value"]
+    out["unit"] = "cells/s"
+    out["autotune"] = b.info()
+    out["workload"] = (f"{len(sv)} 2D diodes from seed 20261203: sides uniform in 16..192 cells, "
+                       "degree uniform in {1, 2, 3}, doping log-uniform 1e21-1e24, junction in "
+                       "[0.3, 0.7], contact bias uniform in [0, 0.6] V; pass 1; degrees 1-2 in one "
+                       "persistent launch per colour, degree 3 on the LDS kernel in FP64 "
+                       "(no FP32 path there)")
+    return out
 
 
 def main():
@@ -336,11 +357,22 @@ def main():
         res, swall = srv.run(reqs)
         eager = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots, graph=False)
         res_e, wall_e = eager.run(reqs)
+        mixed = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots, graph=True,
+                         precision=1)
+        res_m, wall_m = mixed.run(reqs)
+        keys = ("solves_per_s", "p50_latency_s", "p99_latency_s")
+        # FP32+refine against FP64: largest difference of the terminal
+        # currents of a sweep relative to the sweep's largest current (the
+        # low-bias currents are small differences of O(1) fluxes)
+        dI = max(max(abs(a - b) for a, b in zip(rm.currents, rf.currents)) /
+                 max(max(abs(b) for b in rf.currents), 1e-300) for rf, rm in zip(res, res_m))
         line["serving"] = {**summarize(res, swall), "max_concurrent": srv.max_active,
                            "graph_newton": True,
                            "without_graphs": {k: v for k, v in summarize(res_e, wall_e).items()
-                                              if k in ("solves_per_s", "p50_latency_s",
-                                                       "p99_latency_s")},
+                                              if k in keys},
+                           "fp32_refine": {**{k: v for k, v in summarize(res_m, wall_m).items()
+                                              if k in keys + ("newton_iterations",)},
+                                           "max_current_diff_vs_fp64_rel_to_sweep_max": dI},
                            "workload": f"{args.serve} I-V sweeps (13 points 0-0.6 V), 2D, "
                                        f"{32}^2-{96}^2 cells log-uniform, degree 1-2, Poisson "
                                        f"arrivals {args.serve_rate}/s, FP64",

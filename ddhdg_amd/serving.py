@@ -83,7 +83,7 @@ def memory_estimate(req: IVRequest) -> int:
 class _Slot:
     """one admitted request: its solver, stream and stage machine"""
 
-    def __init__(self, req: IVRequest, t0: float, now: float, torch, graph=True):
+    def __init__(self, req: IVRequest, t0: float, now: float, torch, graph=True, precision=0):
         self.req, self.torch, self.graph = req, torch, graph
         self.stream = torch.cuda.Stream()
         with torch.cuda.stream(self.stream):
@@ -92,6 +92,7 @@ class _Slot:
                 doping=configs.step_doping(req.nd, req.na, req.junction))
             self.solver = NPSolver(prob, params, adim)
             s = self.solver
+            s.ctx.set_precision(precision)
             s.set_enabled_components(True, False, False)
             s.compute_local_charge_neutrality()
         self.stage = 0                        # 0 eq, 1 n/p, 2 coupled, 3.. biases
@@ -150,11 +151,14 @@ class IVServer:
     """Admits I-V requests by HBM budget into at most max_slots concurrent
     slots and runs them with continuous batching (see the module doc)."""
 
-    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, keep_state=False, graph=True):
+    def __init__(self, hbm_budget_bytes=64 << 30, max_slots=8, keep_state=False, graph=True,
+                 precision=0):
+        """precision: 0 FP64, 1 FP32 local elimination + FP64 refinement
+        (hdg_set_precision) in every request's pass 1"""
         import torch
         self.torch = torch
         self.budget, self.max_slots = int(hbm_budget_bytes), int(max_slots)
-        self.keep_state, self.graph = keep_state, graph
+        self.keep_state, self.graph, self.precision = keep_state, graph, int(precision)
 
     def run(self, requests, clock=None):
         """serve requests (arrival times in seconds from the start); returns
@@ -173,7 +177,7 @@ class IVServer:
                     break
                 req = pending.pop(0)
                 used += need
-                active.append((_Slot(req, t0, now(), torch, self.graph), need))
+                active.append((_Slot(req, t0, now(), torch, self.graph, self.precision), need))
             if not active:
                 wait = pending[0].arrival - now()
                 if wait > 0:
