@@ -85,11 +85,28 @@ namespace
     return thermodyn || (C->phys.flux_type == 0 && !C->phys.phi_linearize);
   }
 
+  // pass 1 on a register-resident kernel (2D degree 1-2, 3D degree 1)
   bool use_fast(const hdg_ctx *C, int thermodyn)
   {
     if (C->variant == 1 || !default_variant(C, thermodyn))
       return false;
-    return hdg::fast_supported(C->d, C->k, C->ne, thermodyn, C->en);
+    return hdg::fast_supported(C->d, C->k, C->ne, thermodyn, C->en) ||
+           hdg::fast3_supported(C->d, C->k, C->ne, thermodyn, C->en);
+  }
+
+  // pass 2 on the register-resident kernel (2D degree 1-2)
+  bool use_fast_pass2(const hdg_ctx *C, int thermodyn)
+  {
+    return use_fast(C, thermodyn) && C->d == 2;
+  }
+
+  hipError_t launch_pass1(const hdg_ctx *C, bool fast, const hdg::KParams &p, const int *cells,
+                          int ncell, int color, int mode, hipStream_t s)
+  {
+    if (!fast)
+      return hdg::launch_condense(C->d, C->k, p, cells, ncell, color, mode, s);
+    return C->d == 3 ? hdg::launch_fast3_condense(p, cells, ncell, color, mode, s)
+                     : hdg::launch_fast_condense(C->k, p, cells, ncell, color, mode, s);
   }
 
   hdg::KParams params(const hdg_ctx *C, int thermodyn)
@@ -315,7 +332,9 @@ int hdg_set_kernel_variant(hdg_ctx *C, int variant)
   if (variant < 0 || variant > 2)
     return fail("hdg_set_kernel_variant: variant must be 0, 1 or 2", -2);
   if (variant == 2 && !hdg::fast_supported(C->d, C->k, C->ne, 0, C->en) &&
-      !hdg::fast_supported(C->d, C->k, C->ne, 1, C->en))
+      !hdg::fast_supported(C->d, C->k, C->ne, 1, C->en) &&
+      !hdg::fast3_supported(C->d, C->k, C->ne, 0, C->en) &&
+      !hdg::fast3_supported(C->d, C->k, C->ne, 1, C->en))
     return fail("hdg_set_kernel_variant: no register-resident kernel for this shape", -2);
   if (variant == 2 && !default_variant(C, 1))
     return fail("hdg_set_kernel_variant: the register-resident kernel runs the default "
@@ -510,8 +529,7 @@ int hdg_condense(hdg_ctx *C, const double *x_cell, const double *x_trace, int th
   for (int col = 0; col < 2; ++col)
     {
       const int  nc = (int)C->color_cells[col].size();
-      hipError_t e  = fast ? hdg::launch_fast_condense(C->k, p, C->d_cells[col], nc, col, 0, s)
-                           : hdg::launch_condense(C->d, C->k, p, C->d_cells[col], nc, col, 0, s);
+      hipError_t e  = launch_pass1(C, fast, p, C->d_cells[col], nc, col, 0, s);
       if (e != hipSuccess)
         return hip_fail(e, "hdg_condense launch");
     }
@@ -530,7 +548,7 @@ int hdg_reconstruct(hdg_ctx *C, const double *x_cell, const double *x_trace,
   p.xt  = x_trace;
   p.dxt = dx_trace;
   p.dxc = dx_cell;
-  hipError_t e = use_fast(C, thermodyn)
+  hipError_t e = use_fast_pass2(C, thermodyn)
                    ? hdg::launch_fast_reconstruct(C->k, p, (int)C->ncells, (hipStream_t)stream)
                    : hdg::launch_reconstruct(C->d, C->k, p, (int)C->ncells, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_reconstruct launch");
@@ -548,9 +566,8 @@ int hdg_cell_systems(hdg_ctx *C, const double *x_cell, const double *x_trace, in
   p.xt   = x_trace;
   p.Sout = S;
   p.gout = g;
-  hipError_t e = use_fast(C, thermodyn)
-                   ? hdg::launch_fast_condense(C->k, p, cells, ncell, 0, 1, (hipStream_t)stream)
-                   : hdg::launch_condense(C->d, C->k, p, cells, ncell, 0, 1, (hipStream_t)stream);
+  hipError_t e = launch_pass1(C, use_fast(C, thermodyn), p, cells, ncell, 0, 1,
+                              (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_cell_systems launch");
 }
 

@@ -76,6 +76,11 @@ namespace hdg
   hipError_t launch_batch_condense(int ne, int thermo, int mixed, const KParams *plist,
                                    const int *chunk_dev, const int *items, int nchunks,
                                    int color, int grid, hipStream_t s);
+  // register-resident pass 1 for 3D degree 1 (hdg_fast3d.hip): two columns
+  // per lane, all three components or the V-only equilibrium operator
+  bool       fast3_supported(int dim, int deg, int ne, int thermo, const int en[3]);
+  hipError_t launch_fast3_condense(const KParams &p, const int *cells, int ncell, int color,
+                                   int mode, hipStream_t s);
   // register-resident pass 2 (same shapes as launch_fast_condense)
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s);
 } // namespace hdg

@@ -20,7 +20,7 @@ def _torch():
     return torch
 
 
-@pytest.mark.parametrize("dim,n_cells,k", [(2, 4, 1), (2, 4, 2)])
+@pytest.mark.parametrize("dim,n_cells,k", [(2, 4, 1), (2, 4, 2), (3, 3, 1)])
 @pytest.mark.parametrize("thermodyn", [False, True])
 def test_mixed_cell_systems_error(dim, n_cells, k, thermodyn):
     t = _torch()

@@ -258,3 +258,42 @@ def test_iv_points_match_oracle():
         # diffusion fluxes (dimensionless): it agrees to their rounding
         assert np.allclose(I_gpu, I_orc, rtol=1e-8, atol=1e-13 * s.current_scale())
         assert abs(I_gpu[0] + I_gpu[1]) < 1e-5 * abs(I_gpu[1]) + 1e-14 * s.current_scale()
+
+
+@pytest.mark.parametrize("n_cells", [(3, 2, 4), (5, 4, 3)])
+@pytest.mark.parametrize("thermodyn", [False, True])
+def test_3d_register_kernel_matches_lds_kernel(n_cells, thermodyn):
+    """3D degree 1: the register kernel (two columns per lane) against the
+    LDS kernel and the oracle on the same state, per-cell blocks and the
+    global system"""
+    t = _torch()
+    print(f"case 3D register kernel n={n_cells} thermo={thermodyn}", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(3, n_cells, 1)
+    s = NPSolver(prob, params, adim)
+    enabled = (True, False, False) if thermodyn else (True, True, True)
+    s.set_enabled_components(*enabled)
+    o = H.oracle_for(prob, params, adim, enabled)
+    cell, tr = _state(prob, adim, 1, seed=19)
+    s.set_state(cell, tr)
+    o.set_state(cell, tr)
+    ctx = s.ctx
+    cells = t.arange(ctx.ncells, dtype=t.int32, device="cuda")
+    out = {}
+    for variant in (2, 1):
+        ctx.set_kernel_variant(variant)
+        S = t.empty(ctx.ncells * ctx.T * ctx.T, dtype=t.float64, device="cuda")
+        g = t.empty(ctx.ncells * ctx.T, dtype=t.float64, device="cuda")
+        ctx.cell_systems(s.x_cell, s.x_trace, thermodyn, cells, S, g)
+        A, b, dirs, dv = s.assemble_system(thermodyn)
+        t.cuda.synchronize()
+        out[variant] = (S.cpu().numpy().reshape(ctx.ncells, ctx.T, ctx.T),
+                        g.cpu().numpy().reshape(ctx.ncells, ctx.T), A, b)
+    worst = 0.0
+    for c in range(ctx.ncells):
+        So, go, _, _, _ = o.condensed_cell(c, thermodyn)
+        worst = max(worst, H.rel_fro(out[2][0][c], So), H.rel_fro(out[2][1][c], go))
+    print(f"  register kernel vs oracle: worst rel {worst:.3e}")
+    assert worst < 1e-12
+    assert H.rel_fro(out[2][2].toarray(), out[1][2].toarray()) < 1e-12
+    assert H.rel_fro(out[2][3], out[1][3]) < 1e-12
