@@ -38,7 +38,9 @@ namespace hdg
 {
   namespace
   {
-    constexpr int WPB3 = 4; // waves per workgroup
+    // waves per workgroup: four, three with the larger FP32+refine scratch
+    // (two workgroups per CU either way)
+    __host__ __device__ constexpr int wpb3(bool mixed) { return mixed ? 3 : 4; }
 
     typedef const __attribute__((address_space(4))) double ctab3;
 
@@ -103,17 +105,16 @@ namespace hdg
       static constexpr int wdbl(bool mixed) { return mixed ? WDBL_MIXED : WDBL_FP64; }
       static constexpr size_t bytes(bool mixed)
       {
-        return (size_t)(TAB + WPB3 * wdbl(mixed)) * sizeof(double) +
-               (size_t)WPB3 * WINT * sizeof(int);
+        return (size_t)(TAB + wpb3(mixed) * wdbl(mixed)) * sizeof(double) +
+               (size_t)wpb3(mixed) * WINT * sizeof(int);
       }
     };
 
-    template <int NE>
+    template <int NE, int NT>
     __device__ __forceinline__ void stage_tables3(const DevTables &tb, double *tab0, int tid)
     {
       using S          = F3<NE>;
       constexpr int NB = S::NB, NF = S::NF, QC = S::QC, QF = S::QF, NFC = S::NFC, D = S::DIM;
-      constexpr int NT = 64 * WPB3;
       for (int i = tid; i < QC * NB; i += NT)
         {
           tab0[S::tPHIC + i] = tb.phic[i];
@@ -229,7 +230,7 @@ namespace hdg
       const int     lane = tid & 63, wv = tid >> 6;
       constexpr int WST = S::wdbl(MIXED);
       double       *ws  = smem + S::TAB + wv * WST;
-      int *wi = reinterpret_cast<int *>(smem + S::TAB + WPB3 * WST) + wv * S::WINT;
+      int *wi = reinterpret_cast<int *>(smem + S::TAB + wpb3(MIXED) * WST) + wv * S::WINT;
       long long *blk   = reinterpret_cast<long long *>(ws + S::wBLK);
       long long *facev = reinterpret_cast<long long *>(ws + S::wFACE);
       const DevTables &tb = p.tb;
@@ -692,15 +693,17 @@ namespace hdg
                   // FP32 elimination, the FP64 residual r = b - A' x0 (the
                   // scalar lanes keep their FP64 columns of A' in R), a
                   // second FP32 elimination with the same pivots
-                  if (isU)
-#pragma unroll
-                    for (int r = 0; r < NS; ++r)
-                      apr[lane * NS + r] = R[r];
                   float R32[NS], X0[NS];
 #pragma unroll
                   for (int r = 0; r < NS; ++r)
                     R32[r] = (float)R[r];
                   gauss_jordan_rec<NS>(R32, isU, lane, pivs, myk, gjr32);
+                  // the scalar lanes' FP64 columns of A', in pivot order,
+                  // for the slot-1 residual
+                  if (isU)
+#pragma unroll
+                    for (int r = 0; r < NS; ++r)
+                      apr[myk * NS + r] = R[r];
 #pragma unroll
                   for (int r = 0; r < NS; ++r)
                     X0[r] = R32[r];
@@ -709,11 +712,11 @@ namespace hdg
 #pragma unroll
                   for (int k = 0; k < NS; ++k)
                     {
-                      const double *ak = apr + pivs[k] * NS;
-                      const double  xk = mres * (double)X0[k];
+                      const int    pl = __builtin_amdgcn_readfirstlane(pivs[k]);
+                      const double xk = mres * (double)X0[k];
 #pragma unroll
                       for (int r = 0; r < NS; ++r)
-                        R[r] -= ak[r] * xk;
+                        R[r] -= lane_value(R[r], pl) * xk;
                     }
 #pragma unroll
                   for (int r = 0; r < NS; ++r)
@@ -742,11 +745,13 @@ namespace hdg
 #pragma unroll
                   for (int k = 0; k < NS; ++k)
                     {
-                      const double *ak = apr + pivs[k] * NS;
+                      const double *ak = apr + k * NS;
                       const double  xk = (double)X0[k];
 #pragma unroll
                       for (int r = 0; r < NS; ++r)
                         R[r] -= ak[r] * xk;
+                      // pace the NS*NS LDS loads of A'
+                      asm volatile("" ::: "memory");
                     }
 #pragma unroll
                   for (int r = 0; r < NS; ++r)
@@ -854,14 +859,15 @@ namespace hdg
     }
 
     template <int NE, bool THERMO, bool MIXED>
-    __global__ void __launch_bounds__(64 * WPB3) __attribute__((amdgpu_waves_per_eu(2)))
+    __global__ void __launch_bounds__(64 * wpb3(MIXED)) __attribute__((amdgpu_waves_per_eu(2)))
       fast_condense3(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
     {
+      constexpr int WPB = wpb3(MIXED);
       extern __shared__ double smem[];
-      stage_tables3<NE>(p.tb, smem, threadIdx.x);
+      stage_tables3<NE, 64 * WPB>(p.tb, smem, threadIdx.x);
       __syncthreads();
       const int wv = threadIdx.x >> 6;
-      for (int idx = blockIdx.x * WPB3 + wv; idx < ncell; idx += gridDim.x * WPB3)
+      for (int idx = blockIdx.x * WPB + wv; idx < ncell; idx += gridDim.x * WPB)
         cell_body3<NE, THERMO, MIXED>(p, smem, (long long)cells[idx], idx, color, mode);
     }
 
@@ -885,9 +891,10 @@ namespace hdg
       int dev = 0, ncu = 256;
       if (hipGetDevice(&dev) == hipSuccess)
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-      const int want = (ncell + WPB3 - 1) / WPB3;
-      const int grid = want < 4 * ncu ? want : 4 * ncu;
-      hipLaunchKernelGGL((fast_condense3<NE, THERMO, MIXED>), dim3(grid), dim3(64 * WPB3), bytes,
+      constexpr int WPB  = wpb3(MIXED);
+      const int     want = (ncell + WPB - 1) / WPB;
+      const int     grid = want < 4 * ncu ? want : 4 * ncu;
+      hipLaunchKernelGGL((fast_condense3<NE, THERMO, MIXED>), dim3(grid), dim3(64 * WPB), bytes,
                          s, p, cells, ncell, color, mode);
       return hipGetLastError();
     }
