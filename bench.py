@@ -142,6 +142,60 @@ def batched_sample(args, torch, stream):
     return out
 
 
+def sample_3d(args, torch, stream):
+    """the 3D workload: a 1 um cube diode, 48^3 hexahedra of degree 1, a
+    Gaussian acceptor profile (peak 1e24 m^-3, sigma 0.1 um, centred in the
+    cube) over uniform donors 1e22 m^-3; pass 1 in FP64 and in FP32 +
+    refinement, the error of the latter against FP64 on a sample of cells"""
+    from ddhdg_amd import configs
+    from ddhdg_amd.np_solver import NPSolver
+
+    n = args.cells3d
+    prob, params, adim = configs.diode(3, n, 1, doping=configs.gaussian_doping())
+    s = NPSolver(prob, params, adim, device="cuda")
+    s.compute_local_charge_neutrality()
+    s.project_densities_from_potential()
+    ctx = s.ctx
+    vals = torch.empty(ctx.nnz, dtype=torch.float64, device="cuda")
+    rhs = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
+    dirv = torch.zeros(ctx.ntrace, dtype=torch.float64, device="cuda")
+    reps = max(3, min(args.steps, 10))
+    out = {"workload": f"3D diode {n}^3 hexahedra, degree 1, V/n/p (96 cell dofs, 72 trace dofs), "
+                       "Gaussian N_A peak 1e24 m^-3 sigma 0.1 um centred, N_D 1e22 m^-3; pass 1",
+           "cells": int(ctx.ncells)}
+    T = ctx.T
+    sample = torch.arange(0, ctx.ncells, max(1, ctx.ncells // 4096), dtype=torch.int32,
+                          device="cuda")
+    blocks = {}
+    for prec in (0, 1):
+        ctx.set_precision(prec)
+        for _ in range(2):
+            ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        S_ = torch.empty(sample.numel() * T * T, dtype=torch.float64, device="cuda")
+        g_ = torch.empty(sample.numel() * T, dtype=torch.float64, device="cuda")
+        ctx.cell_systems(s.x_cell, s.x_trace, False, sample, S_, g_)
+        blocks[prec] = (S_.view(-1, T * T), g_.view(-1, T))
+        key = "fp64" if prec == 0 else "fp32_refine"
+        out[key] = {"value": ctx.ncells / (ms * 1e-3), "unit": "cells/s", "ms_per_pass": ms}
+    ctx.set_precision(0)
+    torch.cuda.synchronize()
+    (Sd, gd), (Sm, gm) = blocks[0], blocks[1]
+    out["fp32_refine"]["max_rel_err_S_vs_fp64"] = float(((Sm - Sd).norm(dim=1) /
+                                                         Sd.norm(dim=1)).max())
+    out["fp32_refine"]["max_rel_err_g_vs_fp64"] = float(((gm - gd).norm(dim=1) /
+                                                         gd.norm(dim=1).clamp_min(1e-300)).max())
+    out["fp32_refine"]["vs_fp64"] = out["fp32_refine"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +208,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=3,
                     help="whole Newton steps timed beside the metric")
+    ap.add_argument("--cells3d", type=int, default=48,
+                    help="cells per axis of the 3D sample (0: none)")
     ap.add_argument("--batch", type=int, default=64,
                     help="devices condensed together beside the metric (0: none)")
     ap.add_argument("--serve", type=int, default=16,
@@ -344,6 +400,8 @@ def main():
             "trace_solver": {"levels": ts.levels, "max_front": ts.max_front,
                              "front_GB": ts.bytes / 1e9, "factor_GFLOP": ts.flops / 1e9,
                              "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12}}
+    if args.cells3d > 0:
+        line["sample_3d"] = sample_3d(args, torch, stream)
     if args.batch > 0:
         line["batched"] = batched_sample(args, torch, stream)
     if args.serve > 0:

@@ -297,3 +297,33 @@ def test_3d_register_kernel_matches_lds_kernel(n_cells, thermodyn):
     assert worst < 1e-12
     assert H.rel_fro(out[2][2].toarray(), out[1][2].toarray()) < 1e-12
     assert H.rel_fro(out[2][3], out[1][3]) < 1e-12
+
+
+def test_newton_3d_register_kernel_matches_oracle():
+    """the 3D diode workflow on the register-resident 3D kernel (two columns
+    per lane): equal Newton iteration counts and states at every stage"""
+    _torch()
+    print("case newton 3D n=(8,2,2) k=1 doping 1e22", flush=True)
+    s, prob, params, adim = _solver(3, (8, 2, 2), 1, doping=configs.step_doping(1e22, 1e22))
+    o = H.oracle_for(prob, params, adim)
+    s.set_enabled_components(True, False, False)
+    s.compute_local_charge_neutrality()
+    o.set_state(*s.get_state())
+    r_gpu = s._private_run(1e-10, 1e-10, 100, True)
+    r_orc = _oracle_stage(o, (True, False, False), True)
+    assert r_gpu.converged and r_orc.converged and r_gpu.iterations == r_orc.iterations
+    assert _close(s, o)
+    s.project_densities_from_potential()
+    for name, en, bias in [("n,p", (False, True, True), None), ("coupled", (True, True, True), None),
+                           ("0.1 V", (True, True, True), 0.1)]:
+        s.set_enabled_components(*en)
+        if bias is not None:
+            configs.set_bias(s, bias)
+        o = H.oracle_for(prob, params, adim)
+        o.set_state(*s.get_state())
+        r_gpu = s.run()
+        r_orc = _oracle_stage(o, en)
+        print(f"  {name}: iterations gpu={r_gpu.iterations} oracle={r_orc.iterations}")
+        assert r_gpu.converged and r_orc.converged, name
+        assert r_gpu.iterations == r_orc.iterations, name
+        assert _close(s, o), name
