@@ -93,7 +93,9 @@ namespace hdg
       static constexpr int wBLK  = wBC + 64;      // 4 long long (blkptr of the faces)
       static constexpr int wFACE = wBLK + NFC;    // 4 long long (face index)
       static constexpr int wTCL  = wFACE + NFC;   // NS*T: TC' column by column (pass 1)
-      static constexpr int WDBL  = wTCL + NS * T; // per-wave doubles, pass 1
+      static constexpr int wXCH  = wTCL + NS * T; // X rows / Schur rows to and from the MFMA layouts
+      static constexpr int XCH   = (TDF > 4 ? TDF : 4) * (T + 1);
+      static constexpr int WDBL  = wXCH + XCH;    // per-wave doubles, pass 1
       static constexpr int WDBL2 = wTCL;          // pass 2 needs no TC'
       // per-wave ints
       static constexpr int iDIR  = 0;             // NFC*3
@@ -782,21 +784,75 @@ namespace hdg
           WAVE_SYNC();
           return;
         }
-      // ---- 4: [S | g] = [TT' | t'] - TC' X, face by face
+      // ---- 4: [S | g] = [TT' | t'] - TC' X, face by face.  The product
+      // runs on the matrix cores (v_mfma_f64_16x16x4f64, K = 4 pivot rows
+      // per step).  X, NS rows (pivot order) by NC trace and rhs columns,
+      // reaches the B layout through LDS once, four rows at a time, and
+      // stays in registers; per face, TC' (TDF rows padded to 16, columns
+      // in pivot order) is gathered from LDS in the A layout, and the TDF x
+      // NC result returns to column layout through the same buffer.
+      constexpr int NC = T + 1, NKK = (NS + 3) / 4, NCT = (NC + 15) / 16;
+      double       *xch = ws + S::wXCH;
+      const int     q16 = lane >> 4, c16 = lane & 15;
+      const bool    isC = lane >= NS && lane < NS + NC;
+      double        bop[NKK][NCT];
+      int           pka[NKK];
+#pragma unroll
+      for (int kk = 0; kk < NKK; ++kk)
+        {
+          const int k = 4 * kk + q16;
+          pka[kk]     = pivs[k < NS ? k : 0];
+          if (isC)
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq)
+              if (4 * kk + kq < NS)
+                xch[kq * NC + lane - NS] = R[4 * kk + kq];
+          WAVE_SYNC();
+#pragma unroll
+          for (int tj = 0; tj < NCT; ++tj)
+            {
+              const int    c  = 16 * tj + c16;
+              const bool   ok = k < NS && c < NC;
+              const double v  = xch[q16 * NC + (ok ? c : 0)];
+              bop[kk][tj]     = ok ? v : 0.0;
+            }
+          WAVE_SYNC();
+        }
 #pragma unroll 1
       for (int g = 0; g < NFC; ++g)
         {
           double acc[TDF];
           trace_rows(g, acc);
+          dbl4 dm[NCT];
 #pragma unroll
-          for (int k = 0; k < NS; ++k)
+          for (int tj = 0; tj < NCT; ++tj)
+            dm[tj] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int kk = 0; kk < NKK; ++kk)
             {
-              const double *tc = tcl + pivs[k] * T + g * TDF;
-              const double  x  = R[k];
+              const bool   ok = 4 * kk + q16 < NS && c16 < TDF;
+              const double v  = tcl[pka[kk] * T + g * TDF + (ok ? c16 : 0)];
+              const double a  = ok ? v : 0.0;
 #pragma unroll
-              for (int r = 0; r < TDF; ++r)
-                acc[r] -= tc[r] * x;
+              for (int tj = 0; tj < NCT; ++tj)
+                dm[tj] = mfma_f64_16x16x4(a, bop[kk][tj], dm[tj]);
             }
+          // element e of dm[tj] holds row 4e + q16, column 16 tj + c16
+#pragma unroll
+          for (int tj = 0; tj < NCT; ++tj)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              {
+                const int r = 4 * e + q16, c = 16 * tj + c16;
+                if (4 * e < TDF && r < TDF && c < NC)
+                  xch[r * NC + c] = dm[tj][e];
+              }
+          WAVE_SYNC();
+          if (isC)
+#pragma unroll
+            for (int r = 0; r < TDF; ++r)
+              acc[r] -= xch[r * NC + lane - NS];
+          WAVE_SYNC();
           if (mode == 1)
             {
               if (isL)

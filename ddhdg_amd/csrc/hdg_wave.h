@@ -1,5 +1,5 @@
-// hdg_wave.h -- wave64 helpers for gfx950: lane broadcasts by readlane and
-// wave-wide max reductions by DPP (no LDS round trips).
+// hdg_wave.h -- wave64 helpers for gfx950: lane broadcasts by readlane,
+// wave-wide max reductions by DPP (no LDS round trips), the FP64 MFMA.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -37,6 +37,16 @@ namespace hdg
     v = dpp_max_step<0x142, 0xa>(v); // row_bcast:15
     v = dpp_max_step<0x143, 0xc>(v); // row_bcast:31
     return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+  }
+
+  typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+  // v_mfma_f64_16x16x4f64, D = A B + C (64 lanes active): A[i][k] in lane
+  // i + 16k, B[k][j] in lane j + 16k; element e of the result holds
+  // D[4e + lane/16][lane%16]
+  __device__ __forceinline__ dbl4 mfma_f64_16x16x4(double a, double b, dbl4 c)
+  {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
 
   // leading 32 bits of a non-negative double: they order as unsigned
