@@ -15,9 +15,9 @@
 //      trace/rhs slot holds X(p_k, m), X = A'^-1 [CT' | c']
 //   3. face by face: every slot forms its trace-test rows (TC' for the
 //      scalar columns, TT'/t' for the others); the scalar lanes publish
-//      their TC' rows of the face in LDS, the trace/rhs slots subtract
-//      TC' X and write [S | g] straight into the trace CSR (two colour
-//      launches, deterministic)
+//      their TC' rows of the face in LDS, the product TC' X runs on the
+//      FP64 matrix cores, the trace/rhs slots subtract it and write [S | g]
+//      straight into the trace CSR (two colour launches, deterministic)
 // TC' is recomputed face by face instead of being kept for the whole cell:
 // that keeps the per-wave LDS at 11 KB and two workgroups of four waves per
 // CU.  Optional FP32 elimination with one FP64 refinement step (MIXED).
@@ -88,8 +88,9 @@ namespace hdg
       static constexpr int wNEU  = wRESF + NFC * QF * NE;
       static constexpr int wBLK  = wNEU + NFC * NE * NF; // NFC long long
       static constexpr int wFACE = wBLK + NFC;           // NFC long long
-      static constexpr int wTCF  = wFACE + NFC;          // NS*TDF: TC' rows of one face
-      static constexpr int wGJR  = wTCF + NS * TDF;      // NS*NS: pivot columns (FP64 mode)
+      static constexpr int wTCF  = wFACE + NFC;          // TC' rows of one face; the MFMA exchange
+      static constexpr int TCFN  = NS * TDF > 4 * 64 ? NS * TDF : 4 * 64;
+      static constexpr int wGJR  = wTCF + TCFN;          // NS*NS: pivot columns (FP64 mode)
       static constexpr int wAPR  = wGJR;                 // NS*NS: A' columns (MIXED)
       static constexpr int wGJ32 = wAPR + NS * NS;       // NS*NS floats: pivot columns (MIXED)
       static constexpr int WDBL_FP64  = wGJR + NS * NS;
@@ -582,6 +583,11 @@ namespace hdg
           }
       };
 
+      constexpr int NKK = NS / 4;
+      static_assert(NS % 4 == 0, "K steps of four pivot rows");
+      static_assert(NS / 16 * 16 + 48 >= (NCOL < 64 ? NCOL : 64) && NCOL - 64 <= 48,
+                    "three 16-lane column tiles cover the trace and rhs columns of a slot");
+      const int q16 = lane >> 4, c16 = lane & 15;
       int myk = 0; // the equation this scalar lane was the pivot of
       // the slots one after the other: only one slot's rows are live
 #pragma unroll
@@ -764,7 +770,29 @@ namespace hdg
             }
           WAVE_SYNC3();
 
-          // ---- 3: [S | g] = [TT' | t'] - TC' X of this slot, face by face
+          // ---- 3: [S | g] = [TT' | t'] - TC' X of this slot, face by face.
+          // The product runs on the matrix cores (v_mfma_f64_16x16x4f64, K =
+          // 4 pivot rows per step) over the slot's three 16-lane column tiles
+          // from tile tb.  X reaches the B layout through tcf once per slot,
+          // four rows at a time, and stays in registers; per face, TC' (TDF
+          // rows padded to 16) is gathered from tcf in the A layout, and the
+          // result returns to column layout through tcf, four rows at a time.
+          const int tb = h == 0 ? NS / 16 : 0;
+          double    bop[NKK][3];
+#pragma unroll
+          for (int kk = 0; kk < NKK; ++kk)
+            {
+#pragma unroll
+              for (int kq = 0; kq < 4; ++kq)
+                tcf[kq * 64 + lane] = R[4 * kk + kq];
+              WAVE_SYNC3();
+#pragma unroll
+              for (int t = 0; t < 3; ++t)
+                bop[kk][t] = tcf[q16 * 64 + 16 * (tb + t) + c16];
+              WAVE_SYNC3();
+            }
+          const bool isC = kind[h] == 1 || kind[h] == 2;
+          const int  cl  = lane - 16 * tb;
 #pragma unroll 1
           for (int g = 0; g < NFC; ++g)
             {
@@ -791,13 +819,33 @@ namespace hdg
                   trace_rows(h, g, acc);
                 }
               WAVE_SYNC3();
+              dbl4 dm[3];
 #pragma unroll
-              for (int k = 0; k < NS; ++k)
+              for (int t = 0; t < 3; ++t)
+                dm[t] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+              for (int kk = 0; kk < NKK; ++kk)
                 {
-                  const double x = R[k];
+                  const double v = tcf[(4 * kk + q16) * TDF + (c16 < TDF ? c16 : 0)];
+                  const double a = c16 < TDF ? v : 0.0;
 #pragma unroll
-                  for (int r = 0; r < TDF; ++r)
-                    acc[r] -= tcf[k * TDF + r] * x;
+                  for (int t = 0; t < 3; ++t)
+                    dm[t] = mfma_f64_16x16x4(a, bop[kk][t], dm[t]);
+                }
+              // element e of dm[t] holds row 4e + q16, column 16 (tb + t) + c16
+#pragma unroll
+              for (int e = 0; e < (TDF + 3) / 4; ++e)
+                {
+                  WAVE_SYNC3();
+#pragma unroll
+                  for (int t = 0; t < 3; ++t)
+                    tcf[q16 * 48 + 16 * t + c16] = dm[t][e];
+                  WAVE_SYNC3();
+                  if (isC)
+#pragma unroll
+                    for (int qq = 0; qq < 4; ++qq)
+                      if (4 * e + qq < TDF)
+                        acc[4 * e + qq] -= tcf[qq * 48 + cl];
                 }
               // the next face overwrites tcf
               WAVE_SYNC3();
