@@ -5,13 +5,18 @@
 // and solves once per Newton step (src/np_solver.cpp:1326-1354).  Here the
 // plan of hdg_nd_plan.h (tree, fronts, assembly and extend-add maps) is built
 // once per mesh and component set; each factorisation then runs level by
-// level: one strided-batched rocSOLVER getrf of every front's F11 (partial
-// pivoting inside the eliminated block), its row interchanges applied to F12
-// in HIP, two rocBLAS trsm (U12 = L11^-1 P F12, L21 = F21 U11^-1), one gemm
-// (F22 -= L21 U12), and an extend-add of the
-// update blocks into the parents written in HIP, one launch per child slot so
+// level.  A front's eliminated block is taken in panels of at most 72 dofs
+// (whole faces): per panel, the LU of its diagonal block with partial
+// pivoting inside the panel (one workgroup per front, in LDS), the row
+// interchanges applied to the panel's rows right of it, the two batched
+// triangular solves (U12 = L11^-1 P F12, L21 = F21 U11^-1) and one rocBLAS
+// strided-batched gemm (F22 -= L21 U12) on the rest of the front.  That is
+// the elimination of a chain of nodes of at most 72 dofs each, with
+// pivoting inside each node as in any multifrontal code.  An extend-add of
+// the update blocks into the parents follows, one launch per child slot so
 // that no two writes meet (deterministic sums, no atomics).  The fronts stay
-// in HBM as the factors for the forward and backward sweeps.
+// in HBM as the factors for the forward and backward sweeps, which run the
+// same panels.
 #include <algorithm>
 #include <cstdint>
 #include <string>
@@ -19,7 +24,6 @@
 #include <vector>
 
 #include <rocblas/rocblas.h>
-#include <rocsolver/rocsolver.h>
 
 #include "hdg_ctx.h"
 #include "hdg_nd_plan.h"
@@ -92,201 +96,170 @@ namespace
       }
   }
 
-  // getrf's row interchanges applied to F12 (columns S..n-1) of every front
-  // of a group; one thread per column, the swaps in order
-  __global__ void k_laswp_cols(double *F, long long fbase, int n, int S, int I,
-                               const int *__restrict__ ipiv, long long pbase, int batch, int yoff)
-  {
-    const int b = blockIdx.y + yoff;
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= batch || c >= I)
-      return;
-    double    *col = F + fbase + (long long)b * n * n + (long long)(S + c) * n;
-    const int *pv  = ipiv + pbase + (long long)b * S;
-    for (int j = 0; j < S; ++j)
-      {
-        const int p = pv[j] - 1;
-        if (p != j)
-          {
-            const double t = col[j];
-            col[j]         = col[p];
-            col[p]         = t;
-          }
-      }
-  }
-
-  // Batched triangular solves for fronts with a small eliminated block
-  // (S <= TRSM_SMAX), one launch per level.  A lane owns one row (right solve)
-  // or one column (left solve) of the off-diagonal block; the values it has
-  // solved stay in LDS.  The triangular factor is read at wave-uniform
-  // addresses through the constant address space (scalar loads into
-  // SGPRs): this kernel writes only the off-diagonal block, never the
-  // factor it reads.
-  constexpr int TRSM_SMAX = 160;
-  typedef const __attribute__((address_space(4))) double cdouble;
-
-  // B := B U^-1, B = F21 (I x S, rows S.. of columns 0..S-1), U = upper(F11)
-  __global__ void __launch_bounds__(64)
-    k_trsm_right_upper(double *F, long long fbase, int n, int S, int I, int batch, int yoff)
-  {
-    extern __shared__ double xs[]; // S x 64
-    const int b = blockIdx.y + yoff;
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    if (b >= batch)
-      return;
-    cdouble   *U   = (cdouble *)(F + fbase + (long long)b * n * n);
-    double    *B   = F + fbase + (long long)b * n * n + S;
-    const bool act = i < I;
-    for (int j = 0; j < S; ++j)
-      {
-        cdouble *Uj = U + (long long)j * n;
-        // four partial sums: the FMA chain is the latency bound
-        double a0 = act ? B[i + (long long)j * n] : 0.0, a1 = 0, a2 = 0, a3 = 0;
-        int    l  = 0;
-        for (; l + 3 < j; l += 4)
-          {
-            a0 -= xs[l * 64 + threadIdx.x] * Uj[l];
-            a1 -= xs[(l + 1) * 64 + threadIdx.x] * Uj[l + 1];
-            a2 -= xs[(l + 2) * 64 + threadIdx.x] * Uj[l + 2];
-            a3 -= xs[(l + 3) * 64 + threadIdx.x] * Uj[l + 3];
-          }
-        for (; l < j; ++l)
-          a0 -= xs[l * 64 + threadIdx.x] * Uj[l];
-        const double acc            = (a0 + a1) + (a2 + a3);
-        const double x              = acc / Uj[j];
-        xs[j * 64 + threadIdx.x]    = x;
-        if (act)
-          B[i + (long long)j * n] = x;
-      }
-  }
-
-  // B := L^-1 B, B = F12 (S x I, rows 0..S-1 of columns S..), L = unit lower(F11)
-  __global__ void __launch_bounds__(64)
-    k_trsm_left_lower_unit(double *F, long long fbase, int n, int S, int I, int batch, int yoff)
-  {
-    extern __shared__ double ys[]; // S x 64
-    const int b = blockIdx.y + yoff;
-    const int c = blockIdx.x * 64 + threadIdx.x;
-    if (b >= batch)
-      return;
-    cdouble      *L   = (cdouble *)(F + fbase + (long long)b * n * n);
-    double       *B   = F + fbase + (long long)b * n * n + (long long)S * n;
-    const bool    act = c < I;
-    double       *Bc  = B + (long long)(act ? c : 0) * n;
-    for (int j = 0; j < S; ++j)
-      {
-        double a0 = act ? Bc[j] : 0.0, a1 = 0, a2 = 0, a3 = 0;
-        int    l  = 0;
-        for (; l + 3 < j; l += 4)
-          {
-            a0 -= L[j + (long long)l * n] * ys[l * 64 + threadIdx.x];
-            a1 -= L[j + (long long)(l + 1) * n] * ys[(l + 1) * 64 + threadIdx.x];
-            a2 -= L[j + (long long)(l + 2) * n] * ys[(l + 2) * 64 + threadIdx.x];
-            a3 -= L[j + (long long)(l + 3) * n] * ys[(l + 3) * 64 + threadIdx.x];
-          }
-        for (; l < j; ++l)
-          a0 -= L[j + (long long)l * n] * ys[l * 64 + threadIdx.x];
-        const double acc         = (a0 + a1) + (a2 + a3);
-        ys[j * 64 + threadIdx.x] = acc;
-        if (act)
-          Bc[j] = acc;
-      }
-  }
-
   // ---------------------------------------------------------------------
-  // Fronts with a small eliminated block (S <= SMALL_S): F11 = P L11 U11
-  // (partial pivoting) in one workgroup per front, F11 in LDS, ipiv in
-  // LAPACK's 1-based form.  The off-diagonal blocks and the Schur update
-  // follow as for the other fronts.
-  constexpr int SMALL_S = 72;
-  constexpr int FT      = 256; // threads of the small-front kernels
+  // One panel of every front of a group: its S <= PW_MAX eliminated dofs
+  // are the leading block of the trailing front at the panel's corner
+  // (address fbase + b*n*n, leading dimension n).
+  constexpr int PW_MAX = 64;
 
-  __global__ void __launch_bounds__(FT)
-    k_front_lu_small(double *F, long long fbase, int n, int S, int I, int *ipiv, int *perm,
-                     long long pbase, int *info, int batch, int yoff)
+  // pivot-search key of a row lane: the leading bits of |a| (they order as
+  // the values, to about 5 significant digits) with 63 - lane in the low 6
+  // bits (ties go to the lowest row); 0 for a row that is not a candidate
+  __device__ __forceinline__ unsigned row_key(double a, bool cand, int lane)
   {
-    __shared__ double A[SMALL_S * SMALL_S]; // F11, column-major, ld S
-    __shared__ int    piv[SMALL_S], pm[SMALL_S];
-    __shared__ int    pk;
+    return cand ? (leading_bits(fabs(a)) & ~63u) + (unsigned)(63 - lane) + 1u : 0u;
+  }
+
+  // F11 = P L11 U11 of the panel with partial pivoting inside it, one wave
+  // per front: lane i holds row i of the block in registers; the pivot of
+  // column k is the unpivoted row with the largest |a_ik| (DPP max), its
+  // row reaches every lane by readlane and is not moved.  Row k of the
+  // result is [L(k, 0..k-1) | U(k, k..)] of the k-th pivot row, written in
+  // pivot order; perm[k] = that row's index in the panel (front b's at
+  // pbase + b*ps).  An exactly zero pivot is reported as column o + k + 1
+  // (the first one of the front is kept) and its multipliers are 0.
+  __global__ void __launch_bounds__(64)
+    k_panel_lu(double *F, long long fbase, int n, int S, int *perm, long long pbase, int ps,
+               int o, int *info, int batch, int yoff)
+  {
     const int b = blockIdx.x + yoff;
     if (b >= batch)
       return;
-    const int t  = threadIdx.x;
+    const int i  = threadIdx.x;
     double   *Fb = F + fbase + (long long)b * n * n;
-    for (int e = t; e < S * S; e += FT)
-      A[e] = Fb[(e % S) + (long long)(e / S) * n];
-    for (int k = t; k < S; k += FT)
-      pm[k] = k;
-    __syncthreads();
-    int bad = 0;
-    for (int k = 0; k < S; ++k)
+    double    A[PW_MAX];
+#pragma unroll
+    for (int j = 0; j < PW_MAX; ++j)
+      A[j] = (j < S && i < S) ? Fb[i + (long long)j * n] : 0.0;
+    bool pivoted = i >= S;
+    int  myk = 0, bad = 0;
+#pragma unroll
+    for (int k = 0; k < PW_MAX; ++k)
+      if (k < S)
       {
-        // pivot: largest |A(i,k)|, i >= k, to the leading bits of the
-        // value (lowest row on ties), found by wave 0 with a DPP max
-        if (t < 64)
-          {
-            unsigned key = 0;
-            for (int i = k + t; i < S; i += 64)
-              {
-                const unsigned c = (leading_bits(fabs(A[i + k * S])) & ~127u) | (unsigned)(127 - i);
-                key              = c > key ? c : key;
-              }
-            key = wave_max(key);
-            if (t == 0)
-              pk = 127 - (int)(key & 127u);
-          }
-        __syncthreads();
-        const int p = pk;
-        if (t == 0)
-          {
-            piv[k] = p;
-            if (p != k)
-              {
-                const int tmp = pm[k];
-                pm[k]         = pm[p];
-                pm[p]         = tmp;
-              }
-          }
-        if (p != k)
-          for (int j = t; j < S; j += FT)
-            {
-              const double tmp = A[k + j * S];
-              A[k + j * S]     = A[p + j * S];
-              A[p + j * S]     = tmp;
-            }
-        __syncthreads();
-        const double d = A[k + k * S];
-        if (t == 0 && d == 0.0 && !bad)
+        const unsigned key = wave_max(row_key(A[k], !pivoted, i));
+        const int      p   = 63 - (int)((key - 1u) & 63u);
+        const double   d   = lane_value(A[k], p);
+        if (d == 0.0 && bad == 0)
           bad = k + 1;
-        for (int i = k + 1 + t; i < S; i += FT)
-          A[i + k * S] = d != 0.0 ? A[i + k * S] / d : 0.0;
-        __syncthreads();
-        const int m = S - k - 1;
-        for (int e = t; e < m * m; e += FT)
+        const bool   upd = !pivoted && i != p;
+        const double l   = (upd && d != 0.0) ? A[k] / d : 0.0;
+        if (upd)
+          A[k] = l;
+        if (i == p)
           {
-            const int i = k + 1 + e % m, j = k + 1 + e / m;
-            A[i + j * S] -= A[i + k * S] * A[k + j * S];
+            pivoted = true;
+            myk     = k;
           }
-        __syncthreads();
+#pragma unroll
+        for (int j = k + 1; j < PW_MAX; ++j)
+          if (j < S)
+            A[j] -= l * lane_value(A[j], p);
       }
-    if (t == 0)
-      info[b] = bad;
-    for (int e = t; e < S * S; e += FT)
-      Fb[(e % S) + (long long)(e / S) * n] = A[e];
-    for (int k = t; k < S; k += FT)
+    if (i < S)
       {
-        ipiv[pbase + (long long)b * S + k] = piv[k] + 1;
-        perm[pbase + (long long)b * S + k] = pm[k];
+#pragma unroll
+        for (int j = 0; j < PW_MAX; ++j)
+          if (j < S)
+            Fb[myk + (long long)j * n] = A[j];
+        perm[pbase + (long long)b * ps + myk] = i;
+      }
+    if (i == 0 && bad && info[b] == 0)
+      info[b] = o + bad;
+  }
+
+  // The off-diagonal blocks of one panel, one launch: blocks x < nbl (one
+  // lane per column of F12) form U12 = L11^-1 P F12 (the column's rows
+  // gathered in pivot order), blocks x >= nbl (one lane per row of F21)
+  // form L21 = F21 U11^-1.  The factor is staged in LDS and read at
+  // wave-uniform addresses; the lane's column or row stays in registers.
+  __global__ void __launch_bounds__(64)
+    k_panel_trsm(double *F, long long fbase, int n, int S, int I, const int *__restrict__ perm,
+                 long long pbase, int ps, int nbl, int batch, int yoff)
+  {
+    __shared__ double Ts[PW_MAX * PW_MAX]; // left: L row-major; right: U column-major
+    __shared__ int    pm[PW_MAX];
+    const int b = blockIdx.y + yoff;
+    if (b >= batch)
+      return;
+    const bool left = (int)blockIdx.x < nbl;
+    const int  t    = threadIdx.x;
+    const int  r    = (left ? (int)blockIdx.x : (int)blockIdx.x - nbl) * 64 + t;
+    double    *Fb   = F + fbase + (long long)b * n * n;
+    for (int e = t; e < S * S; e += 64)
+      {
+        const int    ii = e % S, jj = e / S;
+        const double v  = Fb[ii + (long long)jj * n];
+        if (left)
+          Ts[ii * PW_MAX + jj] = v;
+        else
+          Ts[ii + jj * PW_MAX] = v;
+      }
+    if (left && t < S)
+      pm[t] = perm[pbase + (long long)b * ps + t];
+    __syncthreads();
+    const bool act = r < I;
+    double     y[PW_MAX];
+    if (left)
+      {
+        double *col = Fb + (long long)(S + (act ? r : 0)) * n;
+#pragma unroll
+        for (int j = 0; j < PW_MAX; ++j)
+          y[j] = (j < S && act) ? col[pm[j < S ? j : 0]] : 0.0;
+#pragma unroll
+        for (int j = 1; j < PW_MAX; ++j)
+          if (j < S)
+          {
+            double a0 = y[j], a1 = 0.0;
+#pragma unroll
+            for (int l = 0; l < j; ++l)
+              if (l & 1)
+                a1 -= Ts[j * PW_MAX + l] * y[l];
+              else
+                a0 -= Ts[j * PW_MAX + l] * y[l];
+            y[j] = a0 + a1;
+          }
+        if (act)
+#pragma unroll
+          for (int j = 0; j < PW_MAX; ++j)
+            if (j < S)
+              col[j] = y[j];
+      }
+    else
+      {
+        double *row = Fb + S + (act ? r : 0);
+#pragma unroll
+        for (int j = 0; j < PW_MAX; ++j)
+          y[j] = (j < S && act) ? row[(long long)j * n] : 0.0;
+#pragma unroll
+        for (int j = 0; j < PW_MAX; ++j)
+          if (j < S)
+          {
+            double a0 = y[j], a1 = 0.0;
+#pragma unroll
+            for (int l = 0; l < j; ++l)
+              if (l & 1)
+                a1 -= y[l] * Ts[l + j * PW_MAX];
+              else
+                a0 -= y[l] * Ts[l + j * PW_MAX];
+            y[j] = (a0 + a1) / Ts[j + j * PW_MAX];
+          }
+        if (act)
+#pragma unroll
+          for (int j = 0; j < PW_MAX; ++j)
+            if (j < S)
+              row[(long long)j * n] = y[j];
       }
   }
 
-  // forward sweep of one small front, one wave: z = L11^-1 P w_s with L11
-  // staged in LDS and z in registers (rows lane and lane + 64; each
+  // forward sweep of one panel of a front, one wave: z = L11^-1 P w_s with
+  // L11 staged in LDS and z in registers (rows lane and lane + 64; each
   // solved entry reaches the other lanes by readlane), then w_i -= L21 z.
-  // perm: row k of the factored block is row perm[k] of the front.
+  // perm: row k of the factored block is row perm[k] of the panel; front
+  // b's at pbase + b*ps.
   __global__ void __launch_bounds__(64)
     k_fwd_small(const double *F, double *w, long long fbase, long long wbase, int n, int S,
-                int I, const int *perm, long long pbase, int batch, int yoff)
+                int I, const int *perm, long long pbase, int ps, int batch, int yoff)
   {
     extern __shared__ double L[]; // S x S, column-major
     const int b = blockIdx.x + yoff;
@@ -295,7 +268,7 @@ namespace
     const int     t  = threadIdx.x;
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
-    const int    *pv = perm + pbase + (long long)b * S;
+    const int    *pv = perm + pbase + (long long)b * ps;
     for (int e = t; e < S * S; e += 64)
       L[e] = Fb[(e % S) + (long long)(e / S) * n];
     const int i0 = t, i1 = t + 64;
@@ -338,8 +311,8 @@ namespace
       }
   }
 
-  // backward sweep of one small front, one wave: z = w_s - U12 x_i, then
-  // U11^-1 z with U11 staged in LDS and z in registers
+  // backward sweep of one panel of a front, one wave: z = w_s - U12 x_i,
+  // then U11^-1 z with U11 staged in LDS and z in registers
   __global__ void __launch_bounds__(64)
     k_bwd_small(const double *F, double *w, long long fbase, long long wbase, int n, int S,
                 int I, int batch, int yoff)
@@ -402,27 +375,6 @@ namespace
       w[wmap[i]] = b[i];
   }
 
-  // row interchanges of getrf applied to the eliminated part of each work vector
-  __global__ void k_laswp(double *w, long long wbase, int n, int S, const int *__restrict__ ipiv,
-                          long long pbase, int batch)
-  {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= batch)
-      return;
-    double    *x  = w + wbase + (long long)b * n;
-    const int *pv = ipiv + pbase + (long long)b * S;
-    for (int j = 0; j < S; ++j)
-      {
-        const int p = pv[j] - 1;
-        if (p != j)
-          {
-            const double t = x[j];
-            x[j]           = x[p];
-            x[p]           = t;
-          }
-      }
-  }
-
   __global__ void k_vec_extend_add(double *w, long long wbase, int n, int S, int I, int batch,
                                    const long long *__restrict__ pw,
                                    const long long *__restrict__ pf, const int *__restrict__ slot,
@@ -466,6 +418,9 @@ namespace
   }
 
   constexpr int YCHUNK = 32768; // nodes per launch along grid y
+  // a sweep's off-diagonal update stays in the one-wave panel kernel up to
+  // this many trailing rows, and goes to a batched gemv above
+  constexpr int FUSE_I = 256;
 
   int grid_for(long long work)
   {
@@ -494,7 +449,8 @@ struct hdg_trace_solver
   std::vector<DevGroup> dg;
   rocblas_handle        h      = nullptr;
   double               *d_F    = nullptr, *d_w = nullptr;
-  int                  *d_ipiv = nullptr, *d_info = nullptr;
+  int                  *d_info = nullptr;
+  int                   pw     = PW_MAX; // panel width: whole faces, at most PW_MAX dofs
   int                  *d_perm = nullptr; // row order of the small fronts' factored blocks
   long long            *d_map = nullptr, *d_pad = nullptr, *d_wmap = nullptr;
 };
@@ -547,8 +503,6 @@ int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
       if (e == hipSuccess)
         e = hipMalloc((void **)&T->d_w, (size_t)std::max<long long>(1, P.wtot) * sizeof(double));
       if (e == hipSuccess)
-        e = hipMalloc((void **)&T->d_ipiv, (size_t)std::max<long long>(1, P.ptot) * sizeof(int));
-      if (e == hipSuccess)
         e = hipMalloc((void **)&T->d_perm, (size_t)std::max<long long>(1, P.ptot) * sizeof(int));
       if (e == hipSuccess)
         e = hipMalloc((void **)&T->d_info, (size_t)P.nnodes * sizeof(int));
@@ -558,16 +512,9 @@ int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
                             -3);
     }
   if (rc == 0)
-    {
-      const int  lds = TRSM_SMAX * 64 * (int)sizeof(double);
-      hipError_t e   = hipFuncSetAttribute((const void *)k_trsm_right_upper,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-      if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void *)k_trsm_left_lower_unit,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-      if (e != hipSuccess)
-        rc = hdg::set_error(std::string("hdg_trace_solver_create: ") + hipGetErrorString(e), -3);
-    }
+    T->pw = std::max(1, PW_MAX / std::max(1, P.tdf)) * std::max(1, P.tdf);
+  if (rc == 0 && T->pw > PW_MAX)
+    rc = hdg::set_error("hdg_trace_solver_create: more than 64 dofs per face", -4);
   if (rc == 0 && rocblas_create_handle(&T->h) != rocblas_status_success)
     rc = hdg::set_error("hdg_trace_solver_create: rocblas_create_handle failed", -3);
   if (rc != 0)
@@ -588,11 +535,11 @@ void hdg_trace_solver_destroy(hdg_trace_solver *T)
                     (void *)D.d_pos, (void *)D.d_edof, (void *)D.d_idof})
       if (p)
         (void)hipFree(p);
-  for (void *p : {(void *)T->d_F, (void *)T->d_w, (void *)T->d_ipiv, (void *)T->d_perm,
+  for (void *p : {(void *)T->d_F, (void *)T->d_w, (void *)T->d_perm,
                   (void *)T->d_info,
                   (void *)T->d_map, (void *)T->d_pad, (void *)T->d_wmap})
     if (p)
-      hipFree(p);
+      (void)hipFree(p);
   if (T->h)
     rocblas_destroy_handle(T->h);
   delete T;
@@ -631,66 +578,33 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
     {
       const hdg::NDGroup &G = T->P.groups[gi];
       const DevGroup     &D = T->dg[gi];
-      double             *F = T->d_F + G.fbase;
-      if (G.S > 0)
+      // the eliminated block in panels of at most pw dofs (whole faces):
+      // panel [o, o + Sp) of each front is the leading block of the
+      // trailing front at (o, o), of dimension n - o
+      for (int o = 0; o < G.S; o += T->pw)
         {
-          // F11 = P L11 U11 (partial pivoting inside the eliminated block)
-          rocblas_status st = rocblas_status_success;
-          if (G.S <= SMALL_S)
-            for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-              hipLaunchKernelGGL(k_front_lu_small, dim3(std::min(YCHUNK, G.batch - y0)),
-                                 dim3(FT), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, T->d_ipiv,
-                                 T->d_perm, G.pbase, T->d_info + info_off, G.batch, y0);
-          else
-            st = rocsolver_dgetrf_strided_batched(
-              T->h, G.S, G.S, F, G.n, (rocblas_stride)G.n * G.n, T->d_ipiv + G.pbase,
-              (rocblas_stride)G.S, T->d_info + info_off, G.batch);
+          const int       Sp = std::min(T->pw, G.S - o);
+          const int       Ip = G.n - o - Sp;
+          const long long fb = G.fbase + (long long)o * G.n + o;
+          double         *Fp = T->d_F + fb;
+          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+            hipLaunchKernelGGL(k_panel_lu, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64), 0, s,
+                               T->d_F, fb, G.n, Sp, T->d_perm, G.pbase + o, G.S, o,
+                               T->d_info + info_off, G.batch, y0);
+          if (Ip == 0)
+            continue;
+          // U12 = L11^-1 P F12, L21 = F21 U11^-1, F22 -= L21 U12
+          const int nbl = (Ip + 63) / 64;
+          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+            hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nbl, std::min(YCHUNK, G.batch - y0)),
+                               dim3(64), 0, s, T->d_F, fb, G.n, Sp, Ip, T->d_perm, G.pbase + o,
+                               G.S, nbl, G.batch, y0);
+          const rocblas_status st = rocblas_dgemm_strided_batched(
+            T->h, rocblas_operation_none, rocblas_operation_none, Ip, Ip, Sp, &mone, Fp + Sp, G.n,
+            (rocblas_stride)G.n * G.n, Fp + (long long)Sp * G.n, G.n, (rocblas_stride)G.n * G.n,
+            &one, Fp + Sp + (long long)Sp * G.n, G.n, (rocblas_stride)G.n * G.n, G.batch);
           if (st != rocblas_status_success)
-            return hdg::set_error("hdg_trace_factor: rocsolver getrf failed", -3);
-          if (G.I > 0)
-            {
-              // U12 = L11^-1 P F12, L21 = F21 U11^-1, F22 -= L21 U12
-              for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-                hipLaunchKernelGGL(k_laswp_cols,
-                                   dim3((G.I + 63) / 64, std::min(YCHUNK, G.batch - y0)),
-                                   dim3(64), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, T->d_ipiv,
-                                   G.pbase, G.batch, y0);
-              if (G.S <= TRSM_SMAX)
-                {
-                  // rocBLAS's strided-batched trsm above 64 loops over the
-                  // batch; these are one launch per level
-                  const size_t lds = (size_t)G.S * 64 * sizeof(double);
-                  for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-                    {
-                      const dim3 grid((G.I + 63) / 64, std::min(YCHUNK, G.batch - y0));
-                      hipLaunchKernelGGL(k_trsm_left_lower_unit, grid, dim3(64), lds, s, T->d_F,
-                                         G.fbase, G.n, G.S, G.I, G.batch, y0);
-                      hipLaunchKernelGGL(k_trsm_right_upper, grid, dim3(64), lds, s, T->d_F,
-                                         G.fbase, G.n, G.S, G.I, G.batch, y0);
-                    }
-                }
-              else
-                {
-                  st = rocblas_dtrsm_strided_batched(
-                    T->h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                    rocblas_diagonal_unit, G.S, G.I, &one, F, G.n, (rocblas_stride)G.n * G.n,
-                    F + (long long)G.S * G.n, G.n, (rocblas_stride)G.n * G.n, G.batch);
-                  if (st == rocblas_status_success)
-                    st = rocblas_dtrsm_strided_batched(
-                      T->h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                      rocblas_diagonal_non_unit, G.I, G.S, &one, F, G.n,
-                      (rocblas_stride)G.n * G.n, F + G.S, G.n, (rocblas_stride)G.n * G.n,
-                      G.batch);
-                }
-              if (st == rocblas_status_success)
-                st = rocblas_dgemm_strided_batched(
-                  T->h, rocblas_operation_none, rocblas_operation_none, G.I, G.I, G.S, &mone,
-                  F + G.S, G.n, (rocblas_stride)G.n * G.n, F + (long long)G.S * G.n, G.n,
-                  (rocblas_stride)G.n * G.n, &one, F + G.S + (long long)G.S * G.n, G.n,
-                  (rocblas_stride)G.n * G.n, G.batch);
-              if (st != rocblas_status_success)
-                return hdg::set_error("hdg_trace_factor: rocblas trsm/gemm failed", -3);
-            }
+            return hdg::set_error("hdg_trace_factor: rocblas gemm failed", -3);
         }
       info_off += G.batch;
       if (G.I > 0)
@@ -745,29 +659,28 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
     {
       const hdg::NDGroup &G = T->P.groups[gi];
       const DevGroup     &D = T->dg[gi];
-      const double       *F = T->d_F + G.fbase;
-      double       *w = T->d_w + G.wbase;
-      if (G.S > 0 && G.S <= SMALL_S)
+      // the panels in order: z_p = L_p^-1 P_p w_p, then the rest of the
+      // trailing front's vector -= L21_p z_p (in the same one-wave kernel
+      // while that rest is short, else by a batched rocBLAS gemv)
+      for (int o = 0; o < G.S; o += T->pw)
         {
+          const int       Sp   = std::min(T->pw, G.S - o), Ip = G.n - o - Sp;
+          const bool      fuse = Ip <= FUSE_I;
+          const long long fb   = G.fbase + (long long)o * G.n + o;
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
             hipLaunchKernelGGL(k_fwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
-                               (size_t)G.S * G.S * sizeof(double), s, T->d_F, T->d_w, G.fbase,
-                               G.wbase, G.n, G.S, G.I, T->d_perm, G.pbase, G.batch, y0);
-        }
-      else if (G.S > 0)
-        {
-          hipLaunchKernelGGL(k_laswp, dim3((G.batch + 63) / 64), dim3(64), 0, s, T->d_w,
-                             G.wbase, G.n, G.S, T->d_ipiv, G.pbase, G.batch);
-          rocblas_status st = rocblas_dtrsv_strided_batched(
-            T->h, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_unit, G.S, F, G.n,
-            (rocblas_stride)G.n * G.n, w, 1, (rocblas_stride)G.n, G.batch);
-          if (st == rocblas_status_success && G.I > 0)
-            st = rocblas_dgemv_strided_batched(T->h, rocblas_operation_none, G.I, G.S, &mone,
-                                               F + G.S, G.n, (rocblas_stride)G.n * G.n, w, 1,
-                                               (rocblas_stride)G.n, &one, w + G.S, 1,
-                                               (rocblas_stride)G.n, G.batch);
-          if (st != rocblas_status_success)
-            return hdg::set_error("hdg_trace_solve: rocblas forward sweep failed", -3);
+                               (size_t)Sp * Sp * sizeof(double), s, T->d_F, T->d_w, fb,
+                               G.wbase + o, G.n, Sp, fuse ? Ip : 0, T->d_perm, G.pbase + o, G.S,
+                               G.batch, y0);
+          if (!fuse)
+            {
+              double *wp = T->d_w + G.wbase + o;
+              if (rocblas_dgemv_strided_batched(
+                    T->h, rocblas_operation_none, Ip, Sp, &mone, T->d_F + fb + Sp, G.n,
+                    (rocblas_stride)G.n * G.n, wp, 1, (rocblas_stride)G.n, &one, wp + Sp, 1,
+                    (rocblas_stride)G.n, G.batch) != rocblas_status_success)
+                return hdg::set_error("hdg_trace_solve: rocblas gemv failed", -3);
+            }
         }
       if (G.I > 0)
         for (int want = 0; want < 2; ++want)
@@ -782,42 +695,38 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
     {
       const hdg::NDGroup &G = T->P.groups[gi];
       const DevGroup     &D = T->dg[gi];
-      const double *F = T->d_F + G.fbase;
-      double       *w = T->d_w + G.wbase;
       if (G.I > 0)
         for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
           hipLaunchKernelGGL(k_gather_x,
                              dim3((G.I + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),
                              dim3(TPB), 0, s, x, T->d_w, G.wbase, G.n, G.S, G.I, G.batch,
                              D.d_idof, y0);
-      if (G.S > 0 && G.S <= SMALL_S)
+      if (G.S > 0)
         {
-          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(k_bwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
-                               (size_t)G.S * G.S * sizeof(double), s, T->d_F, T->d_w, G.fbase,
-                               G.wbase, G.n, G.S, G.I, G.batch, y0);
-          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(k_scatter_x,
-                               dim3((G.S + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),
-                               dim3(TPB), 0, s, x, T->d_w, G.wbase, G.n, G.S, G.batch, D.d_edof,
-                               y0);
-        }
-      else if (G.S > 0)
-        {
-          rocblas_status st = rocblas_status_success;
-          if (G.I > 0)
-            st = rocblas_dgemv_strided_batched(T->h, rocblas_operation_none, G.S, G.I, &mone,
-                                               F + (long long)G.S * G.n, G.n,
-                                               (rocblas_stride)G.n * G.n, w + G.S, 1,
-                                               (rocblas_stride)G.n, &one, w, 1,
-                                               (rocblas_stride)G.n, G.batch);
-          if (st == rocblas_status_success)
-            st = rocblas_dtrsv_strided_batched(T->h, rocblas_fill_upper, rocblas_operation_none,
-                                               rocblas_diagonal_non_unit, G.S, F, G.n,
-                                               (rocblas_stride)G.n * G.n, w, 1,
-                                               (rocblas_stride)G.n, G.batch);
-          if (st != rocblas_status_success)
-            return hdg::set_error("hdg_trace_solve: rocblas backward sweep failed", -3);
+          // the panels in reverse order: x_p = U_p^-1 (z_p - U12_p x_rest)
+          // (U12_p x_rest in the same kernel while the rest is short)
+          const int np = (G.S + T->pw - 1) / T->pw;
+          for (int pi = np - 1; pi >= 0; --pi)
+            {
+              const int       o    = pi * T->pw;
+              const int       Sp   = std::min(T->pw, G.S - o), Ip = G.n - o - Sp;
+              const bool      fuse = Ip <= FUSE_I;
+              const long long fb   = G.fbase + (long long)o * G.n + o;
+              if (!fuse)
+                {
+                  double *wp = T->d_w + G.wbase + o;
+                  if (rocblas_dgemv_strided_batched(
+                        T->h, rocblas_operation_none, Sp, Ip, &mone,
+                        T->d_F + fb + (long long)Sp * G.n, G.n, (rocblas_stride)G.n * G.n,
+                        wp + Sp, 1, (rocblas_stride)G.n, &one, wp, 1, (rocblas_stride)G.n,
+                        G.batch) != rocblas_status_success)
+                    return hdg::set_error("hdg_trace_solve: rocblas gemv failed", -3);
+                }
+              for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+                hipLaunchKernelGGL(k_bwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
+                                   (size_t)Sp * Sp * sizeof(double), s, T->d_F, T->d_w, fb,
+                                   G.wbase + o, G.n, Sp, fuse ? Ip : 0, G.batch, y0);
+            }
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
             hipLaunchKernelGGL(k_scatter_x,
                                dim3((G.S + TPB - 1) / TPB, std::min(YCHUNK, G.batch - y0)),
