@@ -85,19 +85,20 @@ namespace
     return thermodyn || (C->phys.flux_type == 0 && !C->phys.phi_linearize);
   }
 
-  // pass 1 on a register-resident kernel (2D degree 1-2, 3D degree 1)
+  // pass 1 on a register-resident kernel (2D degree 1-3, 3D degree 1)
   bool use_fast(const hdg_ctx *C, int thermodyn)
   {
     if (C->variant == 1 || !default_variant(C, thermodyn))
       return false;
     return hdg::fast_supported(C->d, C->k, C->ne, thermodyn, C->en) ||
+           hdg::fast2p_supported(C->d, C->k, C->ne, thermodyn, C->en) ||
            hdg::fast3_supported(C->d, C->k, C->ne, thermodyn, C->en);
   }
 
   // pass 2 on the register-resident kernel (2D degree 1-2)
   bool use_fast_pass2(const hdg_ctx *C, int thermodyn)
   {
-    return use_fast(C, thermodyn) && C->d == 2;
+    return use_fast(C, thermodyn) && C->d == 2 && C->k <= 2;
   }
 
   hipError_t launch_pass1(const hdg_ctx *C, bool fast, const hdg::KParams &p, const int *cells,
@@ -105,8 +106,11 @@ namespace
   {
     if (!fast)
       return hdg::launch_condense(C->d, C->k, p, cells, ncell, color, mode, s);
-    return C->d == 3 ? hdg::launch_fast3_condense(p, cells, ncell, color, mode, s)
-                     : hdg::launch_fast_condense(C->k, p, cells, ncell, color, mode, s);
+    if (C->d == 3)
+      return hdg::launch_fast3_condense(p, cells, ncell, color, mode, s);
+    if (C->k == 3)
+      return hdg::launch_fast2p_condense(p, cells, ncell, color, mode, s);
+    return hdg::launch_fast_condense(C->k, p, cells, ncell, color, mode, s);
   }
 
   hdg::KParams params(const hdg_ctx *C, int thermodyn)
@@ -333,6 +337,7 @@ int hdg_set_kernel_variant(hdg_ctx *C, int variant)
     return fail("hdg_set_kernel_variant: variant must be 0, 1 or 2", -2);
   if (variant == 2 && !hdg::fast_supported(C->d, C->k, C->ne, 0, C->en) &&
       !hdg::fast_supported(C->d, C->k, C->ne, 1, C->en) &&
+      !hdg::fast2p_supported(C->d, C->k, C->ne, 0, C->en) &&
       !hdg::fast3_supported(C->d, C->k, C->ne, 0, C->en) &&
       !hdg::fast3_supported(C->d, C->k, C->ne, 1, C->en))
     return fail("hdg_set_kernel_variant: no register-resident kernel for this shape", -2);

@@ -81,6 +81,11 @@ namespace hdg
   bool       fast3_supported(int dim, int deg, int ne, int thermo, const int en[3]);
   hipError_t launch_fast3_condense(const KParams &p, const int *cells, int ncell, int color,
                                    int mode, hipStream_t s);
+  // register-resident pass 1 for 2D degree 3 with V/n/p (hdg_fast2dp.hip):
+  // a pair of waves per cell
+  bool       fast2p_supported(int dim, int deg, int ne, int thermo, const int en[3]);
+  hipError_t launch_fast2p_condense(const KParams &p, const int *cells, int ncell, int color,
+                                    int mode, hipStream_t s);
   // register-resident pass 2 (same shapes as launch_fast_condense)
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s);
 } // namespace hdg

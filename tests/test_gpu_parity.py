@@ -143,7 +143,7 @@ def _close(s, o, tol=1e-9):
             np.abs(tg - to).max() <= tol * np.abs(to).max())
 
 
-@pytest.mark.parametrize("dim,n_cells,k", [(2, 12, 1), (2, 8, 2)])
+@pytest.mark.parametrize("dim,n_cells,k", [(2, 12, 1), (2, 8, 2), (2, 6, 3)])
 def test_newton_diode_matches_oracle(dim, n_cells, k):
     """the reference's diode workflow (equilibrium, n and p with V frozen,
     coupled system, 0.1 V and 0.3 V forward bias): equal Newton iteration
@@ -297,6 +297,44 @@ def test_3d_register_kernel_matches_lds_kernel(n_cells, thermodyn):
     assert worst < 1e-12
     assert H.rel_fro(out[2][2].toarray(), out[1][2].toarray()) < 1e-12
     assert H.rel_fro(out[2][3], out[1][3]) < 1e-12
+
+
+@pytest.mark.parametrize("n_cells", [(5, 4), (7, 3), (1, 1)])
+def test_2d_k3_register_kernel_matches_lds_kernel(n_cells):
+    """2D degree 3 with V/n/p: the register kernel (a pair of waves per
+    cell) against the LDS kernel and the oracle on the same state, per-cell
+    blocks and the global system.  Odd cell counts per colour leave a pair
+    of the last workgroup without a cell of its own."""
+    t = _torch()
+    print(f"case 2D k=3 register kernel n={n_cells}", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(2, n_cells, 3)
+    s = NPSolver(prob, params, adim)
+    o = H.oracle_for(prob, params, adim)
+    cell, tr = _state(prob, adim, 3, seed=23)
+    s.set_state(cell, tr)
+    o.set_state(cell, tr)
+    ctx = s.ctx
+    cells = t.arange(ctx.ncells, dtype=t.int32, device="cuda")
+    out = {}
+    for variant in (2, 1):
+        ctx.set_kernel_variant(variant)
+        S = t.empty(ctx.ncells * ctx.T * ctx.T, dtype=t.float64, device="cuda")
+        g = t.empty(ctx.ncells * ctx.T, dtype=t.float64, device="cuda")
+        ctx.cell_systems(s.x_cell, s.x_trace, False, cells, S, g)
+        A, b, dirs, dv = s.assemble_system(False)
+        t.cuda.synchronize()
+        out[variant] = (S.cpu().numpy().reshape(ctx.ncells, ctx.T, ctx.T),
+                        g.cpu().numpy().reshape(ctx.ncells, ctx.T), A, b, dv)
+    worst = 0.0
+    for c in range(ctx.ncells):
+        So, go, _, _, _ = o.condensed_cell(c, False)
+        worst = max(worst, H.rel_fro(out[2][0][c], So), H.rel_fro(out[2][1][c], go))
+    print(f"  register kernel vs oracle: worst rel {worst:.3e}")
+    assert worst < 1e-12
+    assert H.rel_fro(out[2][2].toarray(), out[1][2].toarray()) < 1e-12
+    assert H.rel_fro(out[2][3], out[1][3]) < 1e-12
+    assert np.array_equal(out[2][4], out[1][4])
 
 
 def test_newton_3d_register_kernel_matches_oracle():
