@@ -598,6 +598,7 @@ struct hdg_batch
 {
   std::vector<hdg_ctx *> ctx;
   int                    ne = 0, thermo_ok = 0, full_ok = 0, wpb = 4, ncu = 256;
+  int                    deg3 = 0; // degree-3 meshes: the pair kernel, FP64
   int                   *d_chunk_dev[2] = {nullptr, nullptr}, *d_items[2] = {nullptr, nullptr};
   int                    nchunks[2] = {0, 0};
   long long              ncells = 0;
@@ -613,9 +614,12 @@ namespace
   {
     for (int col = 0; col < 2; ++col)
       {
-        hipError_t e = hdg::launch_batch_condense(B->ne, thermodyn, mixed, B->d_params,
-                                                  B->d_chunk_dev[col], B->d_items[col],
-                                                  B->nchunks[col], col, grid, s);
+        hipError_t e =
+          B->deg3 ? hdg::launch_batch_pair(B->d_params, B->d_chunk_dev[col], B->d_items[col],
+                                           B->nchunks[col], col, grid, s)
+                  : hdg::launch_batch_condense(B->ne, thermodyn, mixed, B->d_params,
+                                               B->d_chunk_dev[col], B->d_items[col],
+                                               B->nchunks[col], col, grid, s);
         if (e != hipSuccess)
           return hip_fail(e, "hdg_batch_condense launch");
       }
@@ -630,22 +634,24 @@ int hdg_batch_create(hdg_ctx *const *ctxs, int n, hdg_batch **out)
   if (!ctxs || n < 1 || !out)
     return fail("hdg_batch_create: invalid argument");
   hdg_batch *B = new hdg_batch();
-  B->ne        = ctxs[0]->ne;
+  B->ne        = ctxs[0] ? ctxs[0]->ne : 0;
+  B->deg3      = ctxs[0] && ctxs[0]->k == 3;
   for (int i = 0; i < n; ++i)
     {
-      hdg_ctx *C = ctxs[i];
-      if (!C || C->d != 2 || C->k < 1 || C->k > 2 || C->ne != B->ne ||
-          C->en[0] != ctxs[0]->en[0] || C->en[1] != ctxs[0]->en[1] ||
-          C->en[2] != ctxs[0]->en[2])
+      hdg_ctx   *C  = ctxs[i];
+      const bool ok = C && C->d == 2 && C->ne == B->ne && C->en[0] == ctxs[0]->en[0] &&
+                      C->en[1] == ctxs[0]->en[1] && C->en[2] == ctxs[0]->en[2] &&
+                      (B->deg3 ? (C->k == 3 && C->ne == 3) : (C->k >= 1 && C->k <= 2));
+      if (!ok)
         {
           delete B;
-          return fail("hdg_batch_create: contexts must be 2D, degree 1 or 2, with the same "
-                      "enabled components", -2);
+          return fail("hdg_batch_create: contexts must be 2D with the same enabled components, "
+                      "all of degree 1 or 2, or all of degree 3 with V, n and p", -2);
         }
       B->ctx.push_back(C);
       B->ncells += C->ncells;
     }
-  B->wpb = hdg::batch_waves_per_group();
+  B->wpb = B->deg3 ? hdg::batch_pairs_per_group() : hdg::batch_waves_per_group();
   int rc = 0;
   for (int col = 0; col < 2 && rc == 0; ++col)
     {
@@ -716,6 +722,8 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
     return fail("hdg_batch_condense: thermodynamic equilibrium runs with V only", -2);
   if (!thermodyn && B->ne != 3)
     return fail("hdg_batch_condense: the register kernel needs V, n and p or V alone", -2);
+  if (thermodyn && B->deg3)
+    return fail("hdg_batch_condense: degree-3 batches run the full operator only", -2);
   hipStream_t s = (hipStream_t)stream;
   // the pinned parameter blocks may still feed the previous call's copy
   hipError_t e = hipEventSynchronize(B->copied);

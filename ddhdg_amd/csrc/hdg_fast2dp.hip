@@ -800,7 +800,60 @@ namespace hdg
                       mode, live);
         }
     }
+    // pass 1 over the cells of several degree-3 meshes in one persistent
+    // launch per colour: the work list holds chunks of PPB cells of one
+    // device (padded with -1 at device boundaries; a chunk's first cell is
+    // real); a workgroup restages the tables when its chunk's device changes
+    __global__ void __launch_bounds__(64 * 2 * PPB) __attribute__((amdgpu_waves_per_eu(2)))
+      batch_condense_p(const KParams *__restrict__ plist, const int *__restrict__ chunk_dev,
+                       const int *__restrict__ items, int nchunks, int color)
+    {
+      extern __shared__ double smem[];
+      const int pr  = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 7);
+      int       cur = -1;
+      for (int c = blockIdx.x; c < nchunks; c += gridDim.x)
+        {
+          const int      dev = chunk_dev[c];
+          const KParams &p   = plist[dev];
+          if (dev != cur)
+            {
+              __syncthreads();
+              stage_tables_p(p.tb, smem, threadIdx.x);
+              __syncthreads();
+              cur = dev;
+            }
+          const int  cell = items[c * PPB + pr];
+          const bool live = cell >= 0;
+          const int  use  = live ? cell : items[c * PPB];
+          cell_body_p(p, smem, use, use, color, 0, live);
+        }
+    }
   } // namespace
+
+  int batch_pairs_per_group()
+  {
+    return PPB;
+  }
+
+  hipError_t launch_batch_pair(const KParams *plist, const int *chunk_dev, const int *items,
+                               int nchunks, int color, int grid, hipStream_t s)
+  {
+    constexpr size_t bytes = FP::bytes();
+    static bool      attr  = false;
+    if (!attr)
+      {
+        hipError_t e = hipFuncSetAttribute((const void *)batch_condense_p,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+        if (e != hipSuccess)
+          return e;
+        attr = true;
+      }
+    if (nchunks <= 0)
+      return hipSuccess;
+    hipLaunchKernelGGL(batch_condense_p, dim3(grid), dim3(64 * 2 * PPB), bytes, s, plist,
+                       chunk_dev, items, nchunks, color);
+    return hipGetLastError();
+  }
 
   bool fast2p_supported(int dim, int deg, int ne, int thermo, const int en[3])
   {

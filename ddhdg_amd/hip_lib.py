@@ -284,8 +284,9 @@ def trace_plan_selftest(dim, n_cells, dofs_per_face, seed=1, csr_vals=None):
 
 
 class HdgBatch:
-    """pass 1 of several contexts (2D, degree 1/2, same components) in one
-    persistent launch per colour; the grid is autotuned on the first call."""
+    """pass 1 of several contexts (2D, same components; degree 1/2, or all
+    of degree 3 with V/n/p) in one persistent launch per colour; the grid is
+    autotuned on the first call."""
 
     def __init__(self, contexts):
         self.L = lib()

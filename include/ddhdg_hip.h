@@ -217,9 +217,11 @@ int hdg_max_abs4(const double *const *vectors, const long long *lengths,
                  double *scratch, double *out, void *stream);
 
 /* ---- Batched pass 1 over several meshes --------------------------------
- * Pass 1 (hdg_condense) of several contexts (2D, degree 1 or 2 mixed, the
- * same enabled components) in one persistent launch per colour; each
- * context's outputs are exactly those of its own hdg_condense.  The
+ * Pass 1 (hdg_condense) of several contexts (2D with the same enabled
+ * components: degree 1 and 2 mixed, or all of degree 3 with V, n and p) in
+ * one persistent launch per colour; each context's outputs are exactly
+ * those of its own hdg_condense.  A degree-3 batch runs in FP64 and the
+ * full operator only.  The
  * persistent grid is picked by timing 1, 2 and 4 workgroups per CU on the
  * first call (hdg_batch_info reports the choice).  Pointer arguments are
  * host arrays of n device pointers, in the order of ctxs.  No counterpart in
@@ -240,7 +242,8 @@ int  hdg_batch_info(const hdg_batch *batch, double out[6]);
 /* ---- Global trace solve ------------------------------------------------
  * Sparse direct LU of the restricted trace system, on the GPU: a nested-
  * dissection multifrontal factorisation over the structured cell grid
- * (dense fronts in HBM, batched rocSOLVER/rocBLAS per tree level).  Replaces
+ * (dense fronts in HBM, eliminated per tree level in panels of whole faces
+ * by batched HIP kernels and rocBLAS gemm).  Replaces
  * the reference's SparseDirectUMFPACK factorize + solve of system_matrix
  * (src/np_solver.cpp:1326-1354).  A solver is built for the context's mesh
  * and its enabled components at creation; rebuild it after

@@ -10,6 +10,8 @@ from tests import helpers as H
 pytestmark = pytest.mark.gpu
 
 MESHES = [((12, 10), 1), ((9, 9), 2), ((16, 5), 2), ((7, 13), 1)]
+# degree 3 (the pair kernel): odd cell counts per colour pad the chunks
+MESHES3 = [((7, 5), 3), ((4, 9), 3), ((1, 1), 3), ((10, 6), 3)]
 
 
 def _torch():
@@ -18,10 +20,10 @@ def _torch():
     return torch
 
 
-def _solvers(thermodyn, precision=0):
+def _solvers(thermodyn, precision=0, meshes=MESHES):
     from ddhdg_amd.np_solver import NPSolver
     out = []
-    for i, (n, k) in enumerate(MESHES):
+    for i, (n, k) in enumerate(meshes):
         prob, params, adim = configs.diode(2, n, k)
         s = NPSolver(prob, params, adim)
         if thermodyn:
@@ -60,3 +62,41 @@ def test_batch_equals_per_mesh(thermodyn, precision):
         c.condense(s.x_cell, s.x_trace, thermodyn, v1, r1, d1)
         t.cuda.synchronize()
         assert t.equal(v, v1) and t.equal(r, r1) and t.equal(d, d1)
+
+
+def test_degree3_batch_equals_per_mesh():
+    """degree-3 meshes in one persistent launch per colour of the pair kernel:
+    every mesh's outputs bitwise equal to its own hdg_condense"""
+    t = _torch()
+    print(f"case degree-3 batch of {len(MESHES3)} meshes", flush=True)
+    from ddhdg_amd.hip_lib import HdgBatch
+    sv = _solvers(False, 0, MESHES3)
+    bufs = []
+    for s in sv:
+        c = s.ctx
+        bufs.append((t.full((c.nnz,), 7.0, dtype=t.float64, device="cuda"),
+                     t.full((c.ntrace,), 7.0, dtype=t.float64, device="cuda"),
+                     t.zeros(c.ntrace, dtype=t.float64, device="cuda")))
+    b = HdgBatch([s.ctx for s in sv])
+    for _ in range(2):   # the first call also autotunes the grid
+        b.condense(False, [s.x_cell for s in sv], [s.x_trace for s in sv],
+                   [x[0] for x in bufs], [x[1] for x in bufs], [x[2] for x in bufs])
+    t.cuda.synchronize()
+    print(f"  {b.info()}")
+    for s, (v, r, d) in zip(sv, bufs):
+        c = s.ctx
+        v1 = t.empty_like(v)
+        r1 = t.empty_like(r)
+        d1 = t.zeros_like(d)
+        c.condense(s.x_cell, s.x_trace, False, v1, r1, d1)
+        t.cuda.synchronize()
+        assert t.equal(v, v1) and t.equal(r, r1) and t.equal(d, d1)
+
+
+def test_batch_rejects_mixed_degree3():
+    """a batch holds degree-1/2 meshes or degree-3 meshes, not both"""
+    _torch()
+    from ddhdg_amd.hip_lib import HdgBatch, HdgError
+    sv = _solvers(False, 0, [((4, 4), 2), ((4, 4), 3)])
+    with pytest.raises(HdgError, match="degree 3"):
+        HdgBatch([s.ctx for s in sv])
