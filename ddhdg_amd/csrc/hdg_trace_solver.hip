@@ -102,6 +102,28 @@ namespace
   // (address fbase + b*n*n, leading dimension n).
   constexpr int PW_MAX = 64;
 
+  // the leading S x S block at Fb (leading dimension n, S <= 64) into LDS,
+  // one wave: lane t loads row t of eight columns at a time (eight loads in
+  // flight, not one), dst(ii, jj) = out[ii * rs + jj * cs]
+  __device__ __forceinline__ void stage_block(const double *Fb, long long n, int S, double *out,
+                                              int rs, int cs, int t)
+  {
+    const int ti = t < S ? t : 0;
+#pragma unroll 1
+    for (int j0 = 0; j0 < S; j0 += 8)
+      {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          v[u] = Fb[ti + (long long)(j0 + u < S ? j0 + u : 0) * n];
+        if (t < S)
+#pragma unroll
+          for (int u = 0; u < 8; ++u)
+            if (j0 + u < S)
+              out[t * rs + (j0 + u) * cs] = v[u];
+      }
+  }
+
   // pivot-search key of a row lane: the leading bits of |a| (they order as
   // the values, to about 5 significant digits) with 63 - lane in the low 6
   // bits (ties go to the lowest row); 0 for a row that is not a candidate
@@ -125,12 +147,19 @@ namespace
     const int b = blockIdx.x + yoff;
     if (b >= batch)
       return;
+    extern __shared__ double blk[]; // S x S, column-major
     const int i  = threadIdx.x;
     double   *Fb = F + fbase + (long long)b * n * n;
-    double    A[PW_MAX];
+    // the block goes through LDS: loads from HBM in flight eight at a time
+    // (row by row into registers they were issued one after the other)
+    stage_block(Fb, n, S, blk, 1, S, i);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double A[PW_MAX];
 #pragma unroll
     for (int j = 0; j < PW_MAX; ++j)
-      A[j] = (j < S && i < S) ? Fb[i + (long long)j * n] : 0.0;
+      A[j] = (j < S && i < S) ? blk[i + j * S] : 0.0;
     bool pivoted = i >= S;
     int  myk = 0, bad = 0;
 #pragma unroll
@@ -186,15 +215,10 @@ namespace
     const int  t    = threadIdx.x;
     const int  r    = (left ? (int)blockIdx.x : (int)blockIdx.x - nbl) * 64 + t;
     double    *Fb   = F + fbase + (long long)b * n * n;
-    for (int e = t; e < S * S; e += 64)
-      {
-        const int    ii = e % S, jj = e / S;
-        const double v  = Fb[ii + (long long)jj * n];
-        if (left)
-          Ts[ii * PW_MAX + jj] = v;
-        else
-          Ts[ii + jj * PW_MAX] = v;
-      }
+    if (left)
+      stage_block(Fb, n, S, Ts, PW_MAX, 1, t);
+    else
+      stage_block(Fb, n, S, Ts, 1, PW_MAX, t);
     if (left && t < S)
       pm[t] = perm[pbase + (long long)b * ps + t];
     __syncthreads();
@@ -205,7 +229,10 @@ namespace
         double *col = Fb + (long long)(S + (act ? r : 0)) * n;
 #pragma unroll
         for (int j = 0; j < PW_MAX; ++j)
-          y[j] = (j < S && act) ? col[pm[j < S ? j : 0]] : 0.0;
+          {
+            const double v = col[pm[j < S ? j : 0]];
+            y[j]           = v * ((j < S && act) ? 1.0 : 0.0);
+          }
 #pragma unroll
         for (int j = 1; j < PW_MAX; ++j)
           if (j < S)
@@ -230,7 +257,10 @@ namespace
         double *row = Fb + S + (act ? r : 0);
 #pragma unroll
         for (int j = 0; j < PW_MAX; ++j)
-          y[j] = (j < S && act) ? row[(long long)j * n] : 0.0;
+          {
+            const double v = row[(long long)(j < S ? j : 0) * n];
+            y[j]           = v * ((j < S && act) ? 1.0 : 0.0);
+          }
 #pragma unroll
         for (int j = 0; j < PW_MAX; ++j)
           if (j < S)
@@ -269,8 +299,7 @@ namespace
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
     const int    *pv = perm + pbase + (long long)b * ps;
-    for (int e = t; e < S * S; e += 64)
-      L[e] = Fb[(e % S) + (long long)(e / S) * n];
+    stage_block(Fb, n, S, L, 1, S, t);
     const int i0 = t, i1 = t + 64;
     double    z0 = i0 < S ? wb[pv[i0]] : 0.0;
     double    z1 = i1 < S ? wb[pv[i1]] : 0.0;
@@ -324,8 +353,7 @@ namespace
     const int     t  = threadIdx.x;
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
-    for (int e = t; e < S * S; e += 64)
-      U[e] = Fb[(e % S) + (long long)(e / S) * n];
+    stage_block(Fb, n, S, U, 1, S, t);
     const int i0 = t, i1 = t + 64;
     double    z0 = 0.0, z1 = 0.0;
     for (int h = 0; h < 2; ++h)
@@ -588,9 +616,10 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
           const long long fb = G.fbase + (long long)o * G.n + o;
           double         *Fp = T->d_F + fb;
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(k_panel_lu, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64), 0, s,
-                               T->d_F, fb, G.n, Sp, T->d_perm, G.pbase + o, G.S, o,
-                               T->d_info + info_off, G.batch, y0);
+            hipLaunchKernelGGL(k_panel_lu, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
+                               (size_t)Sp * Sp * sizeof(double), s, T->d_F, fb, G.n, Sp,
+                               T->d_perm, G.pbase + o, G.S, o, T->d_info + info_off, G.batch,
+                               y0);
           if (Ip == 0)
             continue;
           // U12 = L11^-1 P F12, L21 = F21 U11^-1, F22 -= L21 U12
