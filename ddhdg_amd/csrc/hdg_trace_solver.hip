@@ -139,7 +139,10 @@ namespace
   // result is [L(k, 0..k-1) | U(k, k..)] of the k-th pivot row, written in
   // pivot order; perm[k] = that row's index in the panel (front b's at
   // pbase + b*ps).  An exactly zero pivot is reported as column o + k + 1
-  // (the first one of the front is kept) and its multipliers are 0.
+  // (the first one of the front is kept) and its multipliers are 0.  SM >=
+  // S bounds the unrolled loops: columns S..SM-1 are zero and are updated
+  // without a guard (a branch per entry cost more than the extra FMAs).
+  template <int SM>
   __global__ void __launch_bounds__(64)
     k_panel_lu(double *F, long long fbase, int n, int S, int *perm, long long pbase, int ps,
                int o, int *info, int batch, int yoff)
@@ -156,14 +159,14 @@ namespace
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    double A[PW_MAX];
+    double A[SM];
 #pragma unroll
-    for (int j = 0; j < PW_MAX; ++j)
+    for (int j = 0; j < SM; ++j)
       A[j] = (j < S && i < S) ? blk[i + j * S] : 0.0;
     bool pivoted = i >= S;
     int  myk = 0, bad = 0;
 #pragma unroll
-    for (int k = 0; k < PW_MAX; ++k)
+    for (int k = 0; k < SM; ++k)
       if (k < S)
       {
         const unsigned key = wave_max(row_key(A[k], !pivoted, i));
@@ -181,14 +184,13 @@ namespace
             myk     = k;
           }
 #pragma unroll
-        for (int j = k + 1; j < PW_MAX; ++j)
-          if (j < S)
-            A[j] -= l * lane_value(A[j], p);
+        for (int j = k + 1; j < SM; ++j)
+          A[j] -= l * lane_value(A[j], p);
       }
     if (i < S)
       {
 #pragma unroll
-        for (int j = 0; j < PW_MAX; ++j)
+        for (int j = 0; j < SM; ++j)
           if (j < S)
             Fb[myk + (long long)j * n] = A[j];
         perm[pbase + (long long)b * ps + myk] = i;
@@ -616,10 +618,22 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
           const long long fb = G.fbase + (long long)o * G.n + o;
           double         *Fp = T->d_F + fb;
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(k_panel_lu, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
-                               (size_t)Sp * Sp * sizeof(double), s, T->d_F, fb, G.n, Sp,
-                               T->d_perm, G.pbase + o, G.S, o, T->d_info + info_off, G.batch,
-                               y0);
+            {
+              const dim3   grid(std::min(YCHUNK, G.batch - y0));
+              const size_t lds = (size_t)Sp * Sp * sizeof(double);
+              if (Sp <= 16)
+                hipLaunchKernelGGL(k_panel_lu<16>, grid, dim3(64), lds, s, T->d_F, fb, G.n, Sp,
+                                   T->d_perm, G.pbase + o, G.S, o, T->d_info + info_off,
+                                   G.batch, y0);
+              else if (Sp <= 32)
+                hipLaunchKernelGGL(k_panel_lu<32>, grid, dim3(64), lds, s, T->d_F, fb, G.n, Sp,
+                                   T->d_perm, G.pbase + o, G.S, o, T->d_info + info_off,
+                                   G.batch, y0);
+              else
+                hipLaunchKernelGGL(k_panel_lu<PW_MAX>, grid, dim3(64), lds, s, T->d_F, fb, G.n,
+                                   Sp, T->d_perm, G.pbase + o, G.S, o, T->d_info + info_off,
+                                   G.batch, y0);
+            }
           if (Ip == 0)
             continue;
           // U12 = L11^-1 P F12, L21 = F21 U11^-1, F22 -= L21 U12
