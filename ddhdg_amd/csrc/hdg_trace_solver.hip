@@ -103,22 +103,23 @@ namespace
   constexpr int PW_MAX = 64;
 
   // the leading S x S block at Fb (leading dimension n, S <= 64) into LDS,
-  // one wave: lane t loads row t of eight columns at a time (eight loads in
-  // flight, not one), dst(ii, jj) = out[ii * rs + jj * cs]
+  // one wave: lane t loads row t of G columns at a time (G loads in flight
+  // from HBM, not one), dst(ii, jj) = out[ii * rs + jj * cs]
+  template <int G>
   __device__ __forceinline__ void stage_block(const double *Fb, long long n, int S, double *out,
                                               int rs, int cs, int t)
   {
     const int ti = t < S ? t : 0;
 #pragma unroll 1
-    for (int j0 = 0; j0 < S; j0 += 8)
+    for (int j0 = 0; j0 < S; j0 += G)
       {
-        double v[8];
+        double v[G];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < G; ++u)
           v[u] = Fb[ti + (long long)(j0 + u < S ? j0 + u : 0) * n];
         if (t < S)
 #pragma unroll
-          for (int u = 0; u < 8; ++u)
+          for (int u = 0; u < G; ++u)
             if (j0 + u < S)
               out[t * rs + (j0 + u) * cs] = v[u];
       }
@@ -155,7 +156,7 @@ namespace
     double   *Fb = F + fbase + (long long)b * n * n;
     // the block goes through LDS: loads from HBM in flight eight at a time
     // (row by row into registers they were issued one after the other)
-    stage_block(Fb, n, S, blk, 1, S, i);
+    stage_block<16>(Fb, n, S, blk, 1, S, i);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -218,9 +219,9 @@ namespace
     const int  r    = (left ? (int)blockIdx.x : (int)blockIdx.x - nbl) * 64 + t;
     double    *Fb   = F + fbase + (long long)b * n * n;
     if (left)
-      stage_block(Fb, n, S, Ts, PW_MAX, 1, t);
+      stage_block<16>(Fb, n, S, Ts, PW_MAX, 1, t);
     else
-      stage_block(Fb, n, S, Ts, 1, PW_MAX, t);
+      stage_block<16>(Fb, n, S, Ts, 1, PW_MAX, t);
     if (left && t < S)
       pm[t] = perm[pbase + (long long)b * ps + t];
     __syncthreads();
@@ -301,7 +302,7 @@ namespace
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
     const int    *pv = perm + pbase + (long long)b * ps;
-    stage_block(Fb, n, S, L, 1, S, t);
+    stage_block<32>(Fb, n, S, L, 1, S, t);
     const int i0 = t, i1 = t + 64;
     double    z0 = i0 < S ? wb[pv[i0]] : 0.0;
     double    z1 = i1 < S ? wb[pv[i1]] : 0.0;
@@ -355,7 +356,7 @@ namespace
     const int     t  = threadIdx.x;
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
-    stage_block(Fb, n, S, U, 1, S, t);
+    stage_block<32>(Fb, n, S, U, 1, S, t);
     const int i0 = t, i1 = t + 64;
     double    z0 = 0.0, z1 = 0.0;
     for (int h = 0; h < 2; ++h)
