@@ -428,6 +428,17 @@ def main():
         # low-bias currents are small differences of O(1) fluxes)
         dI = max(max(abs(a - b) for a, b in zip(rm.currents, rf.currents)) /
                  max(max(abs(b) for b in rf.currents), 1e-300) for rf, rm in zip(res, res_m))
+        # capacity: the same requests all arriving at once
+        burst = synthetic_requests(args.serve, seed=20261204, rate=0.0, sizes=(32, 96),
+                                   degrees=(1, 2), points=13, vmax=0.6)
+        res_b, wall_b = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots,
+                                 graph=True).run(burst)
+        # the mix with 20 % 3D requests (cubes of 8^3-12^3 hexahedra, degree 1)
+        mix = synthetic_requests(args.serve, seed=20261204, rate=args.serve_rate, sizes=(32, 96),
+                                 degrees=(1, 2), points=13, vmax=0.6, frac3d=0.2,
+                                 sizes3d=(8, 12))
+        res_x, wall_x = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots,
+                                 graph=True).run(mix)
         line["serving"] = {**summarize(res, swall), "max_concurrent": srv.max_active,
                            "graph_newton": True,
                            "without_graphs": {k: v for k, v in summarize(res_e, wall_e).items()
@@ -435,6 +446,19 @@ def main():
                            "fp32_refine": {**{k: v for k, v in summarize(res_m, wall_m).items()
                                               if k in keys + ("newton_iterations",)},
                                            "max_current_diff_vs_fp64_rel_to_sweep_max": dI},
+                           "burst": {**{k: v for k, v in summarize(res_b, wall_b).items()
+                                        if k in keys + ("converged",)},
+                                     "workload": "the same requests, all arriving at t = 0"},
+                           "with_3d": {**{k: v for k, v in summarize(res_x, wall_x).items()
+                                          if k in keys + ("converged", "requests")},
+                                       "requests_3d": sum(1 for q in mix if q.dim == 3),
+                                       "not_converged": [
+                                           {"dim": q.dim, "cells": list(q.n_cells),
+                                            "nd": q.nd, "na": q.na}
+                                           for q, r in zip(mix, sorted(res_x, key=lambda r: r.rid))
+                                           if not r.converged],
+                                       "workload": "the same arrivals with 20 % of the requests "
+                                                   "3D cubes of 8^3-12^3 hexahedra, degree 1"},
                            "workload": f"{args.serve} I-V sweeps (13 points 0-0.6 V), 2D, "
                                        f"{32}^2-{96}^2 cells log-uniform, degree 1-2, Poisson "
                                        f"arrivals {args.serve_rate}/s, FP64",
