@@ -155,6 +155,7 @@ namespace
     p.flux_type = C->phys.flux_type;
     p.phi_lin   = C->phys.phi_linearize ? 1 : 0;
     p.tau_cf    = C->d_taucf;
+    p.rgscr     = C->d_rgscr;
     p.tb     = C->tb;
     p.doping = C->d_doping;
     p.dirv   = C->d_dirv;
@@ -178,7 +179,7 @@ int hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys, hdg_ctx **ou
   if (!mesh || !phys || !out)
     return fail("hdg_ctx_create: null argument");
   if (!hdg::supported(mesh->dim, mesh->degree))
-    return fail("hdg_ctx_create: (dim, degree) not supported: 2D k=1..3, 3D k=1", -2);
+    return fail("hdg_ctx_create: (dim, degree) not supported: 2D k=1..3, 3D k=1..2", -2);
   if (phys->flux_type < 0 || phys->flux_type > 2)
     return fail("hdg_ctx_create: flux_type must be 0 (use_cell), 1 (use_trace) or 2 "
                 "(qiu_shi_stabilization)", -2);
@@ -277,6 +278,15 @@ int hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys, hdg_ctx **ou
     rc = upload(&C->d_dirv, std::vector<double>(C->nfaces * 3 * C->qf, 0.0));
   if (rc == 0)
     rc = upload(&C->d_neuv, std::vector<double>(C->nfaces * 3 * C->qf, 0.0));
+  if (rc == 0 && hdg::generic_scratch_bytes(C->d, C->k) > 0)
+    {
+      // the reduced systems of the generic kernel do not fit LDS at this
+      // shape: one slice per resident workgroup, allocated with the context
+      hipError_t e =
+        hipMalloc((void **)&C->d_rgscr, hdg::generic_scratch_bytes(C->d, C->k));
+      if (e != hipSuccess)
+        rc = hip_fail(e, "hdg_ctx_create: generic-kernel scratch");
+    }
   if (rc != 0)
     {
       std::string m = g_err;
@@ -292,7 +302,8 @@ void hdg_ctx_destroy(hdg_ctx *C)
   if (!C)
     return;
   for (void *p : {(void *)C->d_tab, (void *)C->d_doping, (void *)C->d_dirv,
-                  (void *)C->d_neuv, (void *)C->d_taucf, (void *)C->d_blkptr, (void *)C->d_nbk,
+                  (void *)C->d_neuv, (void *)C->d_taucf, (void *)C->d_rgscr, (void *)C->d_blkptr,
+                  (void *)C->d_nbk,
                   (void *)C->d_cslot, (void *)C->d_cells[0], (void *)C->d_cells[1]})
     if (p)
       hipFree(p);

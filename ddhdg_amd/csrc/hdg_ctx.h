@@ -29,6 +29,7 @@ struct hdg_ctx
   hdg::DevTables  tb{};
   double         *d_doping = nullptr, *d_dirv = nullptr, *d_neuv = nullptr;
   double         *d_taucf  = nullptr; // per (cell, face, comp) tau, or null
+  double         *d_rgscr  = nullptr; // the generic kernel's global scratch (3D degree 2)
   long long      *d_blkptr = nullptr;
   int            *d_nbk    = nullptr;
   unsigned char  *d_cslot  = nullptr;

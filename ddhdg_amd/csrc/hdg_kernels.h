@@ -48,6 +48,9 @@ namespace hdg
     const double *xc, *xt, *dxt;
     // outputs
     double *vals, *rhs, *dirout, *dxc, *Sout, *gout;
+    // global scratch of the generic kernel where its reduced system does not
+    // fit LDS (generic_scratch_bytes)
+    double *rgscr;
   };
 
   // launchers (hdg_kernels.hip)
@@ -56,6 +59,9 @@ namespace hdg
   hipError_t launch_reconstruct(int dim, int deg, const KParams &p, int ncells,
                                 hipStream_t s);
   bool supported(int dim, int deg);
+  // device bytes of the generic kernel's global scratch for a shape (0 when
+  // its reduced system fits LDS)
+  size_t generic_scratch_bytes(int dim, int deg);
   // register-resident pass 1 (hdg_fast2d.hip): 2D, degree 1-2, all three
   // components or the V-only thermodynamic-equilibrium operator
   bool       fast_supported(int dim, int deg, int ne, int thermo, const int en[3]);

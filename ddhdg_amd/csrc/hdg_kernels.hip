@@ -27,6 +27,8 @@
 // elsewhere (template.h:983-1009, 1249-1263, 1417-1435, 1523-1536,
 // 1865-1903, 2036-2071).  tau is per face: FixedTauComputer or
 // CellFaceTauComputer (include/np_solver_parameters.h:92-115, 157-236).
+#include <algorithm>
+
 #include "hdg_device.h"
 #include "hdg_kernels.h"
 
@@ -51,10 +53,15 @@ namespace hdg
       static constexpr int MAXR  = MAXNS + MAXT;
       static constexpr int LD    = MAXR + 2;
       static constexpr int SZ    = QC * (1 + DIM) + NFC * QF;
-      // LDS layout (doubles)
+      // the reduced system R and the column data G (doubles, from the base
+      // rg_base returns): in LDS with the rest, or, when they do not fit
+      // (3D degree 2: 794 KB), in a global scratch slice per workgroup
       static constexpr int oR     = 0;
       static constexpr int oG     = oR + MAXR * LD;
-      static constexpr int oXS    = oG + SZ * LD;
+      static constexpr int RGD    = (MAXR + SZ) * LD;
+      static constexpr bool BIG   = DIM == 3 && DEG >= 2;
+      // LDS layout (doubles)
+      static constexpr int oXS    = BIG ? 0 : RGD;
       static constexpr int oXT    = oXS + NCD;
       static constexpr int oU0    = oXT + NFC * 3 * NF;
       static constexpr int oW0    = oU0 + 3 * QC;
@@ -82,7 +89,22 @@ namespace hdg
       static constexpr int iBID   = iFACE + 2 * NFC; // face index as 2 ints
       static constexpr int NINT   = iBID + NFC;
       static constexpr size_t bytes = NDBL * sizeof(double) + NINT * sizeof(int);
+      static_assert(bytes <= 160 * 1024, "LDS part of the layout");
     };
+
+    // workgroups of a generic launch whose R and G live in global scratch
+    constexpr int GRID_BIG = 512;
+
+    // base of R and G for this workgroup
+    template <int DIM, int DEG>
+    __device__ __forceinline__ double *rg_base(const KParams &p, double *sm)
+    {
+      using S = Sh<DIM, DEG>;
+      if constexpr (S::BIG)
+        return p.rgscr + (size_t)blockIdx.x * S::RGD;
+      else
+        return sm;
+    }
 
     // ------------------------------------------------------------------
     // Shared first half of both passes: load, state at quadrature points,
@@ -97,7 +119,8 @@ namespace hdg
       constexpr int NB  = S::NB, NF = S::NF, QC = S::QC, QF = S::QF;
       constexpr int NFC = S::NFC, LD = S::LD, SZ = S::SZ;
       const int     tid = threadIdx.x;
-      double *R = sm + S::oR, *G = sm + S::oG, *xs = sm + S::oXS, *xt = sm + S::oXT;
+      double *rg = rg_base<DIM, DEG>(p, sm);
+      double *R = rg + S::oR, *G = rg + S::oG, *xs = sm + S::oXS, *xt = sm + S::oXT;
       double *u0 = sm + S::oU0, *W0 = sm + S::oW0, *u0f = sm + S::oU0F;
       double *W0f = sm + S::oW0F, *tr0 = sm + S::oTR0, *qd = sm + S::oQD;
       double *rhsq = sm + S::oRHSQ, *Wrc = sm + S::oWRC, *Wrf = sm + S::oWRF;
@@ -507,12 +530,12 @@ namespace hdg
     // Gaussian elimination of the first ns columns with row-scaled partial
     // pivoting (rows stay in place; used[] marks pivot rows)
     template <int DIM, int DEG>
-    __device__ void eliminate(double *sm, int *si, int ns, int nrow, int ncol)
+    __device__ void eliminate(double *rg, double *sm, int *si, int ns, int nrow, int ncol)
     {
       using S          = Sh<DIM, DEG>;
       constexpr int LD = S::LD;
       const int     tid = threadIdx.x;
-      double *R = sm + S::oR, *rs = sm + S::oRS, *mul = sm + S::oMUL;
+      double *R = rg + S::oR, *rs = sm + S::oRS, *mul = sm + S::oMUL;
       int    *used = si + S::iUSED, *piv = si + S::iPIV;
       for (int i = tid; i < nrow; i += NT)
         {
@@ -585,14 +608,16 @@ namespace hdg
       extern __shared__ double smem[];
       double *sm = smem;
       int    *si = reinterpret_cast<int *>(smem + S::NDBL);
-      const int b = blockIdx.x;
-      if (b >= ncell)
-        return;
+      // one cell per workgroup and round (a single round unless R and G are
+      // in global scratch, where the grid is capped)
+      for (int b = blockIdx.x; b < ncell; b += gridDim.x)
+      {
       const long long cell = cells[b];
       int             ns, T;
       build_reduced<DIM, DEG>(p, cell, false, sm, si, ns, T);
       constexpr int NF = S::NF, NFC = S::NFC, LD = S::LD;
-      double       *R   = sm + S::oR;
+      double       *rg  = rg_base<DIM, DEG>(p, sm);
+      double       *R   = rg + S::oR;
       double       *del = sm + S::oDEL;
       int          *isdir = si + S::iDIR, *bidv = si + S::iBID;
       long long    *facev = reinterpret_cast<long long *>(si + S::iFACE);
@@ -616,7 +641,7 @@ namespace hdg
           R[r * LD + ncol - 1] -= acc;
         }
       __syncthreads();
-      eliminate<DIM, DEG>(sm, si, ns, nrow, ncol);
+      eliminate<DIM, DEG>(rg, sm, si, ns, nrow, ncol);
       // output
       if (mode == 1)
         {
@@ -627,7 +652,8 @@ namespace hdg
             }
           for (int i = tid; i < T; i += NT)
             p.gout[(long long)b * T + i] = R[(ns + i) * LD + ncol - 1];
-          return;
+          __syncthreads();
+          continue;
         }
       const unsigned char *slot = p.cslot + cell * NFC * NFC;
       for (int e = tid; e < T * T; e += NT)
@@ -661,6 +687,9 @@ namespace hdg
           if (dir)
             p.dirout[row] = del[(gi * 3 + ci) * NF + (ri % NF)];
         }
+      // the next round reuses the workgroup's LDS and scratch
+      __syncthreads();
+      }
     }
 
     template <int DIM, int DEG>
@@ -670,17 +699,17 @@ namespace hdg
       extern __shared__ double smem[];
       double *sm = smem;
       int    *si = reinterpret_cast<int *>(smem + S::NDBL);
-      const long long cell = blockIdx.x;
-      if (cell >= ncells)
-        return;
+      for (long long cell = blockIdx.x; cell < ncells; cell += gridDim.x)
+      {
       int ns, T;
       build_reduced<DIM, DEG>(p, cell, true, sm, si, ns, T);
       constexpr int NB = S::NB, NF = S::NF, LD = S::LD, DIMc = DIM;
-      double       *R = sm + S::oR, *sol = sm + S::oSOL, *rhsq = sm + S::oRHSQ;
+      double       *rg = rg_base<DIM, DEG>(p, sm);
+      double       *R = rg + S::oR, *sol = sm + S::oSOL, *rhsq = sm + S::oRHSQ;
       double       *lupd = sm + S::oLUPD;
       int          *piv  = si + S::iPIV;
       const int     tid  = threadIdx.x;
-      eliminate<DIM, DEG>(sm, si, ns, ns, ns + 1);
+      eliminate<DIM, DEG>(rg, sm, si, ns, ns, ns + 1);
       if (tid == 0)
         for (int k = ns - 1; k >= 0; --k)
           {
@@ -722,6 +751,8 @@ namespace hdg
             }
           out[e] = v;
         }
+      __syncthreads();
+      }
     }
 
     template <int DIM, int DEG>
@@ -746,7 +777,10 @@ namespace hdg
         }
       if (ncell <= 0)
         return hipSuccess;
-      hipLaunchKernelGGL((condense_kernel<DIM, DEG>), dim3(ncell), dim3(NT), S::bytes, s, p,
+      if (S::BIG && !p.rgscr)
+        return hipErrorInvalidValue;
+      const int grid = S::BIG ? std::min(ncell, GRID_BIG) : ncell;
+      hipLaunchKernelGGL((condense_kernel<DIM, DEG>), dim3(grid), dim3(NT), S::bytes, s, p,
                          cells, ncell, color, mode);
       return hipGetLastError();
     }
@@ -758,7 +792,10 @@ namespace hdg
       hipError_t e = launch_c<DIM, DEG>(p, nullptr, 0, 0, 0, s); // sets attributes
       if (e != hipSuccess)
         return e;
-      hipLaunchKernelGGL((reconstruct_kernel<DIM, DEG>), dim3(ncells), dim3(NT), S::bytes, s,
+      if (S::BIG && !p.rgscr)
+        return hipErrorInvalidValue;
+      const int grid = S::BIG ? std::min(ncells, GRID_BIG) : ncells;
+      hipLaunchKernelGGL((reconstruct_kernel<DIM, DEG>), dim3(grid), dim3(NT), S::bytes, s,
                          p, ncells);
       return hipGetLastError();
     }
@@ -914,7 +951,14 @@ namespace hdg
 
   bool supported(int dim, int deg)
   {
-    return (dim == 2 && deg >= 1 && deg <= 3) || (dim == 3 && deg == 1);
+    return (dim == 2 && deg >= 1 && deg <= 3) || (dim == 3 && deg >= 1 && deg <= 2);
+  }
+
+  size_t generic_scratch_bytes(int dim, int deg)
+  {
+    if (dim == 3 && deg == 2)
+      return (size_t)GRID_BIG * Sh<3, 2>::RGD * sizeof(double);
+    return 0;
   }
 
   hipError_t launch_condense(int dim, int deg, const KParams &p, const int *cells, int ncell,
@@ -928,6 +972,8 @@ namespace hdg
       return launch_c<2, 3>(p, cells, ncell, color, mode, s);
     if (dim == 3 && deg == 1)
       return launch_c<3, 1>(p, cells, ncell, color, mode, s);
+    if (dim == 3 && deg == 2)
+      return launch_c<3, 2>(p, cells, ncell, color, mode, s);
     return hipErrorInvalidValue;
   }
 
@@ -941,6 +987,8 @@ namespace hdg
       return launch_r<2, 3>(p, ncells, s);
     if (dim == 3 && deg == 1)
       return launch_r<3, 1>(p, ncells, s);
+    if (dim == 3 && deg == 2)
+      return launch_r<3, 2>(p, ncells, s);
     return hipErrorInvalidValue;
   }
 } // namespace hdg

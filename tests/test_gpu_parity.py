@@ -12,7 +12,8 @@ from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(2, 4, 1), (2, 4, 2), (2, 3, 3), (3, 2, 1)]
+# (3, 2, 2): the generic kernel with its reduced systems in global scratch
+CASES = [(2, 4, 1), (2, 4, 2), (2, 3, 3), (3, 2, 1), (3, 2, 2)]
 
 
 def _torch():

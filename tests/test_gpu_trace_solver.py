@@ -15,7 +15,8 @@ pytestmark = pytest.mark.gpu
 # (24, 24) k=2, (20, 12) k=1 and (4, 4, 4) have separators of more than 72
 # dofs: their fronts are eliminated in several panels
 CASES = [(2, (4, 4), 1), (2, (8, 8), 2), (2, (6, 10), 2), (2, (5, 3), 3), (2, (1, 6), 1),
-         (2, (24, 24), 2), (2, (20, 12), 1), (3, (2, 3, 2), 1), (3, (4, 4, 4), 1)]
+         (2, (24, 24), 2), (2, (20, 12), 1), (3, (2, 3, 2), 1), (3, (4, 4, 4), 1),
+         (3, (2, 3, 2), 2)]
 
 
 def _torch():
