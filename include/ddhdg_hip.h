@@ -43,7 +43,7 @@ typedef struct hdg_ctx hdg_ctx;
 typedef struct
 {
   int    dim;        /* 2 or 3 */
-  int    degree;     /* k: 2D 1..3, 3D 1 */
+  int    degree;     /* k: 2D 1..3, 3D 1..2 */
   int    n_cells[3];
   double lower[3];
   double upper[3];
@@ -117,19 +117,20 @@ int  hdg_sizes(const hdg_ctx *ctx, long long out[HDG_NSIZES]);
  * rebuilds the restricted trace pattern. */
 int hdg_set_enabled(hdg_ctx *ctx, int V, int n, int p);
 
-/* Kernel choice for pass 1: 0 = automatic (register-resident wave-per-cell
- * kernel where it applies: 2D, degree 1-2, all components or the V-only
- * equilibrium operator; otherwise the LDS workgroup-per-cell kernel),
- * 1 = always the LDS kernel, 2 = register-resident (error if it does not
- * apply).  No counterpart in the reference (one CPU code path). */
+/* Kernel choice for pass 1: 0 = automatic (a register-resident kernel
+ * where one applies: 2D degree 1-2 and 3D degree 1, all components or the
+ * V-only equilibrium operator; 2D degree 3 with all components; otherwise
+ * the generic workgroup-per-cell kernel), 1 = always the generic kernel,
+ * 2 = register-resident (error if none applies).  No counterpart in the
+ * reference (one CPU code path). */
 int hdg_set_kernel_variant(hdg_ctx *ctx, int variant);
 
 /* Precision of pass 1's local elimination: 0 = FP64 (default); 1 = the
  * scalar block eliminated in FP32 with one FP64 iterative-refinement step
  * (assembly, the residual, the refinement residual and the Schur product
- * in FP64).  Applies to the register-resident kernel (2D, degree 1-2);
- * the LDS kernel and pass 2 always run in FP64.  No counterpart in the
- * reference (FP64 throughout). */
+ * in FP64).  Applies to the register-resident kernels of 2D degree 1-2 and
+ * 3D degree 1; the other kernels and pass 2 always run in FP64.  No
+ * counterpart in the reference (FP64 throughout). */
 int hdg_set_precision(hdg_ctx *ctx, int precision);
 
 /* Restricted trace matrix pattern, host arrays: rowptr (NTRACE+1),
