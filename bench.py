@@ -433,10 +433,11 @@ def main():
                                    degrees=(1, 2), points=13, vmax=0.6)
         res_b, wall_b = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots,
                                  graph=True).run(burst)
-        # the mix with 20 % 3D requests (cubes of 8^3-12^3 hexahedra, degree 1)
+        # the mix with 20 % 3D requests (cubes of 8^3-12^3 hexahedra, degree
+        # 1 or 2)
         mix = synthetic_requests(args.serve, seed=20261204, rate=args.serve_rate, sizes=(32, 96),
                                  degrees=(1, 2), points=13, vmax=0.6, frac3d=0.2,
-                                 sizes3d=(8, 12))
+                                 sizes3d=(8, 12), degrees3d=(1, 2))
         res_x, wall_x = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots,
                                  graph=True).run(mix)
         line["serving"] = {**summarize(res, swall), "max_concurrent": srv.max_active,
@@ -452,13 +453,16 @@ def main():
                            "with_3d": {**{k: v for k, v in summarize(res_x, wall_x).items()
                                           if k in keys + ("converged", "requests")},
                                        "requests_3d": sum(1 for q in mix if q.dim == 3),
+                                       "requests_3d_degree2": sum(1 for q in mix
+                                                                  if q.dim == 3 and q.degree == 2),
                                        "not_converged": [
                                            {"dim": q.dim, "cells": list(q.n_cells),
-                                            "nd": q.nd, "na": q.na}
+                                            "degree": q.degree, "nd": q.nd, "na": q.na}
                                            for q, r in zip(mix, sorted(res_x, key=lambda r: r.rid))
                                            if not r.converged],
                                        "workload": "the same arrivals with 20 % of the requests "
-                                                   "3D cubes of 8^3-12^3 hexahedra, degree 1"},
+                                                   "3D cubes of 8^3-12^3 hexahedra, degree 1 or "
+                                                   "2"},
                            "workload": f"{args.serve} I-V sweeps (13 points 0-0.6 V), 2D, "
                                        f"{32}^2-{96}^2 cells log-uniform, degree 1-2, Poisson "
                                        f"arrivals {args.serve_rate}/s, FP64",

@@ -219,14 +219,14 @@ def summarize(results, wall):
 
 
 def synthetic_requests(n, seed, rate, sizes=(32, 96), degrees=(1, 2), points=13, vmax=0.6,
-                       frac3d=0.0, sizes3d=(8, 12)):
+                       frac3d=0.0, sizes3d=(8, 12), degrees3d=(1,)):
     """requests with Poisson arrivals at `rate` per second (all at once for
     rate 0), log-uniform mesh sides in `sizes` (2D), degrees drawn from
     `degrees`, log-uniform doping in [1e21, 1e24] m^-3 and a junction in
     [0.3, 0.7]; each sweeps `points` bias points over [0, vmax] V.  A
     fraction `frac3d` of the requests (drawn from a second stream, so the 2D
-    draws do not change) are 3D cubes of degree 1 with log-uniform sides in
-    `sizes3d`."""
+    draws do not change) are 3D cubes with log-uniform sides in `sizes3d`
+    and degrees drawn from `degrees3d`."""
     rng = np.random.default_rng(seed)
     rng3 = np.random.default_rng(seed + 1)
     t, out = 0.0, []
@@ -237,7 +237,7 @@ def synthetic_requests(n, seed, rate, sizes=(32, 96), degrees=(1, 2), points=13,
         dim, cells = 2, (side, side)
         if frac3d > 0 and rng3.uniform() < frac3d:
             s3 = int(round(np.exp(rng3.uniform(np.log(sizes3d[0]), np.log(sizes3d[1])))))
-            dim, cells, degree = 3, (s3, s3, s3), 1
+            dim, cells, degree = 3, (s3, s3, s3), int(rng3.choice(degrees3d))
         out.append(IVRequest(
             rid=i, dim=dim, n_cells=cells, degree=degree,
             biases=tuple(np.linspace(0.0, vmax, points)[1:]),
