@@ -201,13 +201,15 @@ namespace
   }
 
   // The off-diagonal blocks of one panel, one launch: blocks x < nbl (one
-  // lane per column of F12) form U12 = L11^-1 P F12 (the column's rows
-  // gathered in pivot order), blocks x >= nbl (one lane per row of F21)
+  // lane per column, the Ic columns from column c0 of the panel's trailing
+  // front) form U12 = L11^-1 P F12 (the column's rows gathered in pivot
+  // order), blocks x >= nbl (one lane per row, the Ir rows below the panel)
   // form L21 = F21 U11^-1.  The factor is staged in LDS and read at
   // wave-uniform addresses; the lane's column or row stays in registers.
   __global__ void __launch_bounds__(64)
-    k_panel_trsm(double *F, long long fbase, int n, int S, int I, const int *__restrict__ perm,
-                 long long pbase, int ps, int nbl, int batch, int yoff)
+    k_panel_trsm(double *F, long long fbase, int n, int S, int Ic, int c0, int Ir,
+                 const int *__restrict__ perm, long long pbase, int ps, int nbl, int batch,
+                 int yoff)
   {
     __shared__ double Ts[PW_MAX * PW_MAX]; // left: L row-major; right: U column-major
     __shared__ int    pm[PW_MAX];
@@ -225,11 +227,11 @@ namespace
     if (left && t < S)
       pm[t] = perm[pbase + (long long)b * ps + t];
     __syncthreads();
-    const bool act = r < I;
+    const bool act = r < (left ? Ic : Ir);
     double     y[PW_MAX];
     if (left)
       {
-        double *col = Fb + (long long)(S + (act ? r : 0)) * n;
+        double *col = Fb + (long long)(c0 + (act ? r : 0)) * n;
 #pragma unroll
         for (int j = 0; j < PW_MAX; ++j)
           {
@@ -452,6 +454,12 @@ namespace
   // a sweep's off-diagonal update stays in the one-wave panel kernel up to
   // this many trailing rows, and goes to a batched gemv above
   constexpr int FUSE_I = 256;
+  // panels per super-panel of the factorisation (rank of its trailing GEMM:
+  // SP panel widths)
+  constexpr int SP = 4;
+  // super-panels where the trailing fronts of a group hold at least this
+  // many entries (64 MB)
+  constexpr long long SP_MIN_ENTRIES = 8ll << 20;
 
   int grid_for(long long work)
   {
@@ -609,47 +617,94 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
     {
       const hdg::NDGroup &G = T->P.groups[gi];
       const DevGroup     &D = T->dg[gi];
-      // the eliminated block in panels of at most pw dofs (whole faces):
-      // panel [o, o + Sp) of each front is the leading block of the
-      // trailing front at (o, o), of dimension n - o
-      for (int o = 0; o < G.S; o += T->pw)
+      // the eliminated block in panels of at most pw dofs (whole faces),
+      // SP panels to a super-panel: within a super-panel each panel's LU,
+      // its triangular solves and a GEMM update of the super-panel's own
+      // columns; then the super-panel's U12 for the columns beyond it, and
+      // one GEMM of rank up to SP*pw for the trailing front (the rank-pw
+      // updates of single panels each re-read the whole trailing front)
+      const long long nn = (long long)G.n * G.n;
+      auto lu = [&](int q0, int Sp) {
+        const long long fb = G.fbase + (long long)q0 * G.n + q0;
+        for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+          {
+            const dim3   grid(std::min(YCHUNK, G.batch - y0));
+            const size_t lds = (size_t)Sp * Sp * sizeof(double);
+            if (Sp <= 16)
+              hipLaunchKernelGGL(k_panel_lu<16>, grid, dim3(64), lds, s, T->d_F, fb, G.n, Sp,
+                                 T->d_perm, G.pbase + q0, G.S, q0, T->d_info + info_off,
+                                 G.batch, y0);
+            else if (Sp <= 32)
+              hipLaunchKernelGGL(k_panel_lu<32>, grid, dim3(64), lds, s, T->d_F, fb, G.n, Sp,
+                                 T->d_perm, G.pbase + q0, G.S, q0, T->d_info + info_off,
+                                 G.batch, y0);
+            else
+              hipLaunchKernelGGL(k_panel_lu<PW_MAX>, grid, dim3(64), lds, s, T->d_F, fb, G.n,
+                                 Sp, T->d_perm, G.pbase + q0, G.S, q0, T->d_info + info_off,
+                                 G.batch, y0);
+          }
+      };
+      // U12 of panel q0 on Ic columns from column c0 of its corner, L21 on
+      // the Ir rows below it
+      auto trsm = [&](int q0, int Sp, int Ic, int c0, int Ir) {
+        const long long fb  = G.fbase + (long long)q0 * G.n + q0;
+        const int       nbc = (Ic + 63) / 64, nbr = (Ir + 63) / 64;
+        if (nbc + nbr == 0)
+          return;
+        for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+          hipLaunchKernelGGL(k_panel_trsm, dim3(nbc + nbr, std::min(YCHUNK, G.batch - y0)),
+                             dim3(64), 0, s, T->d_F, fb, G.n, Sp, Ic, c0, Ir, T->d_perm,
+                             G.pbase + q0, G.S, nbc, G.batch, y0);
+      };
+      // F(r0.., c0..) -= F(r0.., k0..k0+K) F(k0..k0+K, c0..), M x N, every front
+      auto gemm = [&](int r0, int c0, int k0, int M, int N, int K) -> bool {
+        if (M <= 0 || N <= 0 || K <= 0)
+          return true;
+        double *Fg = T->d_F + G.fbase;
+        return rocblas_dgemm_strided_batched(
+                 T->h, rocblas_operation_none, rocblas_operation_none, M, N, K, &mone,
+                 Fg + r0 + (long long)k0 * G.n, G.n, (rocblas_stride)nn,
+                 Fg + k0 + (long long)c0 * G.n, G.n, (rocblas_stride)nn, &one,
+                 Fg + r0 + (long long)c0 * G.n, G.n, (rocblas_stride)nn,
+                 G.batch) == rocblas_status_success;
+      };
+      bool ok = true;
+      for (int o = 0, SBo = 0; o < G.S && ok; o += SBo)
         {
-          const int       Sp = std::min(T->pw, G.S - o);
-          const int       Ip = G.n - o - Sp;
-          const long long fb = G.fbase + (long long)o * G.n + o;
-          double         *Fp = T->d_F + fb;
-          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+          // a super-panel pays when the trailing fronts are large (each
+          // rank-pw pass re-reads them); otherwise its extra launches cost more
+          const long long rest = (long long)(G.n - o) * (G.n - o) * G.batch;
+          const int SB  = std::min((rest >= SP_MIN_ENTRIES ? SP : 1) * T->pw, G.S - o);
+          const int end = o + SB;
+          SBo           = SB;
+          if (SB <= T->pw)
             {
-              const dim3   grid(std::min(YCHUNK, G.batch - y0));
-              const size_t lds = (size_t)Sp * Sp * sizeof(double);
-              if (Sp <= 16)
-                hipLaunchKernelGGL(k_panel_lu<16>, grid, dim3(64), lds, s, T->d_F, fb, G.n, Sp,
-                                   T->d_perm, G.pbase + o, G.S, o, T->d_info + info_off,
-                                   G.batch, y0);
-              else if (Sp <= 32)
-                hipLaunchKernelGGL(k_panel_lu<32>, grid, dim3(64), lds, s, T->d_F, fb, G.n, Sp,
-                                   T->d_perm, G.pbase + o, G.S, o, T->d_info + info_off,
-                                   G.batch, y0);
-              else
-                hipLaunchKernelGGL(k_panel_lu<PW_MAX>, grid, dim3(64), lds, s, T->d_F, fb, G.n,
-                                   Sp, T->d_perm, G.pbase + o, G.S, o, T->d_info + info_off,
-                                   G.batch, y0);
+              // a single panel: both solves over the whole trailing front
+              const int Ip = G.n - end;
+              lu(o, SB);
+              trsm(o, SB, Ip, SB, Ip);
+              ok = gemm(end, end, o, Ip, Ip, SB);
+              continue;
             }
-          if (Ip == 0)
-            continue;
-          // U12 = L11^-1 P F12, L21 = F21 U11^-1, F22 -= L21 U12
-          const int nbl = (Ip + 63) / 64;
-          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(k_panel_trsm, dim3(2 * nbl, std::min(YCHUNK, G.batch - y0)),
-                               dim3(64), 0, s, T->d_F, fb, G.n, Sp, Ip, T->d_perm, G.pbase + o,
-                               G.S, nbl, G.batch, y0);
-          const rocblas_status st = rocblas_dgemm_strided_batched(
-            T->h, rocblas_operation_none, rocblas_operation_none, Ip, Ip, Sp, &mone, Fp + Sp, G.n,
-            (rocblas_stride)G.n * G.n, Fp + (long long)Sp * G.n, G.n, (rocblas_stride)G.n * G.n,
-            &one, Fp + Sp + (long long)Sp * G.n, G.n, (rocblas_stride)G.n * G.n, G.batch);
-          if (st != rocblas_status_success)
-            return hdg::set_error("hdg_trace_factor: rocblas gemm failed", -3);
+          for (int q0 = o; q0 < end && ok; q0 += T->pw)
+            {
+              const int Sp = std::min(T->pw, end - q0);
+              lu(q0, Sp);
+              trsm(q0, Sp, end - q0 - Sp, Sp, G.n - q0 - Sp);
+              ok = gemm(q0 + Sp, q0 + Sp, q0, G.n - q0 - Sp, end - q0 - Sp, Sp);
+            }
+          const int Icols = G.n - end;
+          for (int q0 = o; q0 < end && ok && Icols > 0; q0 += T->pw)
+            {
+              const int Sp = std::min(T->pw, end - q0);
+              trsm(q0, Sp, Icols, end - q0, 0);
+              ok = gemm(q0 + Sp, end, q0, end - q0 - Sp, Icols, Sp);
+            }
+          if (ok)
+            ok = gemm(end, end, o, G.n - end, Icols, SB);
         }
+      if (!ok)
+        return hdg::set_error("hdg_trace_factor: rocblas gemm failed", -3);
       info_off += G.batch;
       if (G.I > 0)
         for (int want = 0; want < 2; ++want)
