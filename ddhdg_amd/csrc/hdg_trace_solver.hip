@@ -400,6 +400,99 @@ namespace
       wb[i1] = z1;
   }
 
+
+  // ---------------------------------------------------------------------
+  // The sweeps' off-diagonal updates where the rest of a panel's trailing
+  // front is long.  Forward: w_rest -= L21 z, one thread per row, z (the
+  // panel's solved entries) in LDS, each L21 column read coalesced across
+  // the rows.  SM >= S bounds the unrolled loop; z is zero past S.
+  template <int SM>
+  __global__ void __launch_bounds__(256)
+    k_sweep_rows(const double *F, double *w, long long fbase, long long wbase, int n, int S,
+                 int I, int batch, int yoff)
+  {
+    __shared__ double z[SM];
+    const int b = blockIdx.y + yoff;
+    if (b >= batch)
+      return;
+    const double *Fb = F + fbase + (long long)b * n * n;
+    double       *wb = w + wbase + (long long)b * n;
+    const int     t  = threadIdx.x;
+    if (t < SM)
+      z[t] = t < S ? wb[t] : 0.0;
+    __syncthreads();
+    const int r = blockIdx.x * 256 + t;
+    if (r >= I)
+      return;
+    const double *col = Fb + S + r;
+    double        a0 = 0.0, a1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < SM; ++j)
+      {
+        const double v = col[(long long)(j < S ? j : 0) * n] * z[j];
+        if (j & 1)
+          a1 += v;
+        else
+          a0 += v;
+      }
+    wb[S + r] -= a0 + a1;
+  }
+
+  // Backward, first half: partial sums of U12 x_rest over chunks of 256
+  // columns, one workgroup per (chunk, front): lane j of each wave takes
+  // row j (coalesced down a column), the four waves every fourth column;
+  // the chunk's sums go to part[(b * nch + chunk) * 64 + j]
+  __global__ void __launch_bounds__(256)
+    k_sweep_cols_part(const double *F, const double *w, long long fbase, long long wbase, int n,
+                      int S, int I, int nch, double *part, int batch, int yoff)
+  {
+    __shared__ double red[4][64];
+    const int b = blockIdx.y + yoff;
+    if (b >= batch)
+      return;
+    const double *Fb = F + fbase + (long long)b * n * n;
+    const double *wb = w + wbase + (long long)b * n;
+    const int     t = threadIdx.x, j = t & 63, cg = t >> 6;
+    const int     jc = j < S ? j : 0;
+    const double  mj = j < S ? 1.0 : 0.0;
+    const int     c0 = blockIdx.x * 256;
+    double        a0 = 0.0, a1 = 0.0;
+#pragma unroll 8
+    for (int i = 0; i < 64; ++i)
+      {
+        const int c  = c0 + cg + 4 * i;
+        const int cc = c < I ? c : 0;
+        const double v = Fb[jc + (long long)(S + cc) * n] * (c < I ? wb[S + cc] : 0.0);
+        if (i & 1)
+          a1 += v;
+        else
+          a0 += v;
+      }
+    red[cg][j] = (a0 + a1) * mj;
+    __syncthreads();
+    if (t < 64)
+      part[((long long)b * nch + blockIdx.x) * 64 + t] =
+        (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+  }
+
+  // Backward, second half: w_j -= the chunks' partial sums, in chunk order
+  // (deterministic), one wave per front
+  __global__ void __launch_bounds__(64)
+    k_sweep_cols_sum(double *w, long long wbase, int n, int S, int nch, const double *part,
+                     int batch, int yoff)
+  {
+    const int b = blockIdx.x + yoff;
+    if (b >= batch)
+      return;
+    const int t = threadIdx.x;
+    if (t >= S)
+      return;
+    double a = 0.0;
+    for (int k = 0; k < nch; ++k)
+      a += part[((long long)b * nch + k) * 64 + t];
+    w[wbase + (long long)b * n + t] -= a;
+  }
+
   __global__ void k_gather_rhs(const double *__restrict__ b, const long long *__restrict__ wmap,
                                long long ndof, double *__restrict__ w)
   {
@@ -490,6 +583,7 @@ struct hdg_trace_solver
   double               *d_F    = nullptr, *d_w = nullptr;
   int                  *d_info = nullptr;
   int                   pw     = PW_MAX; // panel width: whole faces, at most PW_MAX dofs
+  double               *d_part = nullptr; // backward sweep: chunk partial sums
   int                  *d_perm = nullptr; // row order of the small fronts' factored blocks
   long long            *d_map = nullptr, *d_pad = nullptr, *d_wmap = nullptr;
 };
@@ -541,6 +635,11 @@ int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
       hipError_t e = hipMalloc((void **)&T->d_F, (size_t)P.Ftot * sizeof(double));
       if (e == hipSuccess)
         e = hipMalloc((void **)&T->d_w, (size_t)std::max<long long>(1, P.wtot) * sizeof(double));
+      long long npart = 1;
+      for (const hdg::NDGroup &G : P.groups)
+        npart = std::max<long long>(npart, (long long)G.batch * ((G.n + 255) / 256) * 64);
+      if (e == hipSuccess)
+        e = hipMalloc((void **)&T->d_part, (size_t)npart * sizeof(double));
       if (e == hipSuccess)
         e = hipMalloc((void **)&T->d_perm, (size_t)std::max<long long>(1, P.ptot) * sizeof(int));
       if (e == hipSuccess)
@@ -574,7 +673,7 @@ void hdg_trace_solver_destroy(hdg_trace_solver *T)
                     (void *)D.d_pos, (void *)D.d_edof, (void *)D.d_idof})
       if (p)
         (void)hipFree(p);
-  for (void *p : {(void *)T->d_F, (void *)T->d_w, (void *)T->d_perm,
+  for (void *p : {(void *)T->d_F, (void *)T->d_w, (void *)T->d_part, (void *)T->d_perm,
                   (void *)T->d_info,
                   (void *)T->d_map, (void *)T->d_pad, (void *)T->d_wmap})
     if (p)
@@ -747,9 +846,6 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
   if (!T || !rhs || !x)
     return hdg::set_error("hdg_trace_solve: null argument", -1);
   hipStream_t s = (hipStream_t)stream;
-  rocblas_set_stream(T->h, s);
-  rocblas_set_pointer_mode(T->h, rocblas_pointer_mode_host);
-  const double one = 1.0, mone = -1.0;
   hipLaunchKernelGGL(k_fill, dim3(grid_for(T->P.wtot)), dim3(TPB), 0, s, T->d_w, T->P.wtot, 0.0);
   hipLaunchKernelGGL(k_gather_rhs, dim3(grid_for(T->P.ndof)), dim3(TPB), 0, s, rhs, T->d_wmap,
                      T->P.ndof, T->d_w);
@@ -772,14 +868,19 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
                                G.wbase + o, G.n, Sp, fuse ? Ip : 0, T->d_perm, G.pbase + o, G.S,
                                G.batch, y0);
           if (!fuse)
-            {
-              double *wp = T->d_w + G.wbase + o;
-              if (rocblas_dgemv_strided_batched(
-                    T->h, rocblas_operation_none, Ip, Sp, &mone, T->d_F + fb + Sp, G.n,
-                    (rocblas_stride)G.n * G.n, wp, 1, (rocblas_stride)G.n, &one, wp + Sp, 1,
-                    (rocblas_stride)G.n, G.batch) != rocblas_status_success)
-                return hdg::set_error("hdg_trace_solve: rocblas gemv failed", -3);
-            }
+            for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+              {
+                const dim3 grid((Ip + 255) / 256, std::min(YCHUNK, G.batch - y0));
+                if (Sp <= 16)
+                  hipLaunchKernelGGL(k_sweep_rows<16>, grid, dim3(256), 0, s, T->d_F, T->d_w, fb,
+                                     G.wbase + o, G.n, Sp, Ip, G.batch, y0);
+                else if (Sp <= 32)
+                  hipLaunchKernelGGL(k_sweep_rows<32>, grid, dim3(256), 0, s, T->d_F, T->d_w, fb,
+                                     G.wbase + o, G.n, Sp, Ip, G.batch, y0);
+                else
+                  hipLaunchKernelGGL(k_sweep_rows<PW_MAX>, grid, dim3(256), 0, s, T->d_F, T->d_w,
+                                     fb, G.wbase + o, G.n, Sp, Ip, G.batch, y0);
+              }
         }
       if (G.I > 0)
         for (int want = 0; want < 2; ++want)
@@ -812,15 +913,16 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
               const bool      fuse = Ip <= FUSE_I;
               const long long fb   = G.fbase + (long long)o * G.n + o;
               if (!fuse)
-                {
-                  double *wp = T->d_w + G.wbase + o;
-                  if (rocblas_dgemv_strided_batched(
-                        T->h, rocblas_operation_none, Sp, Ip, &mone,
-                        T->d_F + fb + (long long)Sp * G.n, G.n, (rocblas_stride)G.n * G.n,
-                        wp + Sp, 1, (rocblas_stride)G.n, &one, wp, 1, (rocblas_stride)G.n,
-                        G.batch) != rocblas_status_success)
-                    return hdg::set_error("hdg_trace_solve: rocblas gemv failed", -3);
-                }
+                for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+                  {
+                    const int nb  = std::min(YCHUNK, G.batch - y0);
+                    const int nch = (Ip + 255) / 256;
+                    hipLaunchKernelGGL(k_sweep_cols_part, dim3(nch, nb), dim3(256), 0, s, T->d_F,
+                                       T->d_w, fb, G.wbase + o, G.n, Sp, Ip, nch, T->d_part,
+                                       G.batch, y0);
+                    hipLaunchKernelGGL(k_sweep_cols_sum, dim3(nb), dim3(64), 0, s, T->d_w,
+                                       G.wbase + o, G.n, Sp, nch, T->d_part, G.batch, y0);
+                  }
               for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
                 hipLaunchKernelGGL(k_bwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
                                    (size_t)Sp * Sp * sizeof(double), s, T->d_F, T->d_w, fb,
