@@ -28,6 +28,7 @@
 // 1865-1903, 2036-2071).  tau is per face: FixedTauComputer or
 // CellFaceTauComputer (include/np_solver_parameters.h:92-115, 157-236).
 #include <algorithm>
+#include <utility>
 
 #include "hdg_device.h"
 #include "hdg_kernels.h"
@@ -56,10 +57,10 @@ namespace hdg
       // the reduced system R and the column data G (doubles, from the base
       // rg_base returns): in LDS with the rest, or, when they do not fit
       // (3D degree 2: 794 KB), in a global scratch slice per workgroup
-      static constexpr int oR     = 0;
-      static constexpr int oG     = oR + MAXR * LD;
-      static constexpr int RGD    = (MAXR + SZ) * LD;
       static constexpr bool BIG   = DIM == 3 && DEG >= 2;
+      static constexpr int oR     = 0;
+      static constexpr int oG     = oR + MAXR * LD; // unused when BIG
+      static constexpr int RGD    = (MAXR + (BIG ? 0 : SZ)) * LD;
       // LDS layout (doubles)
       static constexpr int oXS    = BIG ? 0 : RGD;
       static constexpr int oXT    = oXS + NCD;
@@ -79,7 +80,11 @@ namespace hdg
       static constexpr int oMUL   = oRS + MAXR;
       static constexpr int oSOL   = oMUL + MAXR;
       static constexpr int oTAU   = oSOL + MAXNS;   // tau per (face, comp)
-      static constexpr int NDBL   = oTAU + NFC * 3;
+      // BIG: the scalar block (MAXNS x AL) factorised in LDS, later the
+      // staging tiles of the Schur product
+      static constexpr int AL     = MAXNS + 1;
+      static constexpr int oAW    = oTAU + NFC * 3;
+      static constexpr int NDBL   = oAW + (BIG ? MAXNS * AL : 0);
       // ints after the doubles
       static constexpr int iUSED  = 0;
       static constexpr int iPIV   = iUSED + MAXR;
@@ -90,7 +95,18 @@ namespace hdg
       static constexpr int NINT   = iBID + NFC;
       static constexpr size_t bytes = NDBL * sizeof(double) + NINT * sizeof(int);
       static_assert(bytes <= 160 * 1024, "LDS part of the layout");
+      static_assert(!BIG || 2 * bytes <= 160 * 1024, "two workgroups per CU");
     };
+
+    typedef const __attribute__((address_space(4))) double ctabg;
+
+    // f(std::integral_constant<int, I>) for I = 0, 1, ...: loops whose
+    // index selects a register of a per-thread array stay expanded
+    template <class F, int... I>
+    __device__ __forceinline__ void static_for(F &&f, std::integer_sequence<int, I...>)
+    {
+      (f(std::integral_constant<int, I>{}), ...);
+    }
 
     // workgroups of a generic launch whose R and G live in global scratch
     constexpr int GRID_BIG = 512;
@@ -353,11 +369,12 @@ namespace hdg
       for (int ci = 0; ci < ne; ++ci)
         {
           const int c = p.encomp[ci];
-          for (int e = tid; e < SZ * ncol; e += NT)
-            {
-              const int slot = e / ncol, m = e % ncol;
-              // decode column m
-              int kind, cc = 0, j = 0, f = 0;
+          // column m: kind 0 = scalar dof j of component cc, 1 = trace dof j
+          // of component cc on face f, 2 = the rhs
+          auto decode = [&](int m, int &kind, int &cc, int &j, int &f) {
+              cc = 0;
+              j  = 0;
+              f  = 0;
               if (m < ns)
                 {
                   kind = 0;
@@ -374,6 +391,10 @@ namespace hdg
                 }
               else
                 kind = 2;
+          };
+          // the bundle entry of column m at quadrature slot `slot` (cell
+          // points: value and flux axes; then face points)
+          auto colval = [&](int slot, int kind, int cc, int j, int f) -> double {
               double val = 0;
               if (slot < QC * (1 + DIM))
                 {
@@ -487,43 +508,116 @@ namespace hdg
                                tc * p0 - tc * l0;
                     }
                 }
-              G[slot * LD + m] = val;
-            }
-          __syncthreads();
-          // rows of this component: z rows (NB) then trace rows (NFC*NF)
-          const int nzr = NB + (pass2 ? 0 : NFC * NF);
-          for (int e = tid; e < nzr * ncol; e += NT)
+              return val;
+          };
+          if constexpr (!S::BIG)
             {
-              const int rr = e / ncol, m = e % ncol;
-              double    v  = 0;
-              int       row;
-              if (rr < NB)
+              for (int e = tid; e < SZ * ncol; e += NT)
                 {
-                  row = ci * NB + rr;
-                  for (int sl = 0; sl < SZ; ++sl)
-                    v += tb.Bz[sl * NB + rr] * G[sl * LD + m];
+                  const int slot = e / ncol, m = e % ncol;
+                  int       kind, cc, j, f;
+                  decode(m, kind, cc, j, f);
+                  G[slot * LD + m] = colval(slot, kind, cc, j, f);
                 }
-              else
+              __syncthreads();
+              // rows of this component: z rows (NB) then trace rows (NFC*NF)
+              const int nzr = NB + (pass2 ? 0 : NFC * NF);
+              for (int e = tid; e < nzr * ncol; e += NT)
                 {
-                  const int tr = rr - NB, g = tr / NF, i = tr % NF;
-                  row = ns + g * tdf + ci * NF + i;
-                  if (!isdir[g * 3 + c])
+                  const int rr = e / ncol, m = e % ncol;
+                  double    v  = 0;
+                  int       row;
+                  if (rr < NB)
                     {
-                      for (int s = 0; s < QF; ++s)
-                        v += tb.Bt[(g * QF + s) * NF + i] *
-                             G[(QC * (1 + DIM) + g * QF + s) * LD + m];
-                      if (m == ncol - 1 && isneu[g * 3 + c])
-                        for (int s = 0; s < QF; ++s)
-                          v += tb.Bt[(g * QF + s) * NF + i] *
-                               p.neuv[(facev[g] * 3 + c) * QF + s];
+                      row = ci * NB + rr;
+                      for (int sl = 0; sl < SZ; ++sl)
+                        v += tb.Bz[sl * NB + rr] * G[sl * LD + m];
                     }
                   else
-                    v = 0;
+                    {
+                      const int tr = rr - NB, g = tr / NF, i = tr % NF;
+                      row = ns + g * tdf + ci * NF + i;
+                      if (!isdir[g * 3 + c])
+                        {
+                          for (int s = 0; s < QF; ++s)
+                            v += tb.Bt[(g * QF + s) * NF + i] *
+                                 G[(QC * (1 + DIM) + g * QF + s) * LD + m];
+                          if (m == ncol - 1 && isneu[g * 3 + c])
+                            for (int s = 0; s < QF; ++s)
+                              v += tb.Bt[(g * QF + s) * NF + i] *
+                                   p.neuv[(facev[g] * 3 + c) * QF + s];
+                        }
+                      else
+                        v = 0;
+                    }
+                  R[row * LD + m] = v;
                 }
-              R[row * LD + m] = v;
+              __syncthreads();
             }
-          __syncthreads();
+          else
+            {
+              // R and G in global memory: one thread per column computes
+              // its bundle entries slot by slot and folds each into the z
+              // rows and the trace rows of the slot's face at once (the
+              // same sums in the same order as the two-step form), so G is
+              // never stored and R is written once
+              ctabg *Bzc = (ctabg *)tb.Bz, *Btc = (ctabg *)tb.Bt;
+              for (int m = tid; m < ncol; m += NT)
+                {
+                  int kind, cc, j, f;
+                  decode(m, kind, cc, j, f);
+                  double az[NB];
+#pragma unroll
+                  for (int rr = 0; rr < NB; ++rr)
+                    az[rr] = 0;
+                  for (int slot = 0; slot < QC * (1 + DIM); ++slot)
+                    {
+                      const double v = colval(slot, kind, cc, j, f);
+#pragma unroll
+                      for (int rr = 0; rr < NB; ++rr)
+                        az[rr] += Bzc[slot * NB + rr] * v;
+                    }
+                  for (int g = 0; g < NFC; ++g)
+                    {
+                      double at[NF];
+#pragma unroll
+                      for (int i = 0; i < NF; ++i)
+                        at[i] = 0;
+                      for (int s = 0; s < QF; ++s)
+                        {
+                          const int    slot = QC * (1 + DIM) + g * QF + s;
+                          const double v    = colval(slot, kind, cc, j, f);
+#pragma unroll
+                          for (int rr = 0; rr < NB; ++rr)
+                            az[rr] += Bzc[slot * NB + rr] * v;
+#pragma unroll
+                          for (int i = 0; i < NF; ++i)
+                            at[i] += Btc[(g * QF + s) * NF + i] * v;
+                        }
+                      if (!pass2)
+                        {
+                          const bool dir = isdir[g * 3 + c];
+                          const bool neu = m == ncol - 1 && isneu[g * 3 + c];
+#pragma unroll
+                          for (int i = 0; i < NF; ++i)
+                            {
+                              double w = at[i];
+                              if (neu)
+                                for (int s = 0; s < QF; ++s)
+                                  w += tb.Bt[(g * QF + s) * NF + i] *
+                                       p.neuv[(facev[g] * 3 + c) * QF + s];
+                              R[(ns + g * tdf + ci * NF + i) * LD + m] = dir ? 0.0 : w;
+                            }
+                        }
+                    }
+#pragma unroll
+                  for (int rr = 0; rr < NB; ++rr)
+                    R[(ci * NB + rr) * LD + m] = az[rr];
+                }
+            }
         }
+      if constexpr (S::BIG)
+        __syncthreads();
       (void)nrow;
     }
 
@@ -600,8 +694,242 @@ namespace hdg
         }
     }
 
+    // The same elimination on an nr x nc block A in LDS (leading dimension
+    // ld, nr scalar rows, the first nr columns eliminated), with each
+    // row's multiplier of step k (rows not yet pivoted) stored in A[i][k]:
+    // afterwards row piv[k] holds [L(k, 0..k-1) | U(k, k..)] in its own
+    // place.  Pivot choice and update order are those of eliminate().
     template <int DIM, int DEG>
-    __global__ void __launch_bounds__(NT)
+    __device__ __forceinline__ void lu_rows(double *A, int ld, double *sm, int *si, int nr, int nc)
+    {
+      using S   = Sh<DIM, DEG>;
+      const int tid = threadIdx.x;
+      double *rs = sm + S::oRS, *mul = sm + S::oMUL;
+      int    *used = si + S::iUSED, *piv = si + S::iPIV;
+      for (int i = tid; i < nr; i += NT)
+        {
+          double mx = 0;
+          for (int j = 0; j < nr; ++j)
+            mx = fmax(mx, fabs(A[i * ld + j]));
+          rs[i]   = mx > 0 ? 1.0 / mx : 1.0;
+          used[i] = 0;
+        }
+      __syncthreads();
+      // update lanes: row tid % 128, every other column from tid / 128
+      const int ui = tid & 127, uh = tid >> 7;
+      for (int k = 0; k < nr; ++k)
+        {
+          if (tid < 64)
+            {
+              double best = -1;
+              int    bi   = -1;
+              for (int i = tid; i < nr; i += 64)
+                if (!used[i])
+                  {
+                    const double sc = fabs(A[i * ld + k]) * rs[i];
+                    if (sc > best || bi < 0) // a NaN column still picks a row
+                      {
+                        best = sc;
+                        bi   = i;
+                      }
+                  }
+              for (int off = 32; off > 0; off >>= 1)
+                {
+                  const double ob = __shfl_xor(best, off);
+                  const int    oi = __shfl_xor(bi, off);
+                  if ((bi < 0 && oi >= 0) || ob > best ||
+                      (ob == best && oi >= 0 && (bi < 0 || oi < bi)))
+                    {
+                      best = ob;
+                      bi   = oi;
+                    }
+                }
+              if (tid == 0)
+                {
+                  piv[k]   = bi;
+                  used[bi] = 2; // current pivot
+                }
+            }
+          __syncthreads();
+          const int    pr = piv[k];
+          const double pv = A[pr * ld + k];
+          for (int i = tid; i < nr; i += NT)
+            {
+              const bool   act = used[i] == 0;
+              const double m   = act ? A[i * ld + k] / pv : 0.0;
+              mul[i]           = m;
+              if (act)
+                A[i * ld + k] = m;
+            }
+          __syncthreads();
+          if (ui < nr)
+            {
+              const double m = mul[ui];
+              if (m != 0.0)
+                for (int j = k + 1 + uh; j < nc; j += 2)
+                  A[ui * ld + j] -= m * A[pr * ld + j];
+            }
+          __syncthreads();
+          if (tid == 0)
+            used[pr] = 1;
+          __syncthreads();
+        }
+    }
+
+    // The two triangular solves of schur_big for NSS scalar rows (rows of A
+    // in place, leading dimensions LD for R and AL for A):
+    //   Y = L^-1 P [CT' | c']  thread j < T + 1 holds column NSS + j
+    //   M = TC' U^-1           thread t < T holds trace row NSS + t
+    template <int NSS, int LD, int AL>
+    __device__ __forceinline__ void tri_solves(double *R, const double *A, const int *piv,
+                                               int tid, int T)
+    {
+      if (tid < T + 1)
+        {
+          double *col = R + NSS + tid;
+          double  y[NSS];
+#pragma unroll
+          for (int k = 0; k < NSS; ++k)
+            y[k] = col[piv[k] * LD];
+          static_for(
+            [&](auto kc) {
+              constexpr int k = decltype(kc)::value;
+              if constexpr (k >= 1)
+                {
+                  const double *a   = A + piv[k] * AL;
+                  double        acc = y[k];
+#pragma unroll
+                  for (int l = 0; l < k; ++l)
+                    acc -= a[l] * y[l];
+                  y[k] = acc;
+                }
+            },
+            std::make_integer_sequence<int, NSS>{});
+#pragma unroll
+          for (int k = 0; k < NSS; ++k)
+            col[piv[k] * LD] = y[k];
+        }
+      if (tid < T)
+        {
+          double *row = R + (NSS + tid) * LD;
+          double  x[NSS];
+#pragma unroll
+          for (int k = 0; k < NSS; ++k)
+            x[k] = row[k];
+          static_for(
+            [&](auto kc) {
+              constexpr int k = decltype(kc)::value;
+              const double *a = A + piv[k] * AL;
+              const double  m = x[k] / a[k];
+              x[k]            = m;
+#pragma unroll
+              for (int j = k + 1; j < NSS; ++j)
+                x[j] -= m * a[j];
+            },
+            std::make_integer_sequence<int, NSS>{});
+#pragma unroll
+          for (int k = 0; k < NSS; ++k)
+            row[k] = x[k];
+        }
+    }
+
+    // eliminate() for shapes whose R lives in global memory (ns scalar rows,
+    // T trace rows, T + 1 right-hand columns), as a blocked Schur complement:
+    //   A' = P L U            lu_rows on the scalar block, in LDS
+    //   Y  = L^-1 P [CT' | c']  one thread per column, in place in R
+    //   M  = TC' U^-1          one thread per trace row, in place in R
+    //   [S | g] = [TT' | t'] - M Y   LDS-staged product
+    // Every entry receives the same multiply-adds in the same order as in
+    // eliminate(), which reads and writes the whole of R once per pivot.
+    template <int DIM, int DEG>
+    __device__ __forceinline__ void schur_big(double *R, double *sm, int *si, int ns, int T)
+    {
+      using S             = Sh<DIM, DEG>;
+      constexpr int LD    = S::LD, AL = S::AL, MAXNS = S::MAXNS;
+      constexpr int KC    = 9;                      // k per staged chunk
+      constexpr int RA    = 6;                      // row slots per pass
+      constexpr int CB    = (S::MAXT + 1 + 15) / 16; // column slots
+      static_assert(S::NB % KC == 0, "ns is a multiple of the chunk");
+      static_assert(16 * RA * KC + KC * 16 * CB <= MAXNS * AL, "staging fits");
+      double    *A   = sm + S::oAW;
+      const int *piv = si + S::iPIV;
+      const int  tid = threadIdx.x;
+      const int  W   = T + 1;
+      for (int e = tid; e < ns * ns; e += NT)
+        {
+          const int i = e / ns, j = e % ns;
+          A[i * AL + j] = R[i * LD + j];
+        }
+      __syncthreads();
+      lu_rows<DIM, DEG>(A, AL, sm, si, ns, ns);
+      if (ns == MAXNS)
+        tri_solves<MAXNS, LD, AL>(R, A, piv, tid, T);
+      else if (ns == 2 * S::NB)
+        tri_solves<2 * S::NB, LD, AL>(R, A, piv, tid, T);
+      else
+        tri_solves<S::NB, LD, AL>(R, A, piv, tid, T);
+      __syncthreads();
+      constexpr int WP = 16 * CB;       // staged Y width (zero beyond W)
+      double   *Ms = A;                // [16 RA][KC], rows of this pass
+      double   *Ys = A + 16 * RA * KC; // [KC][WP]
+      const int tx = tid & 15, ty = tid >> 4;
+      for (int r0 = 0; r0 < T; r0 += 16 * RA)
+        {
+          double acc[RA][CB];
+#pragma unroll
+          for (int a = 0; a < RA; ++a)
+#pragma unroll
+            for (int b = 0; b < CB; ++b)
+              {
+                const int t = r0 + ty + 16 * a, c = tx + 16 * b;
+                acc[a][b]   = (t < T && c < W) ? R[(ns + t) * LD + ns + c] : 0.0;
+              }
+          for (int k0 = 0; k0 < ns; k0 += KC)
+            {
+              __syncthreads();
+              for (int e = tid; e < 16 * RA * KC; e += NT)
+                {
+                  const int t = r0 + e / KC;
+                  Ms[e]       = t < T ? R[(ns + t) * LD + k0 + e % KC] : 0.0;
+                }
+              for (int e = tid; e < KC * WP; e += NT)
+                {
+                  const int kk = e / WP, c = e % WP;
+                  Ys[e]        = c < W ? R[piv[k0 + kk] * LD + ns + c] : 0.0;
+                }
+              __syncthreads();
+#pragma unroll
+              for (int kk = 0; kk < KC; ++kk)
+                {
+                  double mv[RA], yv[CB];
+#pragma unroll
+                  for (int a = 0; a < RA; ++a)
+                    mv[a] = Ms[(ty + 16 * a) * KC + kk];
+#pragma unroll
+                  for (int b = 0; b < CB; ++b)
+                    yv[b] = Ys[kk * WP + tx + 16 * b];
+#pragma unroll
+                  for (int a = 0; a < RA; ++a)
+#pragma unroll
+                    for (int b = 0; b < CB; ++b)
+                      acc[a][b] -= mv[a] * yv[b];
+                }
+            }
+#pragma unroll
+          for (int a = 0; a < RA; ++a)
+#pragma unroll
+            for (int b = 0; b < CB; ++b)
+              {
+                const int t = r0 + ty + 16 * a, c = tx + 16 * b;
+                if (t < T && c < W)
+                  R[(ns + t) * LD + ns + c] = acc[a][b];
+              }
+        }
+      __syncthreads();
+    }
+
+    template <int DIM, int DEG>
+    __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
       condense_kernel(KParams p, const int *cells, int ncell, int color, int mode)
     {
       using S = Sh<DIM, DEG>;
@@ -626,7 +954,10 @@ namespace hdg
       const int     nrow = ns + T, ncol = ns + T + 1;
       // LocalCondenser::condense_ct_matrix (src/local_condenser.cpp:141-160)
       // on the reduced columns: rhs -= delta_j * col_j, col_j = 0
-      for (int r = tid; r < nrow; r += NT)
+      bool anydir = false;
+      for (int g = 0; g < 3 * NFC; ++g)
+        anydir = anydir || isdir[g];
+      for (int r = tid; anydir && r < nrow; r += NT)
         {
           double acc = 0;
           for (int lt = 0; lt < T; ++lt)
@@ -641,7 +972,10 @@ namespace hdg
           R[r * LD + ncol - 1] -= acc;
         }
       __syncthreads();
-      eliminate<DIM, DEG>(rg, sm, si, ns, nrow, ncol);
+      if constexpr (S::BIG)
+        schur_big<DIM, DEG>(R, sm, si, ns, T);
+      else
+        eliminate<DIM, DEG>(rg, sm, si, ns, nrow, ncol);
       // output
       if (mode == 1)
         {
@@ -693,7 +1027,7 @@ namespace hdg
     }
 
     template <int DIM, int DEG>
-    __global__ void __launch_bounds__(NT) reconstruct_kernel(KParams p, int ncells)
+    __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) reconstruct_kernel(KParams p, int ncells)
     {
       using S = Sh<DIM, DEG>;
       extern __shared__ double smem[];
@@ -709,15 +1043,33 @@ namespace hdg
       double       *lupd = sm + S::oLUPD;
       int          *piv  = si + S::iPIV;
       const int     tid  = threadIdx.x;
-      eliminate<DIM, DEG>(rg, sm, si, ns, ns, ns + 1);
+      // U and the reduced rhs: R itself, or for R in global memory a copy
+      // of the ns x (ns + 1) system factorised in LDS
+      const double *U  = R;
+      int           lu = LD;
+      if constexpr (S::BIG)
+        {
+          double *A = sm + S::oAW;
+          for (int e = tid; e < ns * (ns + 1); e += NT)
+            {
+              const int i = e / (ns + 1), j = e % (ns + 1);
+              A[i * S::AL + j] = R[i * LD + j];
+            }
+          __syncthreads();
+          lu_rows<DIM, DEG>(A, S::AL, sm, si, ns, ns + 1);
+          U  = A;
+          lu = S::AL;
+        }
+      else
+        eliminate<DIM, DEG>(rg, sm, si, ns, ns, ns + 1);
       if (tid == 0)
         for (int k = ns - 1; k >= 0; --k)
           {
             const int pr = piv[k];
-            double    v  = R[pr * LD + ns];
+            double    v  = U[pr * lu + ns];
             for (int j = k + 1; j < ns; ++j)
-              v -= R[pr * LD + j] * sol[j];
-            sol[k] = v / R[pr * LD + k];
+              v -= U[pr * lu + j] * sol[j];
+            sol[k] = v / U[pr * lu + k];
           }
       __syncthreads();
       // W = M^-1 (rhs_q + D_a u - sum_f C_{a,f} lambda_f), written with u

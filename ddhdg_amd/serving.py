@@ -78,9 +78,9 @@ def memory_estimate(req: IVRequest) -> int:
     ndof = nfaces * tdf
     fronts = 8 * (11 * ndof * tdf * np.log2(max(ncells, 2)) if d == 2 else 4 * ndof ** (4 / 3) * tdf)
     # the generic kernel's global scratch where its reduced systems do not
-    # fit LDS (3D degree 2): 512 slices of (R + G)
+    # fit LDS (3D degree 2): 512 slices of R
     maxr = 3 * nb + 2 * d * 3 * nf
-    scratch = 512 * (maxr + nb * (1 + d) + 2 * d * nf) * (maxr + 2) * 8 if (d == 3 and k >= 2) else 0
+    scratch = 512 * maxr * (maxr + 2) * 8 if (d == 3 and k >= 2) else 0
     return int(8 * (4 * ncells * NC + 6 * nfaces * 3 * nf + nnz) + fronts + scratch + (64 << 20))
 
 
