@@ -200,6 +200,53 @@ def sample_3d(args, torch, stream):
     return out
 
 
+def sample_3d_k2(args, torch, stream):
+    """3D at degree 2 on the generic kernels: a cube diode of n^3 hexahedra
+    (324 cell dofs, 162 trace dofs per cell) with a step junction; pass 1,
+    pass 2 and the trace factorisation and solve, FP64"""
+    from ddhdg_amd import configs
+    from ddhdg_amd.np_solver import NPSolver
+
+    n = args.cells3d_k2
+    prob, params, adim = configs.diode(3, n, 2, doping=configs.step_doping(1e22, 1e22))
+    s = NPSolver(prob, params, adim, device="cuda")
+    s.compute_local_charge_neutrality()
+    s.project_densities_from_potential()
+    ctx = s.ctx
+    vals = torch.empty(ctx.nnz, dtype=torch.float64, device="cuda")
+    rhs = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
+    dirv = torch.zeros(ctx.ntrace, dtype=torch.float64, device="cuda")
+    dxt = torch.zeros_like(s.x_trace)
+    dxc = torch.empty_like(s.x_cell)
+    ts = s.trace_solver()
+    sol = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
+    reps = 3
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    p1 = timed(lambda: ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv))
+    p2 = timed(lambda: ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc))
+    fm = timed(lambda: ts.factor(vals))
+    sm = timed(lambda: ts.solve(rhs, sol))
+    return {"workload": f"3D diode {n}^3 hexahedra, degree 2, V/n/p (324 cell dofs, 162 trace "
+                        "dofs), step junction N_D = N_A = 1e22 m^-3; generic kernels, FP64",
+            "cells": int(ctx.ncells),
+            "pass1": {"value": ctx.ncells / (p1 * 1e-3), "unit": "cells/s", "ms_per_pass": p1},
+            "pass2_ms": p2, "factor_ms": fm, "solve_ms": sm,
+            "factor_GFLOP": ts.flops / 1e9, "factor_TFLOPs": ts.flops / (fm * 1e-3) / 1e12,
+            "max_front": int(ts.max_front)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -214,6 +261,8 @@ def main():
                     help="whole Newton steps timed beside the metric")
     ap.add_argument("--cells3d", type=int, default=48,
                     help="cells per axis of the 3D sample (0: none)")
+    ap.add_argument("--cells3d-k2", type=int, default=16,
+                    help="cells per axis of the 3D degree-2 sample (0: none)")
     ap.add_argument("--batch", type=int, default=64,
                     help="devices condensed together beside the metric (0: none)")
     ap.add_argument("--serve", type=int, default=16,
@@ -406,6 +455,9 @@ def main():
                              "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12}}
     if args.cells3d > 0:
         line["sample_3d"] = sample_3d(args, torch, stream)
+    if args.cells3d_k2 > 0:
+        line["sample_3d_k2"] = sample_3d_k2(args, torch, stream)
+        torch.cuda.empty_cache()
     if args.batch > 0:
         line["batched"] = batched_sample(args, torch, stream)
     if args.serve > 0:
