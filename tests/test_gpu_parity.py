@@ -67,6 +67,37 @@ def test_cell_systems_match_oracle(dim, n_cells, k, thermodyn, variant, consiste
     assert worst_S < 1e-12 and worst_g < 1e-12
 
 
+@pytest.mark.parametrize("dim,n_cells,k", [(2, 4, 2), (3, 2, 1), (3, 2, 2)])
+@pytest.mark.parametrize("variant", [1, 0])
+def test_cell_systems_carriers_only_match_oracle(dim, n_cells, k, variant):
+    """n and p enabled with V frozen (the second stage of the diode
+    workflow): two components, so 2 (k+1)^d scalar rows per cell"""
+    t = _torch()
+    print(f"case cell_systems n,p dim={dim} n={n_cells} k={k} variant={variant}", flush=True)
+    s, prob, params, adim = _solver(dim, n_cells, k, variant)
+    enabled = (False, True, True)
+    s.set_enabled_components(*enabled)
+    o = H.oracle_for(prob, params, adim, enabled)
+    cell, tr = _state(prob, adim, k, seed=5 + dim + k)
+    s.set_state(cell, tr)
+    o.set_state(cell, tr)
+    ctx = s.ctx
+    cells = t.arange(ctx.ncells, dtype=t.int32, device="cuda")
+    S = t.empty(ctx.ncells * ctx.T * ctx.T, dtype=t.float64, device="cuda")
+    g = t.empty(ctx.ncells * ctx.T, dtype=t.float64, device="cuda")
+    ctx.cell_systems(s.x_cell, s.x_trace, False, cells, S, g)
+    t.cuda.synchronize()
+    S = S.cpu().numpy().reshape(ctx.ncells, ctx.T, ctx.T)
+    g = g.cpu().numpy().reshape(ctx.ncells, ctx.T)
+    worst_S = worst_g = 0.0
+    for c in range(ctx.ncells):
+        So, go, _, _, _ = o.condensed_cell(c, False)
+        worst_S = max(worst_S, H.rel_fro(S[c], So))
+        worst_g = max(worst_g, H.rel_fro(g[c], go))
+    print(f"  worst rel S {worst_S:.3e} g {worst_g:.3e}")
+    assert worst_S < 1e-12 and worst_g < 1e-12
+
+
 @pytest.mark.parametrize("dim,n_cells,k", CASES)
 @pytest.mark.parametrize("variant", [1, 0])
 def test_global_system_matches_oracle(dim, n_cells, k, variant):
@@ -144,7 +175,7 @@ def _close(s, o, tol=1e-9):
             np.abs(tg - to).max() <= tol * np.abs(to).max())
 
 
-@pytest.mark.parametrize("dim,n_cells,k", [(2, 12, 1), (2, 8, 2), (2, 6, 3)])
+@pytest.mark.parametrize("dim,n_cells,k", [(2, 12, 1), (2, 8, 2), (2, 6, 3), (3, 2, 2)])
 def test_newton_diode_matches_oracle(dim, n_cells, k):
     """the reference's diode workflow (equilibrium, n and p with V frozen,
     coupled system, 0.1 V and 0.3 V forward bias): equal Newton iteration
