@@ -14,11 +14,14 @@ points, which change only boundary data in place.
 Scheduling: requests wait in arrival order and are admitted while a slot is
 free and their device-memory estimate (context tables, trace-solver fronts,
 Newton buffers) fits the HBM budget, so requests arriving while others run
-join the running set as soon as slots and memory free up.  Every tick launches
-one Newton iteration of every active request, each on the stream of its slot
-(default priority), synchronises once and lets every request advance its own
-state machine.  Requests share no buffers and every kernel is deterministic,
-so a request's results are the same alone and in any mix.
+join the running set as soon as slots and memory free up.  Every active
+request keeps one Newton iteration in flight on the stream of its slot
+(default priority), with an event recorded behind it; the loop polls the
+events and advances each request whose iteration has finished (its state
+machine reads the update norms from pinned host memory) and launches its next
+one, so a slow request (a large 3D cube) does not hold back the others.
+Requests share no buffers and every kernel is deterministic, so a request's
+results are the same alone and in any mix.
 """
 from __future__ import annotations
 
@@ -105,6 +108,8 @@ class _Slot:
         self.admitted = now
         self.t0 = t0
         self.ok = True
+        self.done = torch.cuda.Event()
+        self.inflight = False
         self._start_stage()
 
     def _start_stage(self):
@@ -132,9 +137,19 @@ class _Slot:
     def launch(self):
         with self.torch.cuda.stream(self.stream):
             self.it.launch()
+            self.done.record(self.stream)
+        self.inflight = True
+
+    def poll(self):
+        """True once the iteration in flight has finished on the device"""
+        if self.inflight and self.done.query():
+            self.inflight = False
+            return True
+        return False
 
     def advance(self, now):
-        """after the tick's synchronisation: True when the request is done"""
+        """after the iteration in flight has finished: True when the request
+        is done"""
         status = self.it.finish()
         if status == "continue":
             return False
@@ -170,6 +185,7 @@ class IVServer:
         torch = self.torch
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         active, done, used = [], {}, 0
+        # ticks: Newton iterations launched
         self.max_active, self.ticks = 0, 0
         t0 = time.perf_counter() if clock is None else clock()
         now = lambda: (time.perf_counter() if clock is None else clock()) - t0
@@ -188,14 +204,18 @@ class IVServer:
                     time.sleep(min(wait, 0.05))
                 continue
             self.max_active = max(self.max_active, len(active))
-            self.ticks += 1
             for slot, _ in active:
-                slot.launch()
-            torch.cuda.synchronize()
+                if not slot.inflight:
+                    slot.launch()
+                    self.ticks += 1
+            ready = [slot.poll() for slot, _ in active]
+            if not any(ready):
+                time.sleep(5e-5)
+                continue
             t = now()
             keep = []
-            for slot, need in active:
-                if slot.advance(t):
+            for (slot, need), fin in zip(active, ready):
+                if fin and slot.advance(t):
                     r = slot.req
                     state = slot.solver.get_state() if self.keep_state else None
                     done[r.rid] = IVResult(r.rid, list(r.biases), slot.currents, slot.iterations,
