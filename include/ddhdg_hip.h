@@ -106,8 +106,9 @@ enum
 const char *hdg_last_error(void);
 
 /* Context over one mesh on the current HIP device.  Allocates the device
- * tables (geometry, lifting operators, scatter map).  All components
- * enabled on creation. */
+ * tables (geometry, lifting operators, scatter map) and, for 3D degree 2,
+ * the generic kernel's working storage for its reduced systems (512
+ * slices of 476 KB, 244 MB).  All components enabled on creation. */
 int  hdg_ctx_create(const hdg_mesh_desc *mesh, const hdg_phys *phys,
                     hdg_ctx **out);
 void hdg_ctx_destroy(hdg_ctx *ctx);
