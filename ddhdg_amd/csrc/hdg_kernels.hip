@@ -703,6 +703,7 @@ namespace hdg
     __device__ __forceinline__ void lu_rows(double *A, int ld, double *sm, int *si, int nr, int nc)
     {
       using S   = Sh<DIM, DEG>;
+      static_assert(S::MAXNS <= 128, "one update lane pair per row");
       const int tid = threadIdx.x;
       double *rs = sm + S::oRS, *mul = sm + S::oMUL;
       int    *used = si + S::iUSED, *piv = si + S::iPIV;
@@ -850,6 +851,7 @@ namespace hdg
       constexpr int RA    = 6;                      // row slots per pass
       constexpr int CB    = (S::MAXT + 1 + 15) / 16; // column slots
       static_assert(S::NB % KC == 0, "ns is a multiple of the chunk");
+      static_assert(S::MAXT + 1 <= NT, "one thread per right-hand column and trace row");
       static_assert(16 * RA * KC + KC * 16 * CB <= MAXNS * AL, "staging fits");
       double    *A   = sm + S::oAW;
       const int *piv = si + S::iPIV;
