@@ -275,8 +275,9 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the per-cell path has no multi-GPU form in this "
-              "build (see DESIGN.md for how it would shard); run with --gpus 1",
+        print("bench.py measures one GPU: the per-cell path runs on one GPU only (a mesh "
+              "would be split into slabs of cells, one per GPU, with the trace rows of the "
+              "cut faces exchanged); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
