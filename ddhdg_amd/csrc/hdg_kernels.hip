@@ -707,9 +707,9 @@ namespace hdg
     __device__ __forceinline__ void lu_rows(double *A, int ld, double *sm, int *si, int nr, int nc)
     {
       using S   = Sh<DIM, DEG>;
-      static_assert(S::MAXNS <= 128, "one update lane pair per row");
+      static_assert(2 * S::MAXNS <= NT, "one update lane pair per row");
       const int tid = threadIdx.x;
-      double *rs = sm + S::oRS, *mul = sm + S::oMUL;
+      double *rs = sm + S::oRS;
       int    *used = si + S::iUSED, *piv = si + S::iPIV;
       for (int i = tid; i < nr; i += NT)
         {
@@ -720,8 +720,10 @@ namespace hdg
           used[i] = 0;
         }
       __syncthreads();
-      // update lanes: row tid % 128, every other column from tid / 128
-      const int ui = tid & 127, uh = tid >> 7;
+      // update lanes: a lane pair per row (row tid / 2, every other column
+      // from tid % 2), in one wave, so both read the row's entry in column k
+      // before the first of them overwrites it with the multiplier
+      const int ui = tid >> 1, uh = tid & 1;
       for (int k = 0; k < nr; ++k)
         {
           if (tid < 64)
@@ -756,27 +758,19 @@ namespace hdg
                 }
             }
           __syncthreads();
+          // rows still to pivot (the pivot rows of this and earlier steps
+          // keep used = 2)
           const int    pr = piv[k];
           const double pv = A[pr * ld + k];
-          for (int i = tid; i < nr; i += NT)
+          if (ui < nr && used[ui] == 0)
             {
-              const bool   act = used[i] == 0;
-              const double m   = act ? A[i * ld + k] / pv : 0.0;
-              mul[i]           = m;
-              if (act)
-                A[i * ld + k] = m;
-            }
-          __syncthreads();
-          if (ui < nr)
-            {
-              const double m = mul[ui];
+              const double m = A[ui * ld + k] / pv;
               if (m != 0.0)
                 for (int j = k + 1 + uh; j < nc; j += 2)
                   A[ui * ld + j] -= m * A[pr * ld + j];
+              if (uh == 0)
+                A[ui * ld + k] = m;
             }
-          __syncthreads();
-          if (tid == 0)
-            used[pr] = 1;
           __syncthreads();
         }
     }
