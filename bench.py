@@ -255,7 +255,8 @@ def main():
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--degree", type=int, default=2)
     ap.add_argument("--bias", type=float, default=0.3)
-    ap.add_argument("--cpu-sample", type=int, default=96)
+    ap.add_argument("--cpu-sample", type=int, default=192,
+                    help="cells per axis of the CPU baseline sample (about 10 s of CPU work)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=3,
                     help="whole Newton steps timed beside the metric")
