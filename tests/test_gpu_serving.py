@@ -73,3 +73,21 @@ def test_served_currents_match_sequential_solve():
         assert s.run().converged
         cur.append(s.terminal_current(1))
     assert np.array_equal(np.asarray(cur), np.asarray(res[0].currents))
+
+
+def test_3d_requests_same_alone_and_in_mix():
+    """3D requests (the register kernel at degree 1, the generic kernel at
+    degree 2) served beside 2D ones: each result bitwise the same as alone"""
+    _torch()
+    print("case serving mix with 3D requests vs each alone", flush=True)
+    reqs = [REQS[0], REQS[2],
+            IVRequest(4, 3, (3, 3, 3), 1, (0.1, 0.2), arrival=0.0),
+            IVRequest(5, 3, (2, 2, 2), 2, (0.1,), nd=3e22, na=2e23, junction=0.4, arrival=0.02)]
+    mix, _ = IVServer(max_slots=4, keep_state=True).run(reqs)
+    by_id = {r.rid: r for r in mix}
+    for req in reqs:
+        r = by_id[req.rid]
+        print(f"  request {r.rid}: iterations {r.iterations} currents {r.currents}")
+        alone, _ = IVServer(max_slots=1, keep_state=True).run(
+            [IVRequest(**{**req.__dict__, "arrival": 0.0})])
+        _same(r, alone[0])
