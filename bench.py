@@ -358,8 +358,18 @@ def main():
     gm, gd = gm.view(-1, T), gd.view(-1, T)
     err_S = float(((Sm - Sd).norm(dim=1) / Sd.norm(dim=1)).max())
     err_g = float(((gm - gd).norm(dim=1) / gd.norm(dim=1).clamp_min(1e-300)).max())
+    m_ms = e0m.elapsed_time(e1m) / args.steps / 2.0
+    # the dense count split by precision: LU and solves in FP32; the
+    # refinement residual and the Schur product in FP64
+    f32 = 2 * N ** 3 / 3 + 2 * N ** 2 * (T + 1)
+    f64 = 2 * N ** 2 * (T + 1) + 2 * T * N * (T + 1)
+    per_s = (cells / 2) / (m_ms * 1e-3) / 1e12
     fp32_refine = {"value": cells * args.steps / wall_m, "unit": "cells/s",
-                   "kernel_ms_per_launch": e0m.elapsed_time(e1m) / args.steps / 2.0,
+                   "kernel_ms_per_launch": m_ms,
+                   "roofline": {"fp32_flop_per_cell": f32, "fp64_flop_per_cell": f64,
+                                "achieved_fp32_TFLOPs": f32 * per_s, "peak_fp32_TFLOPs": 157.3,
+                                "achieved_fp64_TFLOPs": f64 * per_s,
+                                "peak_fp64_TFLOPs": FP64_PEAK_TFLOPS},
                    "vs_fp64": (cells * args.steps / wall_m) / value,
                    "max_rel_err_S_vs_fp64": err_S, "max_rel_err_g_vs_fp64": err_g,
                    "error_sample_cells": int(sample.numel())}
