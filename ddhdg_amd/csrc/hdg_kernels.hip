@@ -820,15 +820,12 @@ namespace hdg
               constexpr int k = decltype(kc)::value;
               const double *a = A + piv[k] * AL;
               const double  m = x[k] / a[k];
-              x[k]            = m;
+              row[k]          = m; // final: stored at once, not held
 #pragma unroll
               for (int j = k + 1; j < NSS; ++j)
                 x[j] -= m * a[j];
             },
             std::make_integer_sequence<int, NSS>{});
-#pragma unroll
-          for (int k = 0; k < NSS; ++k)
-            row[k] = x[k];
         }
     }
 
