@@ -268,7 +268,7 @@ namespace hdg
             {
               const int bt      = (bid >= 0 && p.en[c]) ? p.bc[bid][c] : 0;
               isdir[lf * 3 + c] = bt == 1;
-              isneu[lf * 3 + c] = bt == 2;
+              isneu[lf * 3 + c] = bt == 2 && !THERMO;  // zero Neumann data at equilibrium
             }
         }
       if (lane < NFC * NFC)

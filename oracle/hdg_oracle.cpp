@@ -1300,7 +1300,9 @@ namespace
                   // Q6: V keeps the use_cell tt inside the use_trace branch
                   tt_matrix(lf, comp, ft);
                   tt_rhs(lf, comp);
-                  if (bt == 2)
+                  // in thermodynamic equilibrium the Neumann data is the zero
+                  // function (template.h:2229-2236)
+                  if (bt == 2 && !thermo)
                     nbc(cell, lf, comp);
                 }
             }
