@@ -44,8 +44,43 @@ def bytes_per_cell(N, T):
     return 8 * (N + T + T * T + T)
 
 
-def cpu_baseline(n_sample, degree, threads, bias):
-    """oracle pass 1 on an n_sample^2 diode of the same per-cell shape."""
+def host_cpu():
+    """the host the CPU baseline runs on: logical CPUs, the ones this
+    process may use, the model name"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
+def native_oracle():
+    """builds the oracle for this host (-march=native, outside the tree); the
+    shipped build targets x86-64-v3 so that it runs on any host.  Returns
+    the flags of the build in use."""
+    import subprocess
+    import tempfile
+
+    from oracle import oracle as O
+    out = os.path.join(tempfile.gettempdir(), f"liboracle_native_{os.getpid()}.so")
+    src = os.path.join(ROOT, "oracle", "hdg_oracle.cpp")
+    r = subprocess.run(["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-shared", "-o",
+                        out, src, "-lpthread"], capture_output=True, text=True)
+    if r.returncode == 0:
+        O.use_library(out)
+        return "-O3 -march=native (built on this host)"
+    return "-O3 -march=x86-64-v3 (shipped build; the native build failed)"
+
+
+def cpu_baseline(n_sample, degree, threads, bias, reps=5):
+    """oracle pass 1 on an n_sample^2 diode of the same per-cell shape: one
+    warm-up, then the median of reps timed runs"""
     import numpy as np
 
     from ddhdg_amd import configs
@@ -55,8 +90,8 @@ def cpu_baseline(n_sample, degree, threads, bias):
     prob, params, adim = configs.diode(2, n_sample, degree, bias=bias)
     o = H.oracle_for(prob, params, adim)
     o.set_state(*H.equilibrium_guess(prob, adim, degree))
-    o.pass1(False, threads, 64)  # warm-up on a few cells
-    times = [o.pass1(False, threads) for _ in range(3)]
+    o.pass1(False, threads)  # warm-up
+    times = [o.pass1(False, threads) for _ in range(reps)]
     t = float(np.median(times))
     return o.ncells / t, o.ncells, t
 
@@ -73,14 +108,8 @@ def batched_sample(args, torch, stream):
     from ddhdg_amd.hip_lib import HdgBatch
     from ddhdg_amd.np_solver import NPSolver
 
-    rng = np.random.default_rng(20261203)
     sv = []
-    for _ in range(args.batch):
-        n = int(rng.integers(16, 193))
-        k = int(rng.choice([1, 2, 3]))
-        dop = configs.step_doping(10 ** rng.uniform(21, 24), 10 ** rng.uniform(21, 24),
-                                  rng.uniform(0.3, 0.7))
-        prob, params, adim = configs.diode(2, n, k, doping=dop, bias=float(rng.uniform(0, 0.6)))
+    for prob, params, adim in configs.batched_devices(args.batch, 20261203):
         s = NPSolver(prob, params, adim, device="cuda")
         s.compute_local_charge_neutrality()
         s.project_densities_from_potential()
@@ -257,6 +286,8 @@ def main():
     ap.add_argument("--bias", type=float, default=0.3)
     ap.add_argument("--cpu-sample", type=int, default=192,
                     help="cells per axis of the CPU baseline sample (about 10 s of CPU work)")
+    ap.add_argument("--cpu-sample-1core", type=int, default=32,
+                    help="cells per axis of the one-core CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--newton-steps", type=int, default=3,
                     help="whole Newton steps timed beside the metric")
@@ -532,12 +563,22 @@ def main():
                                        f"arrivals {args.serve_rate}/s, FP64",
                            "unit": "I-V solves/s"}
     if not args.no_cpu_baseline:
-        threads = min(os.cpu_count() or 1, 16)
+        host = host_cpu()
+        build = native_oracle()
+        # all the cores this process may use (the box grants 16 per GPU), and one
+        threads = max(1, min(host["usable"], 16))
         v, nc, t = cpu_baseline(args.cpu_sample, args.degree, threads, args.bias)
+        v1, nc1, t1 = cpu_baseline(args.cpu_sample_1core, args.degree, 1, args.bias)
         line["cpu_baseline"] = {
             "value": v, "unit": "cells/s", "cores": threads, "kind": "port",
-            "sample": f"oracle pass 1 over a {args.cpu_sample}x{args.cpu_sample} diode of the same "
-                      f"per-cell shape ({nc} cells, median of 3 runs, {t:.2f} s each)"}
+            "sample": f"oracle pass 1 (full-inverse Gauss-Jordan + mmult, WorkStream threads, "
+                      f"in-order scatter) over a {args.cpu_sample}x{args.cpu_sample} diode of "
+                      f"the same per-cell shape ({nc} cells), median of 5 runs after 1 warm-up, "
+                      f"{t:.2f} s each",
+            "one_core": {"value": v1, "unit": "cells/s", "cores": 1,
+                         "sample": f"{args.cpu_sample_1core}x{args.cpu_sample_1core} diode "
+                                   f"({nc1} cells), median of 5 runs, {t1:.2f} s each"},
+            "host": host, "build": build}
         line["speedup_vs_cpu"] = value / v
     print(json.dumps(line), flush=True)
 

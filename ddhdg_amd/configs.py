@@ -140,3 +140,20 @@ def solve_equilibrium_and_bias(solver, biases=(), log=None):
         set_bias(solver, b)
         results.append(solver.run(log=log))
     return results
+
+
+def batched_devices(count=64, seed=20261203):
+    """the batched workload: `count` 2D diodes drawn from `seed`, side
+    uniform in 16..192 cells, degree uniform in {1, 2, 3}, donor and acceptor
+    densities log-uniform in [1e21, 1e24] m^-3, junction uniform in
+    [0.3, 0.7] um, contact bias uniform in [0, 0.6] V.  Returns a list of
+    (problem, parameters, adimensionalizer) in draw order."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        side = int(rng.integers(16, 193))
+        k = int(rng.choice([1, 2, 3]))
+        dop = step_doping(10 ** rng.uniform(21, 24), 10 ** rng.uniform(21, 24),
+                          rng.uniform(0.3, 0.7))
+        out.append(diode(2, side, k, doping=dop, bias=float(rng.uniform(0, 0.6))))
+    return out

@@ -609,6 +609,7 @@ struct hdg_batch
 {
   std::vector<hdg_ctx *> ctx;
   int                    ne = 0, thermo_ok = 0, full_ok = 0, wpb = 4, ncu = 256;
+  int                    en[3] = {0, 0, 0}; // enabled components at creation
   int                    deg3 = 0; // degree-3 meshes: the pair kernel, FP64
   int                   *d_chunk_dev[2] = {nullptr, nullptr}, *d_items[2] = {nullptr, nullptr};
   int                    nchunks[2] = {0, 0};
@@ -646,6 +647,8 @@ int hdg_batch_create(hdg_ctx *const *ctxs, int n, hdg_batch **out)
     return fail("hdg_batch_create: invalid argument");
   hdg_batch *B = new hdg_batch();
   B->ne        = ctxs[0] ? ctxs[0]->ne : 0;
+  for (int c = 0; c < 3; ++c)
+    B->en[c] = ctxs[0] ? ctxs[0]->en[c] : 0;
   B->deg3      = ctxs[0] && ctxs[0]->k == 3;
   for (int i = 0; i < n; ++i)
     {
@@ -729,7 +732,7 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
 {
   if (!B || !x_cell || !x_trace || !csr_vals || !rhs || !dir_vals)
     return fail("hdg_batch_condense: null argument");
-  if (thermodyn && (B->ne != 1 || !B->ctx[0]->en[0]))
+  if (thermodyn && (B->ne != 1 || !B->en[0]))
     return fail("hdg_batch_condense: thermodynamic equilibrium runs with V only", -2);
   if (!thermodyn && B->ne != 3)
     return fail("hdg_batch_condense: the register kernel needs V, n and p or V alone", -2);
@@ -746,6 +749,18 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
     {
       if (!x_cell[i] || !x_trace[i] || !csr_vals[i] || !rhs[i] || !dir_vals[i])
         return fail("hdg_batch_condense: null device pointer");
+      // a context changed after hdg_batch_create would be run with the
+      // batch's component template (out of bounds in its smaller buffers)
+      // or in context 0's precision
+      const hdg_ctx *C = B->ctx[i];
+      if (C->ne != B->ne || C->en[0] != B->en[0] || C->en[1] != B->en[1] || C->en[2] != B->en[2])
+        return fail("hdg_batch_condense: context " + std::to_string(i) +
+                        " changed its enabled components since hdg_batch_create",
+                    -2);
+      if (C->precision != mixed)
+        return fail("hdg_batch_condense: context " + std::to_string(i) +
+                        " has another precision than context 0",
+                    -2);
       if (!default_variant(B->ctx[i], thermodyn))
         return fail("hdg_batch_condense: context " + std::to_string(i) +
                         " uses a non-default flux, tau or phi_linearize; condense it alone",

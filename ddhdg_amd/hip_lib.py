@@ -122,6 +122,23 @@ def _stream():
     return ct.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _dev(t, name, numel, dtype="float64"):
+    """host-side check of a device buffer before its pointer crosses the C
+    ABI: a CUDA tensor, contiguous, of the kernel's element type and at
+    least numel elements (the kernels index it without bounds)"""
+    import torch
+    want = getattr(torch, dtype)
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise HdgError(f"{name}: expected a device tensor")
+    if t.dtype != want:
+        raise HdgError(f"{name}: expected {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise HdgError(f"{name}: expected a contiguous tensor")
+    if t.numel() < numel:
+        raise HdgError(f"{name}: {t.numel()} elements, {numel} needed")
+    return ct.c_void_p(t.data_ptr())
+
+
 class HdgContext:
     """One mesh on the GPU: owns the device tables of the C ABI context."""
 
@@ -212,24 +229,41 @@ class HdgContext:
                                         None if m is None else _dp(m)))
 
     # -- device-pointer calls (torch tensors on the current device)
+    def _state(self, x_cell, x_trace):
+        return (_dev(x_cell, "x_cell", self.ncells * self.NC),
+                _dev(x_trace, "x_trace", self.nfaces * 3 * self.nf))
+
     def condense(self, x_cell, x_trace, thermodyn, vals, rhs, dir_vals):
-        _check(self.L.hdg_condense(self.ctx, _ptr(x_cell), _ptr(x_trace), int(thermodyn),
-                                   _ptr(vals), _ptr(rhs), _ptr(dir_vals), _stream()))
+        xc, xt = self._state(x_cell, x_trace)
+        _check(self.L.hdg_condense(self.ctx, xc, xt, int(thermodyn),
+                                   _dev(vals, "vals", self.nnz), _dev(rhs, "rhs", self.ntrace),
+                                   _dev(dir_vals, "dir_vals", self.ntrace), _stream()))
 
     def reconstruct(self, x_cell, x_trace, dx_trace, thermodyn, dx_cell):
-        _check(self.L.hdg_reconstruct(self.ctx, _ptr(x_cell), _ptr(x_trace), _ptr(dx_trace),
-                                      int(thermodyn), _ptr(dx_cell), _stream()))
+        xc, xt = self._state(x_cell, x_trace)
+        _check(self.L.hdg_reconstruct(self.ctx, xc, xt,
+                                      _dev(dx_trace, "dx_trace", self.nfaces * 3 * self.nf),
+                                      int(thermodyn), _dev(dx_cell, "dx_cell", self.ncells * self.NC),
+                                      _stream()))
 
     def cell_systems(self, x_cell, x_trace, thermodyn, cells, S, g):
-        _check(self.L.hdg_cell_systems(self.ctx, _ptr(x_cell), _ptr(x_trace), int(thermodyn),
-                                       _ptr(cells), int(cells.numel()), _ptr(S), _ptr(g),
+        """condensed blocks of the listed cells (int32 indices in [0, ncells)),
+        S cell-major (T x T each), g (T each)"""
+        xc, xt = self._state(x_cell, x_trace)
+        m = int(cells.numel())
+        cp = _dev(cells, "cells", m, "int32")
+        if m and (int(cells.min()) < 0 or int(cells.max()) >= self.ncells):
+            raise HdgError(f"cells: indices outside [0, {self.ncells})")
+        _check(self.L.hdg_cell_systems(self.ctx, xc, xt, int(thermodyn), cp, m,
+                                       _dev(S, "S", m * self.T * self.T), _dev(g, "g", m * self.T),
                                        _stream()))
 
     def terminal_current(self, x_cell, x_trace, boundary_id, out):
         """out (device, 2): integrals of the numerical fluxes of n and p over
         the faces of boundary_id (dimensionless)."""
-        _check(self.L.hdg_terminal_current(self.ctx, _ptr(x_cell), _ptr(x_trace),
-                                           int(boundary_id), _ptr(out), _stream()))
+        xc, xt = self._state(x_cell, x_trace)
+        _check(self.L.hdg_terminal_current(self.ctx, xc, xt, int(boundary_id),
+                                           _dev(out, "out", 2), _stream()))
 
 
 class TraceSolver:
@@ -247,6 +281,7 @@ class TraceSolver:
         self.levels, self.max_front = int(info[HDG_TS_LEVELS]), int(info[HDG_TS_MAX_FRONT])
         self.nodes = int(info[HDG_TS_NODES])
         self._ctx = ctx
+        self.nnz, self.ntrace = ctx.nnz, ctx.ntrace
 
     def close(self):
         if self.h:
@@ -260,12 +295,13 @@ class TraceSolver:
             pass
 
     def factor(self, vals, check=False):
-        _check(self.L.hdg_trace_factor(self.h, _ptr(vals), _stream()))
+        _check(self.L.hdg_trace_factor(self.h, _dev(vals, "vals", self.nnz), _stream()))
         if check:
             _check(self.L.hdg_trace_factor_status(self.h, _stream()))
 
     def solve(self, rhs, x):
-        _check(self.L.hdg_trace_solve(self.h, _ptr(rhs), _ptr(x), _stream()))
+        _check(self.L.hdg_trace_solve(self.h, _dev(rhs, "rhs", self.ntrace),
+                                      _dev(x, "x", self.ntrace), _stream()))
 
 
 def trace_plan_selftest(dim, n_cells, dofs_per_face, seed=1, csr_vals=None):
@@ -308,9 +344,18 @@ class HdgBatch:
 
     def condense(self, thermodyn, x_cells, x_traces, vals, rhs, dirv):
         n = len(self.ctxs)
-        arr = lambda ts: (ct.c_void_p * n)(*[t.data_ptr() for t in ts])
-        _check(self.L.hdg_batch_condense(self.h, int(thermodyn), arr(x_cells), arr(x_traces),
-                                         arr(vals), arr(rhs), arr(dirv), _stream()))
+        for lst in (x_cells, x_traces, vals, rhs, dirv):
+            if len(lst) != n:
+                raise HdgError(f"batch of {n} contexts, {len(lst)} buffers")
+
+        def arr(ts, name, size):
+            return (ct.c_void_p * n)(*[_dev(t, f"{name}[{i}]", size(c)).value
+                                       for i, (t, c) in enumerate(zip(ts, self.ctxs))])
+        _check(self.L.hdg_batch_condense(
+            self.h, int(thermodyn), arr(x_cells, "x_cell", lambda c: c.ncells * c.NC),
+            arr(x_traces, "x_trace", lambda c: c.nfaces * 3 * c.nf),
+            arr(vals, "vals", lambda c: c.nnz), arr(rhs, "rhs", lambda c: c.ntrace),
+            arr(dirv, "dir_vals", lambda c: c.ntrace), _stream()))
 
     def info(self):
         out = np.zeros(6)
@@ -321,6 +366,10 @@ class HdgBatch:
 
 def max_abs4(vectors, scratch, out):
     """out[v] = max |vectors[v]| (four device tensors; scratch: 1024 doubles)"""
-    arr = (ct.c_void_p * 4)(*[t.data_ptr() for t in vectors])
+    if len(vectors) != 4:
+        raise HdgError("max_abs4 takes four vectors")
+    arr = (ct.c_void_p * 4)(*[_dev(t, f"vectors[{i}]", t.numel()).value
+                               for i, t in enumerate(vectors)])
     lens = (ct.c_longlong * 4)(*[t.numel() for t in vectors])
-    _check(lib().hdg_max_abs4(arr, lens, _ptr(scratch), _ptr(out), _stream()))
+    _check(lib().hdg_max_abs4(arr, lens, _dev(scratch, "scratch", 1024), _dev(out, "out", 4),
+                              _stream()))

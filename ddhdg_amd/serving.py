@@ -45,6 +45,7 @@ class IVRequest:
     na: float = 1e23
     junction: float = 0.5
     arrival: float = 0.0          # seconds after the server starts
+    max_it: int = 0               # Newton iteration limit per stage (0: the solver's)
 
 
 @dataclass
@@ -58,6 +59,7 @@ class IVResult:
     admitted: float
     finished: float
     state: tuple = field(default=None, repr=False)
+    error: str = None             # why a failed request failed (None when converged)
 
     @property
     def latency(self):
@@ -102,12 +104,14 @@ class _Slot:
             s.ctx.set_precision(precision)
             s.set_enabled_components(True, False, False)
             s.compute_local_charge_neutrality()
+        self.max_it = int(req.max_it) or None
         self.stage = 0                        # 0 eq, 1 n/p, 2 coupled, 3.. biases
         self.iterations, self.currents = [], []
         self.result = None
         self.admitted = now
         self.t0 = t0
         self.ok = True
+        self.error = None
         self.done = torch.cuda.Event()
         self.inflight = False
         self._start_stage()
@@ -131,8 +135,22 @@ class _Slot:
                 self.it.reset()
             else:
                 self.it = NewtonIteration(s, self.stage == 0, nl.absolute_tolerance,
-                                          nl.relative_tolerance, nl.max_number_of_iterations,
+                                          nl.relative_tolerance,
+                                          self.max_it or nl.max_number_of_iterations,
                                           graph=self.graph)
+
+    def close(self):
+        """release the request's device memory: trace solvers, graph,
+        context"""
+        s = getattr(self, "solver", None)
+        if s is None:
+            return
+        for ts in s._solver_cache.values():
+            ts.close()
+        s._solver_cache.clear()
+        self.it = None
+        s.ctx.close()
+        self.solver = None
 
     def launch(self):
         with self.torch.cuda.stream(self.stream):
@@ -156,6 +174,7 @@ class _Slot:
         self.iterations.append(self.it.step)
         if status == "failed":
             self.ok = False
+            self.error = f"Newton did not converge in stage {self.stage}"
         elif self.stage >= 3:
             with self.torch.cuda.stream(self.stream):
                 self.currents.append(self.solver.terminal_current(1))
@@ -181,7 +200,13 @@ class IVServer:
 
     def run(self, requests, clock=None):
         """serve requests (arrival times in seconds from the start); returns
-        the results in request order and the wall time"""
+        the results in request order and the wall time.
+
+        A request that raises (its set-up, a launch or a stage change) or
+        whose Newton solve fails is reported failed, with the reason, and
+        evicted: its device memory and slot are released and the others go
+        on unchanged (they share no buffers with it).  A request whose memory
+        estimate exceeds the whole budget is rejected at admission."""
         torch = self.torch
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
         active, done, used = [], {}, 0
@@ -189,37 +214,70 @@ class IVServer:
         self.max_active, self.ticks = 0, 0
         t0 = time.perf_counter() if clock is None else clock()
         now = lambda: (time.perf_counter() if clock is None else clock()) - t0
+
+        def failed(req, admitted, err):
+            done[req.rid] = IVResult(req.rid, list(req.biases), [], [], False, req.arrival,
+                                     admitted, now(), None, err)
+
         while pending or active:
             # admission: arrived, fits the budget, a slot is free
             while pending and len(active) < self.max_slots and pending[0].arrival <= now():
                 need = memory_estimate(pending[0])
-                if active and used + need > self.budget:
+                if need > self.budget:
+                    req = pending.pop(0)
+                    failed(req, now(), f"needs {need / 2**30:.1f} GiB, budget "
+                                       f"{self.budget / 2**30:.1f} GiB")
+                    continue
+                if used + need > self.budget:
                     break
                 req = pending.pop(0)
+                try:
+                    slot = _Slot(req, t0, now(), torch, self.graph, self.precision)
+                except Exception as e:          # noqa: BLE001 - isolate the request
+                    failed(req, now(), f"{type(e).__name__}: {e}")
+                    continue
                 used += need
-                active.append((_Slot(req, t0, now(), torch, self.graph, self.precision), need))
+                active.append((slot, need))
             if not active:
-                wait = pending[0].arrival - now()
-                if wait > 0:
-                    time.sleep(min(wait, 0.05))
+                if pending:
+                    wait = pending[0].arrival - now()
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
                 continue
             self.max_active = max(self.max_active, len(active))
+            ready = []
             for slot, _ in active:
-                if not slot.inflight:
-                    slot.launch()
-                    self.ticks += 1
-            ready = [slot.poll() for slot, _ in active]
-            if not any(ready):
+                try:
+                    if not slot.inflight:
+                        slot.launch()
+                        self.ticks += 1
+                    ready.append(slot.poll())
+                except Exception as e:          # noqa: BLE001
+                    slot.ok, slot.error = False, f"{type(e).__name__}: {e}"
+                    ready.append(None)
+            if not any(r is not False for r in ready):
                 time.sleep(5e-5)
                 continue
             t = now()
             keep = []
             for (slot, need), fin in zip(active, ready):
-                if fin and slot.advance(t):
+                finished = fin is None
+                if fin:
+                    try:
+                        finished = slot.advance(t)
+                    except Exception as e:      # noqa: BLE001
+                        slot.ok, slot.error = False, f"{type(e).__name__}: {e}"
+                        finished = True
+                if finished:
                     r = slot.req
-                    state = slot.solver.get_state() if self.keep_state else None
+                    state = None
+                    if self.keep_state and slot.ok:
+                        state = slot.solver.get_state()
                     done[r.rid] = IVResult(r.rid, list(r.biases), slot.currents, slot.iterations,
-                                           slot.ok, r.arrival, slot.admitted, t, state)
+                                           slot.ok, r.arrival, slot.admitted, t, state,
+                                           slot.error)
+                    slot.stream.synchronize()
+                    slot.close()
                     used -= need
                 else:
                     keep.append((slot, need))
@@ -264,3 +322,39 @@ def synthetic_requests(n, seed, rate, sizes=(32, 96), degrees=(1, 2), points=13,
             nd=float(10 ** rng.uniform(21, 24)), na=float(10 ** rng.uniform(21, 24)),
             junction=float(rng.uniform(0.3, 0.7)), arrival=t))
     return out
+
+
+def lognormal_side(rng, lo, hi):
+    """a mesh side lognormal between lo and hi: log-side normal with its mean
+    at the middle of [log lo, log hi] and its standard deviation a quarter of
+    that interval (so 95 % of the draws fall inside), redrawn outside it"""
+    mu, sd = 0.5 * (np.log(lo) + np.log(hi)), 0.25 * (np.log(hi) - np.log(lo))
+    while True:
+        side = int(round(np.exp(rng.normal(mu, sd))))
+        if lo <= side <= hi:
+            return side
+
+
+def served_requests(n=2000, seed=20261204, rate=4.0, frac3d=0.2, sides2d=(32, 256),
+                    sides3d=(16, 40), degrees=(1, 2), points=13, vmax=0.6):
+    """the served workload: n I-V sweep requests with Poisson arrivals at
+    `rate` per second; 2D meshes with sides lognormal in sides2d (32^2 to
+    256^2 cells) or, for a fraction frac3d, 3D cubes with sides lognormal in
+    sides3d (16^3 to 40^3); degree uniform in `degrees`; donor and acceptor
+    densities log-uniform in [1e21, 1e24] m^-3 and a junction uniform in
+    [0.3, 0.7] um; each a `points`-point continuation over [0, vmax] V (the
+    coupled solve at 0 V, then points - 1 bias points)"""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    biases = tuple(float(b) for b in np.linspace(0.0, vmax, points)[1:])
+    for i in range(n):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        dim = 3 if rng.uniform() < frac3d else 2
+        side = lognormal_side(rng, *(sides3d if dim == 3 else sides2d))
+        degree = int(rng.choice(degrees))
+        out.append(IVRequest(
+            rid=i, dim=dim, n_cells=(side,) * dim, degree=degree, biases=biases,
+            nd=float(10 ** rng.uniform(21, 24)), na=float(10 ** rng.uniform(21, 24)),
+            junction=float(rng.uniform(0.3, 0.7)), arrival=t))
+    return out
+

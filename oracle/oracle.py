@@ -45,10 +45,19 @@ class OrcProblem(ct.Structure):
     ]
 
 
-def lib():
+def use_library(path):
+    """load the restatement from another build of hdg_oracle.cpp (the CPU
+    baseline builds one for the host it runs on); Oracle objects created
+    afterwards use it"""
+    global _LIB
+    _LIB = None
+    return lib(path)
+
+
+def lib(path=None):
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = path or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: build it with make -C {_HERE}")
         L = ct.CDLL(path)

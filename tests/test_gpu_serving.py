@@ -91,3 +91,31 @@ def test_3d_requests_same_alone_and_in_mix():
         alone, _ = IVServer(max_slots=1, keep_state=True).run(
             [IVRequest(**{**req.__dict__, "arrival": 0.0})])
         _same(r, alone[0])
+
+
+def test_failed_requests_are_evicted_and_isolated():
+    """one request stopped by an iteration limit of 1, one whose set-up
+    raises (degree 4 is not supported), one larger than the whole budget:
+    each is reported failed with its reason, and the others finish bitwise
+    the same as alone"""
+    _torch()
+    print("case serving failure isolation", flush=True)
+    bad = [IVRequest(10, 2, (12, 12), 1, (0.1,), arrival=0.0, max_it=1),
+           IVRequest(11, 2, (8, 8), 4, (0.1,), arrival=0.01),
+           IVRequest(12, 2, (4096, 4096), 2, (0.1,), arrival=0.02)]
+    good = [REQS[0], REQS[1]]
+    budget = 8 << 30
+    assert memory_estimate(bad[2]) > budget
+    mix, _ = IVServer(hbm_budget_bytes=budget, max_slots=4, keep_state=True).run(good + bad)
+    by_id = {r.rid: r for r in mix}
+    for rid in (10, 11, 12):
+        r = by_id[rid]
+        print(f"  request {rid}: failed with {r.error!r}")
+        assert not r.converged and r.error
+    assert "did not converge" in by_id[10].error
+    assert "not supported" in by_id[11].error
+    assert "budget" in by_id[12].error
+    for req in good:
+        alone, _ = IVServer(max_slots=1, keep_state=True).run(
+            [IVRequest(**{**req.__dict__, "arrival": 0.0})])
+        _same(by_id[req.rid], alone[0])

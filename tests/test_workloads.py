@@ -1,0 +1,41 @@
+"""The workload generators (CPU): the batched device set and the served
+request stream follow their stated distributions."""
+import numpy as np
+
+from ddhdg_amd import configs
+from ddhdg_amd.serving import served_requests
+
+
+def test_batched_devices_seeded_and_in_range():
+    a = configs.batched_devices(64, 20261203)
+    b = configs.batched_devices(64, 20261203)
+    sides = [p.mesh.n[0] for p, _, _ in a]
+    degs = [pa.V_degree for _, pa, _ in a]
+    assert sides == [p.mesh.n[0] for p, _, _ in b]
+    assert min(sides) >= 16 and max(sides) <= 192
+    assert set(degs) == {1, 2, 3}
+    assert sum(p.mesh.ncells for p, _, _ in a) == 925965
+
+
+def test_served_requests_distribution():
+    """2,000 requests: Poisson arrivals at the stated rate, 20 % 3D, sides
+    lognormal inside [32, 256] (2D) and [16, 40] (3D), degree 1-2, doping
+    log-uniform in [1e21, 1e24], 13-point sweeps to 0.6 V"""
+    reqs = served_requests(2000, 20261204, rate=4.0)
+    arr = np.array([r.arrival for r in reqs])
+    gaps = np.diff(np.concatenate([[0.0], arr]))
+    assert np.all(gaps >= 0) and abs(gaps.mean() - 0.25) < 0.02
+    assert abs(gaps.std() / gaps.mean() - 1.0) < 0.1        # exponential: CV 1
+    frac3d = np.mean([r.dim == 3 for r in reqs])
+    assert abs(frac3d - 0.2) < 0.03
+    for dim, (lo, hi) in ((2, (32, 256)), (3, (16, 40))):
+        s = np.array([r.n_cells[0] for r in reqs if r.dim == dim])
+        assert s.min() >= lo and s.max() <= hi
+        ls = np.log(s)
+        assert abs(np.median(ls) - 0.5 * np.log(lo * hi)) < 0.1
+        assert abs(ls.std() - 0.25 * np.log(hi / lo)) < 0.1 * np.log(hi / lo)
+    assert {r.degree for r in reqs} == {1, 2}
+    dop = np.log10([r.nd for r in reqs] + [r.na for r in reqs])
+    assert dop.min() >= 21 and dop.max() <= 24 and abs(dop.mean() - 22.5) < 0.1
+    assert all(len(r.biases) == 12 and abs(r.biases[-1] - 0.6) < 1e-12 for r in reqs)
+    assert all(0.3 <= r.junction <= 0.7 for r in reqs)
