@@ -845,6 +845,18 @@ int hdg_max_abs4(const double *const *vectors, const long long *lengths, double 
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_max_abs4 launch");
 }
 
+int hdg_max_abs4_noalloc(const double *const *vectors, const long long *lengths, double *out,
+                         void *stream)
+{
+  if (!vectors || !lengths || !out)
+    return fail("hdg_max_abs4_noalloc: null argument");
+  for (int v = 0; v < 4; ++v)
+    if (!vectors[v] || lengths[v] < 0)
+      return fail("hdg_max_abs4_noalloc: invalid vector");
+  hipError_t e = hdg::launch_norms_static(vectors, lengths, out, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : hip_fail(e, "hdg_max_abs4_noalloc launch");
+}
+
 int hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face, unsigned seed,
                             const double *csr_vals, double out[3])
 {

@@ -70,6 +70,10 @@ namespace hdg
   // out[v] = max |a[v]_k| (device); scratch: 4*256 doubles
   hipError_t launch_norms(const double *const a[4], const long long n[4], double *scratch,
                           double *out, hipStream_t s);
+  // the same with the partials in a static device array (allocates nothing;
+  // concurrent calls on one GPU race)
+  hipError_t launch_norms_static(const double *const a[4], const long long n[4], double *out,
+                                 hipStream_t s);
   // terminal current through boundary id bid: out[0] = integral of the
   // numerical flux of n, out[1] of p (device, dimensionless)
   hipError_t launch_current(int dim, int nb, int nf, int qf, const KParams &p, int bid,

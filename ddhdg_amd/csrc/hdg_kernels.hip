@@ -27,6 +27,7 @@
 // elsewhere (template.h:983-1009, 1249-1263, 1417-1435, 1523-1536,
 // 1865-1903, 2036-2071).  tau is per face: FixedTauComputer or
 // CellFaceTauComputer (include/np_solver_parameters.h:92-115, 157-236).
+#include <mutex>
 #include <algorithm>
 #include <utility>
 
@@ -1289,6 +1290,55 @@ namespace hdg
                        a[2], n[2], a[3], n[3], scratch);
     hipLaunchKernelGGL(norms_final, dim3(1), dim3(NRM_T), 0, s, scratch, out);
     return hipGetLastError();
+  }
+
+  // Partial maxima of launch_norms_static: a file-scope device array, so the
+  // entry point that uses it allocates nothing.  Every GPU holds its own copy
+  // until the process ends; calls that overlap on one GPU share it and race.
+  // Each call's first kernel overwrites every partial it later reads, so
+  // nothing has to be cleared between calls.
+  constexpr int NRM_SCRATCH = 4 * NRM_B;
+  static __device__ __align__(256) double scratch[NRM_SCRATCH];
+
+  namespace
+  {
+    constexpr int MAX_DEVICES = 64;
+    std::mutex    scratch_mutex;
+    double       *scratch_ptr[MAX_DEVICES] = {};
+  } // namespace
+
+  hipError_t norms_static_scratch(double **ptr)
+  {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+      return e;
+    if (dev < 0 || dev >= MAX_DEVICES)
+      return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev])
+      {
+        // the device address of this device's copy: the array's name is a
+        // host-side stand-in and must never reach a kernel
+        void *p = nullptr;
+        e       = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess)
+          return e;
+        scratch_ptr[dev] = static_cast<double *>(p);
+      }
+    *ptr = scratch_ptr[dev];
+    return hipSuccess;
+  }
+
+  hipError_t launch_norms_static(const double *const a[4], const long long n[4], double *out,
+                                 hipStream_t s)
+  {
+    double    *part = nullptr;
+    hipError_t e    = norms_static_scratch(&part);
+    if (e != hipSuccess)
+      return e;
+    static_assert(4 * NRM_B <= NRM_SCRATCH, "partials exceed the static scratch");
+    return launch_norms(a, n, part, out, s);
   }
 
   hipError_t launch_current(int dim, int nb, int nf, int qf, const KParams &p, int bid,

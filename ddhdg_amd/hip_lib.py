@@ -29,7 +29,7 @@ EXPORTS = [
     "hdg_trace_factor", "hdg_trace_factor_status", "hdg_trace_solve",
     "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
     "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_condense", "hdg_batch_info",
-    "hdg_max_abs4", "hdg_set_cell_face_tau",
+    "hdg_max_abs4", "hdg_set_cell_face_tau", "hdg_max_abs4_noalloc",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -93,6 +93,7 @@ def lib():
             "hdg_batch_condense": (i, [v, i, P(v), P(v), P(v), P(v), P(v), v]),
             "hdg_batch_info": (i, [v, dp]),
             "hdg_max_abs4": (i, [P(v), lp, v, v, v]),
+            "hdg_max_abs4_noalloc": (i, [P(v), lp, v, v]),
             "hdg_set_cell_face_tau": (i, [v, dp]),
         }
         for name, (res, args) in sigs.items():
@@ -373,3 +374,15 @@ def max_abs4(vectors, scratch, out):
     lens = (ct.c_longlong * 4)(*[t.numel() for t in vectors])
     _check(lib().hdg_max_abs4(arr, lens, _dev(scratch, "scratch", 1024), _dev(out, "out", 4),
                               _stream()))
+
+
+def max_abs4_noalloc(vectors, out):
+    """out[v] = max |vectors[v]| with the library's static scratch (no
+    allocation; concurrent calls on one GPU race)"""
+    if len(vectors) != 4:
+        raise HdgError("max_abs4 takes four vectors")
+    arr = (ct.c_void_p * 4)(*[_dev(t, f"vectors[{i}]", t.numel()).value
+                               for i, t in enumerate(vectors)])
+    lens = (ct.c_longlong * 4)(*[t.numel() for t in vectors])
+    _check(lib().hdg_max_abs4_noalloc(arr, lens, _dev(out, "out", 4), _stream()))
+

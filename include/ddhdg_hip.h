@@ -218,6 +218,17 @@ int hdg_terminal_current(hdg_ctx *ctx, const double *x_cell,
 int hdg_max_abs4(const double *const *vectors, const long long *lengths,
                  double *scratch, double *out, void *stream);
 
+/* hdg_max_abs4 without the scratch argument: the partial maxima live in a
+ * static device array of the library (4 x 256 doubles, 8 KiB, one copy per
+ * GPU, held until the process ends), so the call allocates nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops).  The scratch is fixed
+ * in size, so vectors of any length are accepted; a negative length is an
+ * error.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (different streams or threads) race. */
+int hdg_max_abs4_noalloc(const double *const *vectors, const long long *lengths,
+                         double *out, void *stream);
+
 /* ---- Batched pass 1 over several meshes --------------------------------
  * Pass 1 (hdg_condense) of several contexts (2D with the same enabled
  * components: degree 1 and 2 mixed, or all of degree 3 with V, n and p) in
