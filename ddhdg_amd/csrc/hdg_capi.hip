@@ -857,6 +857,32 @@ int hdg_max_abs4_noalloc(const double *const *vectors, const long long *lengths,
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_max_abs4_noalloc launch");
 }
 
+int hdg_trace_plan_size(int dim, const int *n_cells, int dofs_per_face, double out[6])
+{
+  if (!n_cells || !out || (dim != 2 && dim != 3) || dofs_per_face < 1)
+    return fail("hdg_trace_plan_size: invalid argument");
+  int n[3] = {1, 1, 1};
+  for (int a = 0; a < dim; ++a)
+    {
+      if (n_cells[a] < 1)
+        return fail("hdg_trace_plan_size: invalid mesh", -2);
+      n[a] = n_cells[a];
+    }
+  hdg::Topology T;
+  hdg::build_topology(dim, n, T);
+  hdg::NDPlan P;
+  std::string err;
+  if (!hdg::build_nd_plan(T, dofs_per_face, P, err, true))
+    return fail("hdg_trace_plan_size: " + err, -4);
+  out[0] = (double)P.Ftot;
+  out[1] = (double)P.wtot;
+  out[2] = (double)P.ptot;
+  out[3] = P.flops;
+  out[4] = (double)P.max_front;
+  out[5] = (double)P.nnodes;
+  return 0;
+}
+
 int hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face, unsigned seed,
                             const double *csr_vals, double out[3])
 {

@@ -1235,6 +1235,13 @@ namespace hdg
     // workgroup (max is exact in any order: deterministic)
     constexpr int NRM_T = 256, NRM_B = 256;
 
+    // max that propagates NaN (fmax drops it): a NaN update must not read
+    // as a converged Newton step
+    __device__ __forceinline__ double nan_max(double a, double b)
+    {
+      return (a > b || a != a) ? a : b;
+    }
+
     __global__ void __launch_bounds__(NRM_T)
       norms_partial(const double *a0, long long n0, const double *a1, long long n1,
                     const double *a2, long long n2, const double *a3, long long n3,
@@ -1248,13 +1255,13 @@ namespace hdg
           double m = 0;
           for (long long i = blockIdx.x * (long long)NRM_T + threadIdx.x; i < n[v];
                i += (long long)gridDim.x * NRM_T)
-            m = fmax(m, fabs(a[v][i]));
+            m = nan_max(m, fabs(a[v][i]));
           red[threadIdx.x] = m;
           __syncthreads();
           for (int off = NRM_T / 2; off > 0; off >>= 1)
             {
               if ((int)threadIdx.x < off)
-                red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+                red[threadIdx.x] = nan_max(red[threadIdx.x], red[threadIdx.x + off]);
               __syncthreads();
             }
           if (threadIdx.x == 0)
@@ -1273,7 +1280,7 @@ namespace hdg
           for (int off = NRM_T / 2; off > 0; off >>= 1)
             {
               if ((int)threadIdx.x < off)
-                red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+                red[threadIdx.x] = nan_max(red[threadIdx.x], red[threadIdx.x + off]);
               __syncthreads();
             }
           if (threadIdx.x == 0)

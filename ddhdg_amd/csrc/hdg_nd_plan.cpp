@@ -66,7 +66,7 @@ namespace hdg
     }
   } // namespace
 
-  bool build_nd_plan(const Topology &T, int tdf, NDPlan &P, std::string &err)
+  bool build_nd_plan(const Topology &T, int tdf, NDPlan &P, std::string &err, bool sizes_only)
   {
     P        = NDPlan();
     P.tdf    = tdf;
@@ -194,6 +194,8 @@ namespace hdg
         err = "nested dissection: front too large";
         return false;
       }
+    if (sizes_only)
+      return true;
 
     std::vector<std::unordered_map<long long, int>> intf_pos(nodes.size());
     for (int id = 0; id < (int)nodes.size(); ++id)

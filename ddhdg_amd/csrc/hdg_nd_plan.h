@@ -58,7 +58,10 @@ namespace hdg
   // builds the plan for the trace system with tdf dofs per face (trace CSR
   // in the face-block order of the topology's block pattern); false and a
   // message on failure
-  bool build_nd_plan(const Topology &topo, int tdf, NDPlan &P, std::string &err);
+  // sizes_only: stop once the groups' shapes are known (Ftot, wtot, ptot,
+  // flops, max_front, nnodes): no index maps, so it is cheap at any size
+  bool build_nd_plan(const Topology &topo, int tdf, NDPlan &P, std::string &err,
+                     bool sizes_only = false);
 
   // every index against its buffer, each front entry written by at most one
   // matrix entry, every trace dof eliminated exactly once

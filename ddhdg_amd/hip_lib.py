@@ -29,7 +29,7 @@ EXPORTS = [
     "hdg_trace_factor", "hdg_trace_factor_status", "hdg_trace_solve",
     "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
     "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_condense", "hdg_batch_info",
-    "hdg_max_abs4", "hdg_set_cell_face_tau", "hdg_max_abs4_noalloc",
+    "hdg_max_abs4", "hdg_set_cell_face_tau", "hdg_max_abs4_noalloc", "hdg_trace_plan_size",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -94,6 +94,7 @@ def lib():
             "hdg_batch_info": (i, [v, dp]),
             "hdg_max_abs4": (i, [P(v), lp, v, v, v]),
             "hdg_max_abs4_noalloc": (i, [P(v), lp, v, v]),
+            "hdg_trace_plan_size": (i, [i, ip, i, dp]),
             "hdg_set_cell_face_tau": (i, [v, dp]),
         }
         for name, (res, args) in sigs.items():
@@ -318,6 +319,17 @@ def trace_plan_selftest(dim, n_cells, dofs_per_face, seed=1, csr_vals=None):
     _check(lib().hdg_trace_plan_selftest(int(dim), n, int(dofs_per_face), int(seed),
                                          _dp(vals) if vals is not None else None, _dp(out)))
     return float(out[0]), int(out[1]), int(out[2])
+
+
+def trace_plan_size(dim, n_cells, dofs_per_face):
+    """host-only sizes of the trace solver's plan (no GPU): front, work and
+    pivot entries, factorisation FLOPs, largest front, tree nodes"""
+    n = (ct.c_int * 3)(*(list(n_cells) + [1] * (3 - len(n_cells))))
+    out = np.zeros(6)
+    _check(lib().hdg_trace_plan_size(int(dim), n, int(dofs_per_face), _dp(out)))
+    return {"front_entries": int(out[0]), "work_entries": int(out[1]),
+            "pivot_entries": int(out[2]), "flops": float(out[3]), "max_front": int(out[4]),
+            "nodes": int(out[5])}
 
 
 class HdgBatch:

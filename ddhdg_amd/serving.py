@@ -25,6 +25,7 @@ results are the same alone and in any mix.
 """
 from __future__ import annotations
 
+import functools
 import time
 from dataclasses import dataclass, field
 
@@ -66,27 +67,38 @@ class IVResult:
         return self.finished - self.arrival
 
 
+@functools.lru_cache(maxsize=256)
+def _trace_solver_bytes(dim, n_cells, tdf):
+    """device bytes of the trace solver of a mesh: fronts, work vectors,
+    pivots, the CSR-to-front map and the groups' index lists, from the plan's
+    exact sizes (built on the host without its maps)"""
+    from .hip_lib import HdgError, trace_plan_size
+    try:
+        z = trace_plan_size(dim, n_cells, tdf)
+    except HdgError:
+        return 1 << 62          # no plan (a front beyond the solver's limits)
+    return 8 * z["front_entries"] + 24 * z["work_entries"] + 4 * z["pivot_entries"]
+
+
 def memory_estimate(req: IVRequest) -> int:
     """device bytes a request holds while it runs: cell and trace state and
-    updates, the trace matrix, the trace solver's fronts (closed form of the
-    nested-dissection plan's size, from the face count), tables."""
+    updates, the trace matrix, the trace solver of the full V/n/p system (the
+    largest of its stages; it holds one at a time), the generic kernel's
+    scratch where it needs one, tables"""
     k, d = req.degree, req.dim
-    n = list(req.n_cells) + [1] * (3 - len(req.n_cells))
+    n = tuple(list(req.n_cells) + [1] * (3 - len(req.n_cells)))
     ncells = int(np.prod(n[:d]))
     nb, nf = (k + 1) ** d, (k + 1) ** (d - 1)
     nfaces = sum(int(np.prod([n[b] + (1 if a == b else 0) for b in range(d)])) for a in range(d))
     NC = 3 * (d + 1) * nb
     tdf = 3 * nf
     nnz = nfaces * (4 * d - 1) * tdf * tdf
-    # fronts: about 11 dense (face-block) entries per trace dof and level in 2D,
-    # bounded above by the measured plans; 3D fronts grow as ndof^(4/3)
-    ndof = nfaces * tdf
-    fronts = 8 * (11 * ndof * tdf * np.log2(max(ncells, 2)) if d == 2 else 4 * ndof ** (4 / 3) * tdf)
+    solver = _trace_solver_bytes(d, n[:d], tdf) + 8 * nnz
     # the generic kernel's global scratch where its reduced systems do not
     # fit LDS (3D degree 2): 512 slices of R
     maxr = 3 * nb + 2 * d * 3 * nf
     scratch = 512 * maxr * (maxr + 2) * 8 if (d == 3 and k >= 2) else 0
-    return int(8 * (4 * ncells * NC + 6 * nfaces * 3 * nf + nnz) + fronts + scratch + (64 << 20))
+    return int(8 * (4 * ncells * NC + 6 * nfaces * 3 * nf + nnz) + solver + scratch + (64 << 20))
 
 
 class _Slot:

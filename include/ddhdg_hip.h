@@ -296,6 +296,15 @@ int  hdg_trace_solve(hdg_trace_solver *solver, const double *rhs, double *x,
 int  hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face,
                              unsigned seed, const double *csr_vals, double out[3]);
 
+/* Host-only sizes of the trace solver's plan for a dim-D n_cells grid with
+ * dofs_per_face trace dofs per face, without building its index maps (cheap
+ * at any size): out = {front entries (doubles of factor storage), work-vector
+ * entries, pivot entries, factorisation FLOPs, largest front, tree nodes}.
+ * The device memory hdg_trace_solver_create would take is about
+ * 8*out[0] + 8*out[1] + 4*out[2] + 8*NNZ bytes.  For admission by memory
+ * budget; no counterpart in the reference. */
+int  hdg_trace_plan_size(int dim, const int *n_cells, int dofs_per_face, double out[6]);
+
 #ifdef __cplusplus
 }
 #endif

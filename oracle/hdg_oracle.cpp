@@ -16,6 +16,7 @@
  *                face-local tangent axes (y), (x) in 2D and (y,z), (z,x),
  *                (x,y) in 3D, first one fastest.
  */
+#include <cstdio>
 #include "hdg_oracle.h"
 
 #include <algorithm>
@@ -299,6 +300,7 @@ struct orc_ctx
   std::vector<double> x_cell, x_trace;
 
   // global restricted system
+  bool                   pattern_ready = false;
   std::vector<long long> rowptr;
   std::vector<int>       colind;
   std::vector<double>    vals, rhs;
@@ -374,6 +376,36 @@ namespace
               C->cell_support_on_face[lf].push_back(i);
           }
       }
+    // the trace CSR pattern is built when first needed (pass 1, the system,
+    // the pattern itself), so per-cell calls on large meshes stay cheap
+    C->pattern_ready = false;
+    std::vector<long long>().swap(C->rowptr);
+    std::vector<int>().swap(C->colind);
+    std::vector<double>().swap(C->vals);
+    std::vector<double>().swap(C->rhs);
+  }
+
+  // restricted CSR entries: a boundary face couples to the faces of its one
+  // cell, an interior face to those of its two cells (one shared)
+  long long csr_entries(const orc_ctx *C)
+  {
+    long long nbnd = 0;
+    for (int a = 0; a < C->d; ++a)
+      {
+        long long m = 2;
+        for (int b = 0; b < C->d; ++b)
+          if (b != a)
+            m *= C->n[b];
+        nbnd += m;
+      }
+    const long long tdf = (long long)C->ne * C->nf;
+    return tdf * tdf * (nbnd * C->nfaces_cell + (C->nfaces - nbnd) * (2 * C->nfaces_cell - 1));
+  }
+
+  void ensure_pattern(orc_ctx *C)
+  {
+    if (C->pattern_ready)
+      return;
     // restricted trace CSR pattern: a face's rows couple to the trace dofs
     // of the cells adjacent to it (DoFTools::make_sparsity_pattern,
     // src/np_solver.cpp:644-652)
@@ -409,6 +441,13 @@ namespace
                   C->colind.begin() + C->rowptr[f * tdf + t]);
     C->vals.assign(C->colind.size(), 0.0);
     C->rhs.assign(nr, 0.0);
+    if ((long long)C->colind.size() != csr_entries(C))
+      {
+        std::fprintf(stderr, "oracle: CSR pattern has %lld entries, %lld expected\n",
+                     (long long)C->colind.size(), csr_entries(C));
+        std::abort();
+      }
+    C->pattern_ready = true;
   }
 
   // SRH / linear models (src/recombination_term.cpp:433-480, 123-253)
@@ -1752,7 +1791,7 @@ void orc_sizes(const orc_ctx *C, long long out[12])
   out[8]  = C->N;
   out[9]  = C->T;
   out[10] = C->nfaces * C->ne * C->nf;
-  out[11] = (long long)C->colind.size();
+  out[11] = csr_entries(C);
 }
 
 int orc_set_enabled(orc_ctx *C, int V, int n, int p)
@@ -1914,12 +1953,14 @@ int orc_condensed_cell(orc_ctx *C, int cell, int thermo, double *S, double *g,
 
 void orc_csr_pattern(const orc_ctx *C, long long *rowptr, int *colind)
 {
+  ensure_pattern(const_cast<orc_ctx *>(C));
   std::copy(C->rowptr.begin(), C->rowptr.end(), rowptr);
   std::copy(C->colind.begin(), C->colind.end(), colind);
 }
 
 double orc_pass1(orc_ctx *C, int thermo, int nthreads, long long limit)
 {
+  ensure_pattern(C);
   std::fill(C->vals.begin(), C->vals.end(), 0.0);
   std::fill(C->rhs.begin(), C->rhs.end(), 0.0);
   C->dir_idx.clear();
@@ -1949,6 +1990,7 @@ double orc_pass1(orc_ctx *C, int thermo, int nthreads, long long limit)
 
 void orc_get_system(const orc_ctx *C, double *vals, double *rhs)
 {
+  ensure_pattern(const_cast<orc_ctx *>(C));
   if (vals)
     std::copy(C->vals.begin(), C->vals.end(), vals);
   if (rhs)

@@ -397,3 +397,51 @@ def test_newton_3d_register_kernel_matches_oracle():
         assert r_gpu.converged and r_orc.converged, name
         assert r_gpu.iterations == r_orc.iterations, name
         assert _close(s, o), name
+
+
+def test_3d_k2_generic_kernel_multiple_rounds():
+    """3D degree 2 on the generic kernel, whose reduced systems live in a
+    global scratch slice per workgroup of a persistent grid of 512: an
+    11x11x10 mesh has about 605 cells per colour, so part of the grid runs a
+    second round on its reused slice.  Cells of the second round against the
+    oracle, the global system against the oracle's, and pass 2 against the
+    oracle's on every cell."""
+    t = _torch()
+    print("case 3D k=2 generic kernel, 11x11x10 (two rounds per colour)", flush=True)
+    s, prob, params, adim = _solver(3, (11, 11, 10), 2)
+    o = H.oracle_for(prob, params, adim)
+    cell, tr = _state(prob, adim, 2, seed=31)
+    s.set_state(cell, tr)
+    o.set_state(cell, tr)
+    ctx = s.ctx
+    # all 1210 cells in one call: the grid of 512 takes list entries 512 and
+    # beyond in its second and third rounds
+    idx = t.arange(ctx.ncells, dtype=t.int32, device="cuda")
+    S = t.empty(ctx.ncells * ctx.T * ctx.T, dtype=t.float64, device="cuda")
+    g = t.empty(ctx.ncells * ctx.T, dtype=t.float64, device="cuda")
+    ctx.cell_systems(s.x_cell, s.x_trace, False, idx, S, g)
+    S = S.cpu().numpy().reshape(-1, ctx.T, ctx.T)
+    g = g.cpu().numpy().reshape(-1, ctx.T)
+    sample = np.concatenate([np.arange(0, 512, 97), np.arange(512, ctx.ncells, 23)])
+    worst = 0.0
+    for c in sample:
+        So, go, *_ = o.condensed_cell(int(c), False)
+        worst = max(worst, H.rel_fro(S[c], So), H.rel_fro(g[c], go))
+    print(f"  {len(sample)} cells, most of later rounds: worst rel {worst:.3e}", flush=True)
+    assert worst < 1e-12
+    A, b, dirs, dv = s.assemble_system()
+    o.pass1(False, 16)
+    Ao, bo, dio, dvo = o.system()
+    keep = np.ones(A.shape[0], dtype=bool)
+    keep[dirs] = False
+    errA = H.rel_fro(A[keep][:, keep].data, Ao[keep][:, keep].data)
+    errb = H.rel_fro(b[keep], bo[keep])
+    print(f"  global system: values {errA:.3e} rhs {errb:.3e}", flush=True)
+    assert errA < 1e-12 and errb < 1e-12
+    rng = np.random.default_rng(6)
+    upd = 1e-2 * rng.standard_normal(s.x_trace.numel())
+    dxc = t.empty_like(s.x_cell)
+    ctx.reconstruct(s.x_cell, s.x_trace, t.as_tensor(upd, device="cuda"), False, dxc)
+    err = H.rel_fro(dxc.cpu().numpy(), o.pass2(upd, False, 16))
+    print(f"  pass 2 rel err {err:.3e}", flush=True)
+    assert err < 1e-11

@@ -58,6 +58,10 @@ def test_noalloc_norms_edge_cases():
     scratch = torch.empty(1024, dtype=torch.float64, device="cuda")
     ref = _call(torch, L, vecs, scratch, lengths=[2, 0, 1, 1])
     assert np.array_equal(got, [2.0, 0.0, 0.5, 7.0]) and np.array_equal(got, ref)
+    # a NaN anywhere makes its norm NaN (a NaN update must not read as converged)
+    vecs[2] = torch.tensor([0.5, float("nan"), 1.0], dtype=torch.float64, device="cuda")
+    got = _call(torch, L, vecs, lengths=[2, 0, 3, 1])
+    assert np.isnan(got[2]) and np.array_equal(got[[0, 1, 3]], [2.0, 0.0, 7.0])
     arr = (ct.c_void_p * 4)(*[t.data_ptr() for t in vecs])
     lens = (ct.c_longlong * 4)(2, -1, 1, 1)
     out = torch.empty(4, dtype=torch.float64, device="cuda")
