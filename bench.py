@@ -226,6 +226,62 @@ def sample_3d(args, torch, stream):
     out["fp32_refine"]["max_rel_err_g_vs_fp64"] = float(((gm - gd).norm(dim=1) /
                                                          gd.norm(dim=1).clamp_min(1e-300)).max())
     out["fp32_refine"]["vs_fp64"] = out["fp32_refine"]["value"] / out["fp64"]["value"]
+    # pass 2 at the same size (the generic kernel's reconstruct)
+    dxt = torch.full((ctx.nfaces * 3 * ctx.nf,), 1e-3, dtype=torch.float64, device="cuda")
+    dxc = torch.empty_like(s.x_cell)
+    out["pass2_ms"] = event_ms(torch, stream, lambda: ctx.reconstruct(s.x_cell, s.x_trace, dxt,
+                                                                     False, dxc), reps)
+    # the first Newton stage at full size: the thermodynamic equilibrium (V
+    # only; its trace factors take 63 GB, a coupled step's would take 567 GB)
+    s.set_enabled_components(True, False, False)
+    out["newton_equilibrium"] = newton_phases(torch, stream, s, True, 2)
+    s.set_enabled_components(True, True, True)
+    del s
+    # a coupled Newton step of the same device at 24^3
+    prob, params, adim = configs.diode(3, 24, 1, doping=configs.gaussian_doping())
+    s = NPSolver(prob, params, adim, device="cuda")
+    s.compute_local_charge_neutrality()
+    s.project_densities_from_potential()
+    out["newton_step_24cube"] = newton_phases(torch, stream, s, False, 2)
+    return out
+
+
+def event_ms(torch, stream, fn, reps):
+    """average device time of fn over reps calls (HIP events on the stream)"""
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def newton_phases(torch, stream, s, thermodyn, reps):
+    """per-phase device times of a whole Newton step of solver s (its
+    current enabled components): pass 1, trace factorisation, solve,
+    pass 2; the step itself is not applied"""
+    ctx = s.ctx
+    ts = s.trace_solver()
+    rp, ci, dirs, r2f = s._pattern()
+    vals = torch.empty(ctx.nnz, dtype=torch.float64, device="cuda")
+    rhs = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
+    dirv = torch.zeros(ctx.ntrace, dtype=torch.float64, device="cuda")
+    sol = torch.empty_like(rhs)
+    dxt = torch.zeros(ctx.nfaces * 3 * ctx.nf, dtype=torch.float64, device="cuda")
+    dxc = torch.empty_like(s.x_cell)
+    ph = {"condense": event_ms(torch, stream, lambda: ctx.condense(s.x_cell, s.x_trace, thermodyn,
+                                                                  vals, rhs, dirv), reps),
+          "factor": event_ms(torch, stream, lambda: ts.factor(vals), reps),
+          "solve": event_ms(torch, stream, lambda: ts.solve(rhs, sol), reps)}
+    dxt[r2f] = sol
+    ph["reconstruct"] = event_ms(torch, stream, lambda: ctx.reconstruct(
+        s.x_cell, s.x_trace, dxt, thermodyn, dxc), reps)
+    out = {"ms": sum(ph.values()), "phase_ms": ph, "trace_dofs": ctx.ntrace,
+           "max_front": ts.max_front, "front_GB": ts.bytes / 1e9, "factor_GFLOP": ts.flops / 1e9,
+           "factor_TFLOPs": ts.flops / (ph["factor"] * 1e-3) / 1e12}
+    s._solver_cache.clear()
     return out
 
 
@@ -298,9 +354,10 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="devices condensed together beside the metric (0: none)")
     ap.add_argument("--serve", type=int, default=16,
-                    help="I-V sweep requests served beside the metric (0: none)")
+                    help="requests of the served stream served beside the metric (0: none)")
     ap.add_argument("--serve-rate", type=float, default=4.0)
     ap.add_argument("--serve-slots", type=int, default=8)
+    ap.add_argument("--serve-budget-gib", type=int, default=200)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB "
                          "(3 launches): the known-size kernel that calibrates the "
@@ -504,64 +561,53 @@ def main():
     if args.batch > 0:
         line["batched"] = batched_sample(args, torch, stream)
     if args.serve > 0:
-        # served I-V sweeps: Poisson arrivals, continuous batching, state kept
-        # on the device across the bias points (a bounded sample of the
-        # serving workload: 2D requests only)
-        from ddhdg_amd.serving import IVServer, summarize, synthetic_requests
-        reqs = synthetic_requests(args.serve, seed=20261204, rate=args.serve_rate,
-                                  sizes=(32, 96), degrees=(1, 2), points=13, vmax=0.6)
-        srv = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots, graph=True)
-        res, swall = srv.run(reqs)
-        eager = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots, graph=False)
-        res_e, wall_e = eager.run(reqs)
-        mixed = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots, graph=True,
-                         precision=1)
-        res_m, wall_m = mixed.run(reqs)
-        keys = ("solves_per_s", "p50_latency_s", "p99_latency_s")
-        # FP32+refine against FP64: largest difference of the terminal
-        # currents of a sweep relative to the sweep's largest current (the
-        # low-bias currents are small differences of O(1) fluxes)
-        dI = max(max(abs(a - b) for a, b in zip(rm.currents, rf.currents)) /
-                 max(max(abs(b) for b in rf.currents), 1e-300) for rf, rm in zip(res, res_m))
-        # capacity: the same requests all arriving at once
-        burst = synthetic_requests(args.serve, seed=20261204, rate=0.0, sizes=(32, 96),
-                                   degrees=(1, 2), points=13, vmax=0.6)
-        res_b, wall_b = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots,
-                                 graph=True).run(burst)
-        # the mix with 20 % 3D requests (cubes of 8^3-12^3 hexahedra, degree
-        # 1 or 2)
-        mix = synthetic_requests(args.serve, seed=20261204, rate=args.serve_rate, sizes=(32, 96),
-                                 degrees=(1, 2), points=13, vmax=0.6, frac3d=0.2,
-                                 sizes3d=(8, 12), degrees3d=(1, 2))
-        res_x, wall_x = IVServer(hbm_budget_bytes=64 << 30, max_slots=args.serve_slots,
-                                 graph=True).run(mix)
-        line["serving"] = {**summarize(res, swall), "max_concurrent": srv.max_active,
-                           "graph_newton": True,
-                           "without_graphs": {k: v for k, v in summarize(res_e, wall_e).items()
-                                              if k in keys},
-                           "fp32_refine": {**{k: v for k, v in summarize(res_m, wall_m).items()
-                                              if k in keys + ("newton_iterations",)},
-                                           "max_current_diff_vs_fp64_rel_to_sweep_max": dI},
-                           "burst": {**{k: v for k, v in summarize(res_b, wall_b).items()
-                                        if k in keys + ("converged",)},
-                                     "workload": "the same requests, all arriving at t = 0"},
-                           "with_3d": {**{k: v for k, v in summarize(res_x, wall_x).items()
-                                          if k in keys + ("converged", "requests")},
-                                       "requests_3d": sum(1 for q in mix if q.dim == 3),
-                                       "requests_3d_degree2": sum(1 for q in mix
-                                                                  if q.dim == 3 and q.degree == 2),
-                                       "not_converged": [
-                                           {"dim": q.dim, "cells": list(q.n_cells),
-                                            "degree": q.degree, "nd": q.nd, "na": q.na}
-                                           for q, r in zip(mix, sorted(res_x, key=lambda r: r.rid))
-                                           if not r.converged],
-                                       "workload": "the same arrivals with 20 % of the requests "
-                                                   "3D cubes of 8^3-12^3 hexahedra, degree 1 or "
-                                                   "2"},
-                           "workload": f"{args.serve} I-V sweeps (13 points 0-0.6 V), 2D, "
-                                       f"{32}^2-{96}^2 cells log-uniform, degree 1-2, Poisson "
-                                       f"arrivals {args.serve_rate}/s, FP64",
-                           "unit": "I-V solves/s"}
+        # the served workload: the first requests of the seeded stream
+        # (Poisson arrivals, lognormal 2D meshes 32^2-256^2, 20 % 3D cubes
+        # 16^3-40^3, degree 1-2, 13-point sweeps), continuous batching, state
+        # kept on the device across the bias points, admission by HBM budget
+        # (requests larger than the whole budget are refused)
+        from ddhdg_amd.serving import IVServer, memory_estimate, served_requests, summarize
+        budget = int(args.serve_budget_gib) << 30
+        reqs = served_requests(args.serve, seed=20261204, rate=args.serve_rate)
+        runs = {}
+        for prec in (0, 1):
+            srv = IVServer(hbm_budget_bytes=budget, max_slots=args.serve_slots, graph=True,
+                           precision=prec)
+            res, wall = srv.run(reqs)
+            runs[prec] = (srv, res, wall)
+        keys = ("solves_per_s", "p50_latency_s", "p99_latency_s", "converged", "requests",
+                "wall_s", "newton_iterations")
+
+        def stats(srv, res, wall):
+            admitted = [r for r in res if not (r.error and "budget" in r.error)]
+            out = {k: v for k, v in summarize(admitted, wall).items() if k in keys}
+            out["refused_over_budget"] = len(res) - len(admitted)
+            out["max_concurrent"] = srv.max_active
+            return out
+        s64, r64, w64 = runs[0]
+        s32, r32, w32 = runs[1]
+        # FP32+refine against FP64: the largest difference of a sweep's
+        # terminal currents relative to the sweep's largest current
+        dI = max((max(abs(a - b) for a, b in zip(rm.currents, rf.currents)) /
+                  max(max(abs(b) for b in rf.currents), 1e-300))
+                 for rf, rm in zip(r64, r32) if rf.converged and rm.converged)
+        same_its = all(rf.iterations == rm.iterations for rf, rm in zip(r64, r32))
+        line["serving"] = {
+            **stats(s64, r64, w64), "precision": "fp64", "graph_newton": True,
+            "fp32_refine": {**stats(s32, r32, w32),
+                            "max_current_diff_vs_fp64_rel_to_sweep_max": dI,
+                            "same_iteration_counts_as_fp64": same_its},
+            "requests_detail": [{"dim": q.dim, "side": q.n_cells[0], "degree": q.degree,
+                                 "memory_GiB": round(memory_estimate(q) / 2**30, 2),
+                                 "converged": r.converged, "latency_s": round(r.latency, 3),
+                                 "error": r.error}
+                                for q, r in zip(reqs, sorted(r64, key=lambda r: r.rid))],
+            "workload": f"the first {args.serve} requests of the served stream (seed 20261204): "
+                        f"Poisson arrivals {args.serve_rate}/s, 2D sides lognormal in 32..256, "
+                        "20 % 3D cubes with sides lognormal in 16..40, degree 1-2, doping "
+                        "log-uniform 1e21-1e24, 13-point sweeps 0-0.6 V; HBM budget "
+                        f"{args.serve_budget_gib} GiB, {args.serve_slots} slots",
+            "unit": "I-V solves/s"}
     if not args.no_cpu_baseline:
         host = host_cpu()
         build = native_oracle()

@@ -308,34 +308,6 @@ def summarize(results, wall):
             "newton_iterations": int(sum(sum(r.iterations) for r in results))}
 
 
-def synthetic_requests(n, seed, rate, sizes=(32, 96), degrees=(1, 2), points=13, vmax=0.6,
-                       frac3d=0.0, sizes3d=(8, 12), degrees3d=(1,)):
-    """requests with Poisson arrivals at `rate` per second (all at once for
-    rate 0), log-uniform mesh sides in `sizes` (2D), degrees drawn from
-    `degrees`, log-uniform doping in [1e21, 1e24] m^-3 and a junction in
-    [0.3, 0.7]; each sweeps `points` bias points over [0, vmax] V.  A
-    fraction `frac3d` of the requests (drawn from a second stream, so the 2D
-    draws do not change) are 3D cubes with log-uniform sides in `sizes3d`
-    and degrees drawn from `degrees3d`."""
-    rng = np.random.default_rng(seed)
-    rng3 = np.random.default_rng(seed + 1)
-    t, out = 0.0, []
-    for i in range(n):
-        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
-        side = int(round(np.exp(rng.uniform(np.log(sizes[0]), np.log(sizes[1])))))
-        degree = int(rng.choice(degrees))
-        dim, cells = 2, (side, side)
-        if frac3d > 0 and rng3.uniform() < frac3d:
-            s3 = int(round(np.exp(rng3.uniform(np.log(sizes3d[0]), np.log(sizes3d[1])))))
-            dim, cells, degree = 3, (s3, s3, s3), int(rng3.choice(degrees3d))
-        out.append(IVRequest(
-            rid=i, dim=dim, n_cells=cells, degree=degree,
-            biases=tuple(np.linspace(0.0, vmax, points)[1:]),
-            nd=float(10 ** rng.uniform(21, 24)), na=float(10 ** rng.uniform(21, 24)),
-            junction=float(rng.uniform(0.3, 0.7)), arrival=t))
-    return out
-
-
 def lognormal_side(rng, lo, hi):
     """a mesh side lognormal between lo and hi: log-side normal with its mean
     at the middle of [log lo, log hi] and its standard deviation a quarter of
