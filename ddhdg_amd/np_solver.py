@@ -122,13 +122,21 @@ class NPSolver:
         self._sync_tau()
 
     # ------------------------------------------------------------ set-up
+    def doping_values(self):
+        """rescaled doping at the cell quadrature points (what hdg_set_doping
+        receives)"""
+        return evaluate(self.problem.doping, self.cell_qp) / self.adim.get_component_rescaling_factor(n)
+
     def _upload_problem_data(self):
-        ctx = self.ctx
-        dop = evaluate(self.problem.doping, self.cell_qp) / self.adim.get_component_rescaling_factor(n)
-        ctx.set_doping(dop)
+        self.ctx.set_doping(self.doping_values())
         self._upload_bc()
 
     def _upload_bc(self):
+        self.ctx.set_bc_values(*self.bc_values())
+
+    def bc_values(self):
+        """rescaled Dirichlet and Neumann data at the face quadrature points
+        of the boundary faces (what hdg_set_bc_values receives)"""
         ctx, bh = self.ctx, self.problem.boundary_handler
         dirv = np.zeros((ctx.nfaces, 3, ctx.Qf))
         neuv = np.zeros((ctx.nfaces, 3, ctx.Qf))
@@ -145,7 +153,7 @@ class NPSolver:
                     dirv[faces, c, :] = vals / self.adim.get_component_rescaling_factor(c)
                 else:
                     neuv[faces, c, :] = vals / self.adim.get_neumann_boundary_condition_rescaling_factor(c)
-        ctx.set_bc_values(dirv.ravel(), neuv.ravel())
+        return dirv.ravel(), neuv.ravel()
 
     def _sync_tau(self):
         """CellFaceTauComputer reads the parameters' tau maps at every
