@@ -118,6 +118,7 @@ class NPSolver:
                                    device=self.device)
         self._pattern_cache = {}
         self._solver_cache = {}
+        self.update_cell = None        # the last Newton update of the cells
         self._tau_version = None
         self._sync_tau()
 
@@ -330,6 +331,24 @@ class NPSolver:
                 break
         return it.result()
 
+    def output_results(self, solution_filename, trace_filename=None, save_update=False,
+                       redimensionalize_quantities=True):
+        """NPSolver::output_results (src/np_solver.cpp:1737-2033): the cell
+        fields (and the last Newton update with save_update) as a VTK .vtu
+        file; with trace_filename also the trace dofs."""
+        from . import output
+        cell, tr = self.get_state()
+        upd = None
+        if save_update:
+            if self.update_cell is None:
+                raise RuntimeError("output_results: no Newton update to save yet")
+            upd = self.update_cell.cpu().numpy()
+        output.write_cells(solution_filename, self.mesh, self.degree, cell, self.adim, upd,
+                           redimensionalize_quantities, self.problem.doping)
+        if trace_filename is not None:
+            output.write_trace(trace_filename, self.mesh, self.degree, tr, self.adim,
+                               redimensionalize_quantities)
+
     def current_scale(self):
         """dimensionless flux integral -> physical current: q mu_mag N V_rf
         l^(dim-2) with V_rf = k_B T_mag / q (A per metre of depth in 2D)."""
@@ -449,6 +468,7 @@ class NewtonIteration:
         self.upd_trace.zero_()
         self.upd_trace[self.r2f] = self.sol
         s.ctx.reconstruct(s.x_cell, s.x_trace, self.upd_trace, self.thermodyn, self.upd_cell)
+        s.update_cell = self.upd_cell
         max_abs4([self.upd_cell, self.upd_trace, s.x_cell, s.x_trace], self.nscratch,
                  self.dnorms)
         self.norms.copy_(self.dnorms, non_blocking=True)
