@@ -95,10 +95,10 @@ namespace
            hdg::fast3_supported(C->d, C->k, C->ne, thermodyn, C->en);
   }
 
-  // pass 2 on the register-resident kernel (2D degree 1-2)
+  // pass 2 on the register-resident kernels (2D degree 1-2, 3D degree 1)
   bool use_fast_pass2(const hdg_ctx *C, int thermodyn)
   {
-    return use_fast(C, thermodyn) && C->d == 2 && C->k <= 2;
+    return use_fast(C, thermodyn) && ((C->d == 2 && C->k <= 2) || (C->d == 3 && C->k == 1));
   }
 
   hipError_t launch_pass1(const hdg_ctx *C, bool fast, const hdg::KParams &p, const int *cells,
@@ -565,7 +565,9 @@ int hdg_reconstruct(hdg_ctx *C, const double *x_cell, const double *x_trace,
   p.dxt = dx_trace;
   p.dxc = dx_cell;
   hipError_t e = use_fast_pass2(C, thermodyn)
-                   ? hdg::launch_fast_reconstruct(C->k, p, (int)C->ncells, (hipStream_t)stream)
+                   ? (C->d == 3 ? hdg::launch_fast3_reconstruct(p, (int)C->ncells, (hipStream_t)stream)
+                                : hdg::launch_fast_reconstruct(C->k, p, (int)C->ncells,
+                                                               (hipStream_t)stream))
                    : hdg::launch_reconstruct(C->d, C->k, p, (int)C->ncells, (hipStream_t)stream);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_reconstruct launch");
 }

@@ -213,8 +213,10 @@ namespace hdg
         }
     }
 
-    // one cell (one wave), pass 1 (mode 0: CSR scatter; mode 1: S, g out)
-    template <int NE, bool THERMO, bool MIXED>
+    // one cell (one wave), pass 1 (mode 0: CSR scatter; mode 1: S, g out),
+    // or PASS2: the reconstruction, columns = the scalar dofs and one rhs
+    // column in slot 0 (the rhs bundle carries the whole trace update)
+    template <int NE, bool THERMO, bool MIXED, bool PASS2 = false>
     __device__ __forceinline__ void cell_body3(const KParams &p, double *smem, long long cell,
                                                long long idx, int color, int mode)
     {
@@ -364,7 +366,9 @@ namespace hdg
         {
           const int g = e / (3 * NF), c = (e / NF) % 3, m = e % NF;
           double    v = 0;
-          if (!THERMO && isdir[g * 3 + c])
+          if (PASS2)
+            v = p.dxt[facev[g] * 3 * NF + c * NF + m];
+          else if (!THERMO && isdir[g * 3 + c])
             {
               const int a = g >> 1;
 #pragma unroll
@@ -499,7 +503,16 @@ namespace hdg
         {
           const int col = lane + 64 * h;
           cc[h] = jj[h] = ff[h] = 0;
-          if (col < NS)
+          if (PASS2)
+            kind[h] = col < NS ? 0 : (col == NS ? 2 : 3);
+          if (PASS2 && col < NS)
+            {
+              cc[h] = col / NB;
+              jj[h] = col % NB;
+            }
+          else if (PASS2)
+            ;
+          else if (col < NS)
             {
               kind[h] = 0;
               cc[h]   = col / NB;
@@ -589,9 +602,11 @@ namespace hdg
                     "three 16-lane column tiles cover the trace and rhs columns of a slot");
       const int q16 = lane >> 4, c16 = lane & 15;
       int myk = 0; // the equation this scalar lane was the pivot of
-      // the slots one after the other: only one slot's rows are live
+      // the slots one after the other: only one slot's rows are live (pass
+      // 2 has slot 0 only)
+      constexpr int NHX = PASS2 ? 1 : NH;
 #pragma unroll
-      for (int h = 0; h < NH; ++h)
+      for (int h = 0; h < NHX; ++h)
         {
           // ---- 1a: scalar-test rows, cell quadrature points (q stays a
           // runtime index so the uniform Bz loads are not all hoisted)
@@ -692,7 +707,54 @@ namespace hdg
           // slot 1 replays the recorded steps
           if (h == 0)
             {
-              if constexpr (!MIXED)
+              if constexpr (PASS2)
+                {
+                  gauss_jordan_rec<NS>(R, isU, lane, pivs, myk, gjr);
+                  WAVE_SYNC3();
+                  // the rhs lane holds u(p_k) in row k: publish u, then
+                  // W = M^-1 (rhs_q + D_a u - sum_f C_{a,f} lambda_upd,f)
+                  // and u go out lane-parallel (reference template.h:
+                  // 2688-2697)
+                  double *sol = tcf;
+                  if (kind[0] == 2)
+#pragma unroll
+                    for (int k = 0; k < NS; ++k)
+                      sol[pivs[k]] = R[k];
+                  WAVE_SYNC3();
+                  for (int e = lane; e < S::NCD; e += 64)
+                    {
+                      const int c = e / ((D + 1) * NB), r = e % ((D + 1) * NB);
+                      const int blkc = r / NB, i = r % NB;
+                      double    v = 0;
+                      if (c < NE)
+                        {
+                          const double *u = sol + c * NB;
+                          if (blkc == D)
+                            v = u[i];
+                          else
+                            {
+                              const int a = blkc;
+#pragma unroll
+                              for (int l = 0; l < NB; ++l)
+                                v += tb.Minv[i * NB + l] * rhsq[(c * D + a) * NB + l] +
+                                     tb.MinvD[(a * NB + i) * NB + l] * u[l];
+#pragma unroll
+                              for (int s1 = 0; s1 < 2; ++s1)
+                                {
+                                  const int f = 2 * a + s1;
+#pragma unroll
+                                  for (int m = 0; m < NF; ++m)
+                                    v -= tb.MinvC[(f * NB + i) * NF + m] *
+                                         lupd[(f * 3 + c) * NF + m];
+                                }
+                            }
+                        }
+                      p.dxc[cell * S::NCD + e] = v;
+                    }
+                  WAVE_SYNC3();
+                  return;
+                }
+              else if constexpr (!MIXED)
                 gauss_jordan_rec<NS>(R, isU, lane, pivs, myk, gjr);
               else
                 {
@@ -906,7 +968,8 @@ namespace hdg
         }
     }
 
-    template <int NE, bool THERMO, bool MIXED>
+    // PASS2: cells = all cells in order (cells unused)
+    template <int NE, bool THERMO, bool MIXED, bool PASS2 = false>
     __global__ void __launch_bounds__(64 * wpb3(MIXED)) __attribute__((amdgpu_waves_per_eu(2)))
       fast_condense3(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
     {
@@ -916,10 +979,11 @@ namespace hdg
       __syncthreads();
       const int wv = threadIdx.x >> 6;
       for (int idx = blockIdx.x * WPB + wv; idx < ncell; idx += gridDim.x * WPB)
-        cell_body3<NE, THERMO, MIXED>(p, smem, (long long)cells[idx], idx, color, mode);
+        cell_body3<NE, THERMO, MIXED, PASS2>(p, smem, PASS2 ? (long long)idx : (long long)cells[idx],
+                                             idx, color, mode);
     }
 
-    template <int NE, bool THERMO, bool MIXED>
+    template <int NE, bool THERMO, bool MIXED, bool PASS2 = false>
     hipError_t launch_fast3(const KParams &p, const int *cells, int ncell, int color, int mode,
                             hipStream_t s)
     {
@@ -927,7 +991,7 @@ namespace hdg
       static bool      attr  = false;
       if (!attr)
         {
-          hipError_t e = hipFuncSetAttribute((const void *)fast_condense3<NE, THERMO, MIXED>,
+          hipError_t e = hipFuncSetAttribute((const void *)fast_condense3<NE, THERMO, MIXED, PASS2>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)bytes);
           if (e != hipSuccess)
@@ -942,8 +1006,8 @@ namespace hdg
       constexpr int WPB  = wpb3(MIXED);
       const int     want = (ncell + WPB - 1) / WPB;
       const int     grid = want < 4 * ncu ? want : 4 * ncu;
-      hipLaunchKernelGGL((fast_condense3<NE, THERMO, MIXED>), dim3(grid), dim3(64 * WPB), bytes,
-                         s, p, cells, ncell, color, mode);
+      hipLaunchKernelGGL((fast_condense3<NE, THERMO, MIXED, PASS2>), dim3(grid), dim3(64 * WPB),
+                         bytes, s, p, cells, ncell, color, mode);
       return hipGetLastError();
     }
   } // namespace
@@ -965,5 +1029,12 @@ namespace hdg
                      : launch_fast3<1, true, false>(p, cells, ncell, color, mode, s);
     return p.mixed ? launch_fast3<3, false, true>(p, cells, ncell, color, mode, s)
                    : launch_fast3<3, false, false>(p, cells, ncell, color, mode, s);
+  }
+
+  hipError_t launch_fast3_reconstruct(const KParams &p, int ncells, hipStream_t s)
+  {
+    if (p.thermo)
+      return launch_fast3<1, true, false, true>(p, nullptr, ncells, 0, 0, s);
+    return launch_fast3<3, false, false, true>(p, nullptr, ncells, 0, 0, s);
   }
 } // namespace hdg

@@ -103,4 +103,6 @@ namespace hdg
                                int nchunks, int color, int grid, hipStream_t s);
   // register-resident pass 2 (same shapes as launch_fast_condense)
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s);
+  // pass 2 on the 3D degree-1 register kernel (same operators as its pass 1)
+  hipError_t launch_fast3_reconstruct(const KParams &p, int ncells, hipStream_t s);
 } // namespace hdg
