@@ -589,6 +589,40 @@ int hdg_cell_systems(hdg_ctx *C, const double *x_cell, const double *x_trace, in
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_cell_systems launch");
 }
 
+int hdg_local_system(hdg_ctx *C, const double *x_cell, const double *x_trace, int thermodyn,
+                     const int *cells, int ncell, double *A, double *CT, double *TC, double *TT,
+                     double *c, double *t, void *stream)
+{
+  if (!C || !x_cell || !x_trace || !cells || !A || !CT || !TC || !TT || !c || !t || ncell < 0)
+    return fail("hdg_local_system: null argument");
+  if (thermodyn && (C->en[1] || C->en[2] || !C->en[0]))
+    return fail("hdg_local_system: thermodynamic equilibrium runs with V only", -2);
+  hdg::KParams p = params(C, thermodyn);
+  p.xc           = x_cell;
+  p.xt           = x_trace;
+  hipStream_t s  = (hipStream_t)stream;
+  // the column forms go through a workspace: cells in chunks of at most
+  // 256 MB of it
+  const size_t per   = hdg::local_system_workspace(C->d, C->k, C->ne);
+  const int    chunk = (int)std::max<size_t>(1, std::min<size_t>(ncell, (256u << 20) / per));
+  double      *G     = nullptr;
+  hipError_t   e     = hipMalloc((void **)&G, per * std::max(chunk, 1));
+  if (e != hipSuccess)
+    return hip_fail(e, "hdg_local_system workspace");
+  const long long N = (long long)C->ne * (C->d + 1) * C->nb, T = (long long)C->nfc * C->ne * C->nf;
+  for (int a = 0; a < ncell && e == hipSuccess; a += chunk)
+    {
+      const int m = std::min(chunk, ncell - a);
+      e = hdg::launch_local_system(C->d, C->k, p, cells + a, m, G, A + a * N * N, CT + a * N * T,
+                                   TC + a * T * N, TT + a * T * T, c + a * N, t + a * T, s);
+    }
+  const hipError_t e2 = hipStreamSynchronize(s);
+  (void)hipFree(G);
+  if (e == hipSuccess)
+    e = e2;
+  return e == hipSuccess ? 0 : hip_fail(e, "hdg_local_system launch");
+}
+
 int hdg_terminal_current(hdg_ctx *C, const double *x_cell, const double *x_trace,
                          int boundary_id, double *out, void *stream)
 {

@@ -103,6 +103,12 @@ namespace hdg
                                int nchunks, int color, int grid, hipStream_t s);
   // register-resident pass 2 (same shapes as launch_fast_condense)
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s);
+  // the uncondensed local system of listed cells (hdg_local.hip); G: a
+  // workspace of local_system_workspace(dim, deg, ne) bytes per cell
+  size_t     local_system_workspace(int dim, int deg, int ne);
+  hipError_t launch_local_system(int dim, int deg, const KParams &p, const int *cells, int ncell,
+                                 double *G, double *A, double *CT, double *TC, double *TT,
+                                 double *c, double *t, hipStream_t s);
   // pass 2 on the 3D degree-1 register kernel (same operators as its pass 1)
   hipError_t launch_fast3_reconstruct(const KParams &p, int ncells, hipStream_t s);
 } // namespace hdg

@@ -30,6 +30,7 @@ EXPORTS = [
     "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
     "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_condense", "hdg_batch_info",
     "hdg_max_abs4", "hdg_set_cell_face_tau", "hdg_max_abs4_noalloc", "hdg_trace_plan_size",
+    "hdg_local_system",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -79,6 +80,7 @@ def lib():
             "hdg_condense": (i, [v, v, v, i, v, v, v, v]),
             "hdg_reconstruct": (i, [v, v, v, v, i, v, v]),
             "hdg_cell_systems": (i, [v, v, v, i, v, i, v, v, v]),
+            "hdg_local_system": (i, [v, v, v, i, v, i, v, v, v, v, v, v, v]),
             "hdg_trace_solver_create": (i, [v, P(v)]),
             "hdg_trace_solver_destroy": (None, [v]),
             "hdg_trace_solver_info": (i, [v, dp]),
@@ -259,6 +261,20 @@ class HdgContext:
         _check(self.L.hdg_cell_systems(self.ctx, xc, xt, int(thermodyn), cp, m,
                                        _dev(S, "S", m * self.T * self.T), _dev(g, "g", m * self.T),
                                        _stream()))
+
+    def local_system(self, x_cell, x_trace, thermodyn, cells, A, CT, TC, TT, c, t):
+        """uncondensed local blocks of the listed cells (N = NE*(dim+1)*NB):
+        A (N x N), CT (N x T), TC (T x N, negated), TT (T x T), c, t per cell"""
+        xc, xt = self._state(x_cell, x_trace)
+        m = int(cells.numel())
+        cp = _dev(cells, "cells", m, "int32")
+        if m and (int(cells.min()) < 0 or int(cells.max()) >= self.ncells):
+            raise HdgError(f"cells: indices outside [0, {self.ncells})")
+        N, T = self.ne * (self.dim + 1) * self.nb, self.T
+        _check(self.L.hdg_local_system(
+            self.ctx, xc, xt, int(thermodyn), cp, m, _dev(A, "A", m * N * N),
+            _dev(CT, "CT", m * N * T), _dev(TC, "TC", m * T * N), _dev(TT, "TT", m * T * T),
+            _dev(c, "c", m * N), _dev(t, "t", m * T), _stream()))
 
     def terminal_current(self, x_cell, x_trace, boundary_id, out):
         """out (device, 2): integrals of the numerical fluxes of n and p over

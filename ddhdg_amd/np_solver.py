@@ -331,6 +331,36 @@ class NPSolver:
                 break
         return it.result()
 
+    def get_local_cell_system(self, cell_level, cell_index, compute_thermodynamic_equilibrium=False):
+        """NPSolver::get_local_cell_system (src/np_solver.cpp:2337-2445): the
+        cell's local blocks as the per-cell operator assembles them before
+        its Schur complement (after the LocalCondenser), computed on the
+        device: ({"cc", "ct", "tc", "tt"}, {"c", "t"}) as numpy arrays (tc
+        negated, as the reference stores it).  The mesh has one level; a
+        thermodynamic-equilibrium call runs with V alone, as the reference's
+        does."""
+        if cell_level != 0:
+            raise ValueError("the structured mesh has one level (cell_level 0)")
+        t = self.torch
+        saved = self.enabled
+        if compute_thermodynamic_equilibrium:
+            self.set_enabled_components(True, False, False)
+        try:
+            ctx = self.ctx
+            N, T = ctx.ne * (self.mesh.dim + 1) * ctx.nb, ctx.T
+            dev = self.device
+            cells = t.tensor([int(cell_index)], dtype=t.int32, device=dev)
+            out = [t.empty(sz, dtype=t.float64, device=dev)
+                   for sz in (N * N, N * T, T * N, T * T, N, T)]
+            ctx.local_system(self.x_cell, self.x_trace, compute_thermodynamic_equilibrium, cells,
+                             *out)
+            A, CT, TC, TT, c, tv = (x.cpu().numpy() for x in out)
+        finally:
+            if compute_thermodynamic_equilibrium:
+                self.set_enabled_components(*saved)
+        return ({"cc": A.reshape(N, N), "ct": CT.reshape(N, T), "tc": TC.reshape(T, N),
+                 "tt": TT.reshape(T, T)}, {"c": c, "t": tv})
+
     def output_results(self, solution_filename, trace_filename=None, save_update=False,
                        redimensionalize_quantities=True):
         """NPSolver::output_results (src/np_solver.cpp:1737-2033): the cell

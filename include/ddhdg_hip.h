@@ -197,6 +197,21 @@ int hdg_cell_systems(hdg_ctx *ctx, const double *x_cell,
                      const double *x_trace, int thermodyn, const int *cells,
                      int ncell, double *S, double *g, void *stream);
 
+/* The uncondensed local system of selected cells: the counterpart of the
+ * reference's get_local_cell_system (src/np_solver.cpp:2337-2445), which runs
+ * the per-cell operator without its Schur complement.  Per listed cell, in
+ * the reference's local order (N = NE*(dim+1)*NB cell dofs, flux axes then
+ * scalar per enabled component; T trace dofs as in hdg_cell_systems):
+ *   A  (N x N), CT (N x T), TC (T x N, stored negated as the reference does),
+ *   TT (T x T), c (N), t (T),
+ * row-major, device arrays of ncell blocks each, after the LocalCondenser
+ * has moved the Dirichlet columns of CT into c.  Synchronises the stream
+ * (a temporary workspace is allocated and freed). */
+int hdg_local_system(hdg_ctx *ctx, const double *x_cell, const double *x_trace,
+                     int thermodyn, const int *cells, int ncell, double *A,
+                     double *CT, double *TC, double *TT, double *c, double *t,
+                     void *stream);
+
 /* Terminal current through the faces of one boundary id (outward normal):
  * out (device, 2 doubles) receives the integrals of the HDG numerical fluxes
  *   F_n =  n mu_n E.nu - U_T mu_n W_n.nu - tau_n (n - lambda_n)
