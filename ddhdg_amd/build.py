@@ -23,7 +23,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-I" + os.path.join(ROOT, "include"), "-Wall", "-Wno-unused-function"]
 SOURCES = ["hdg_tables.cpp", "hdg_topology.cpp", "hdg_nd_plan.cpp", "hdg_kernels.hip",
            "hdg_fast2d.hip", "hdg_fast2dp.hip", "hdg_fast3d.hip", "hdg_capi.hip", "hdg_trace_solver.hip",
-           "hdg_local.hip"]
+           "hdg_local.hip", "hdg_gmres.hip"]
 LIBS = ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 
 

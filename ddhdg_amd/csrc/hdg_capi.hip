@@ -919,6 +919,20 @@ int hdg_trace_plan_size(int dim, const int *n_cells, int dofs_per_face, double o
   return 0;
 }
 
+int hdg_gmres_solve(const long long *rowptr, const int *colind, const double *vals, long long n,
+                    const double *rhs, double *x, int restart, double tol, long long max_it,
+                    long long *iterations, double *residual, void *stream)
+{
+  if (!iterations || !residual || n < 0 || (n > 0 && (!rowptr || !colind || !vals || !rhs || !x)))
+    return fail("hdg_gmres_solve: invalid argument");
+  if (restart < 1 || max_it < 0 || !(tol >= 0))
+    return fail("hdg_gmres_solve: restart >= 1, max_it >= 0 and tol >= 0 are required", -2);
+  std::string err;
+  const int   rc = hdg::gmres_solve(rowptr, colind, vals, n, rhs, x, restart, tol, max_it,
+                                    iterations, residual, (hipStream_t)stream, err);
+  return rc ? fail(err, rc) : 0;
+}
+
 int hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face, unsigned seed,
                             const double *csr_vals, double out[3])
 {

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 namespace hdg
 {
   struct DevTables
@@ -111,4 +113,13 @@ namespace hdg
                                  double *c, double *t, hipStream_t s);
   // pass 2 on the 3D degree-1 register kernel (same operators as its pass 1)
   hipError_t launch_fast3_reconstruct(const KParams &p, int ncells, hipStream_t s);
+  // restarted GMRES, no preconditioner, zero initial guess (hdg_gmres.hip):
+  // x = A^-1 b for the device CSR matrix (rp, ci, vals) of order n; stops
+  // when the Arnoldi residual estimate is at most tol.  Synchronises s.
+  // 0, or -3 (device failure), -5 (residual not a number), -6 (max_it
+  // iterations without convergence), with err set
+  int        gmres_solve(const long long *rp, const int *ci, const double *vals, long long n,
+                         const double *b, double *x, int restart, double tol, long long max_it,
+                         long long *iterations, double *residual, hipStream_t s,
+                         std::string &err);
 } // namespace hdg

@@ -30,7 +30,7 @@ EXPORTS = [
     "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
     "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_condense", "hdg_batch_info",
     "hdg_max_abs4", "hdg_set_cell_face_tau", "hdg_max_abs4_noalloc", "hdg_trace_plan_size",
-    "hdg_local_system",
+    "hdg_local_system", "hdg_gmres_solve",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -98,6 +98,8 @@ def lib():
             "hdg_max_abs4_noalloc": (i, [P(v), lp, v, v]),
             "hdg_trace_plan_size": (i, [i, ip, i, dp]),
             "hdg_set_cell_face_tau": (i, [v, dp]),
+            "hdg_gmres_solve": (i, [v, v, v, ct.c_longlong, v, v, i, ct.c_double,
+                                    ct.c_longlong, lp, dp, v]),
         }
         for name, (res, args) in sigs.items():
             f = getattr(L, name)
@@ -346,6 +348,26 @@ def trace_plan_size(dim, n_cells, dofs_per_face):
     return {"front_entries": int(out[0]), "work_entries": int(out[1]),
             "pivot_entries": int(out[2]), "flops": float(out[3]), "max_front": int(out[4]),
             "nodes": int(out[5])}
+
+
+def gmres_solve(rowptr, colind, vals, rhs, x, restart=28, tol=0.0, max_it=None):
+    """x = A^-1 rhs by restarted GMRES without preconditioner from x = 0
+    (hdg_gmres_solve), for the device CSR matrix (rowptr int64 n+1, colind
+    int32, vals float64) of order n = rhs.numel(); the column indices must lie
+    in [0, n) (the product reads x[colind] unchecked: pass a pattern that
+    hdg_csr_pattern wrote).  Returns (iterations, residual estimate); raises
+    HdgError when it does not converge within max_it (default 10 n)"""
+    n = rhs.numel()
+    nnz = vals.numel()
+    max_it = 10 * n if max_it is None else int(max_it)
+    it, res = ct.c_longlong(0), ct.c_double(0.0)
+    _check(lib().hdg_gmres_solve(_dev(rowptr, "rowptr", n + 1, "int64"),
+                                 _dev(colind, "colind", nnz, "int32"),
+                                 _dev(vals, "vals", nnz), ct.c_longlong(n),
+                                 _dev(rhs, "rhs", n), _dev(x, "x", n), int(restart),
+                                 float(tol), ct.c_longlong(max_it), ct.byref(it),
+                                 ct.byref(res), _stream()))
+    return int(it.value), float(res.value)
 
 
 class HdgBatch:

@@ -22,7 +22,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import fe
-from .hip_lib import HdgContext, MeshDesc, Phys, TraceSolver, max_abs4
+from .hip_lib import HdgContext, MeshDesc, Phys, TraceSolver, gmres_solve, max_abs4
 from .problem import (DIRICHLET, EV, KB, NEUMANN, Q, Adimensionalizer,
                       CellFaceTauNPSolverParameters, FixedTauNPSolverParameters,
                       LinearRecombinationTerm, flux_type_code,
@@ -300,9 +300,33 @@ class NPSolver:
             self._solver_cache[key] = TraceSolver(self.ctx)
         return self._solver_cache[key]
 
+    def _device_pattern(self):
+        """the trace CSR pattern of the enabled components on the device
+        (row pointers int64, column indices int32), checked on the host"""
+        key = ("dev",) + tuple(self.enabled)
+        if key not in self._pattern_cache:
+            rp, ci, _, _ = self._pattern()
+            n_ = self.ctx.ntrace
+            if (rp[0] != 0 or rp[-1] != len(ci) or np.any(np.diff(rp) < 0) or
+                    (len(ci) and (ci.min() < 0 or ci.max() >= n_))):
+                raise RuntimeError("trace CSR pattern out of range")
+            t = self.torch
+            self._pattern_cache[key] = (t.as_tensor(rp, device=self.device),
+                                        t.as_tensor(ci, device=self.device))
+        return self._pattern_cache[key]
+
     def _solve_trace(self, vals, rhs, sol):
         """solve_linear_problem (src/np_solver.cpp:1326-1354): the condensed
-        trace system, on the device or with SuperLU on the host."""
+        trace system, by GMRES on the device when iterative_linear_solver is
+        set (restarted every 28 iterations, no preconditioner, tolerance
+        1e-10 |rhs|_2, at most 10 n iterations, as the reference's
+        SolverGMRES with its default 30 temporary vectors), else by the direct
+        solver, on the device or with SuperLU on the host."""
+        if self.parameters.iterative_linear_solver:
+            rp, ci = self._device_pattern()
+            tol = 1e-10 * float(self.torch.linalg.vector_norm(rhs))
+            self.gmres_iterations, _ = gmres_solve(rp, ci, vals, rhs, sol, 28, tol)
+            return
         if self.parameters.trace_solver == "gpu":
             ts = self.trace_solver()
             ts.factor(vals)
@@ -440,6 +464,9 @@ class NewtonIteration:
         # every later one replays it; all buffers below are fixed for the
         # object's lifetime and boundary data are updated in place
         self.use_graph = solver.parameters.graph_newton if graph is None else graph
+        if self.use_graph and solver.parameters.iterative_linear_solver:
+            raise ValueError("graph_newton needs the direct trace solver: GMRES "
+                             "synchronises with the host every iteration")
         self.graph = None
         # capture needs a stream other than the default one
         cur = t.cuda.current_stream()
@@ -510,7 +537,8 @@ class NewtonIteration:
         self.upd_cell_norm = float(n_[0])
         upd_norm = max(self.upd_cell_norm, float(n_[1]))
         cur_norm = max(float(n_[2]), float(n_[3]))
-        if not np.isfinite(upd_norm) and self.s.parameters.trace_solver == "gpu":
+        if (not np.isfinite(upd_norm) and self.s.parameters.trace_solver == "gpu"
+                and not self.s.parameters.iterative_linear_solver):
             self.s.trace_solver().factor(self.vals, check=True)
         if self.log is not None:
             self.log(self.step, upd_norm, cur_norm)

@@ -165,9 +165,11 @@ class NonlinearSolverParameters:
 
 @dataclass
 class FixedTauNPSolverParameters:
-    V_degree: int = 2
-    n_degree: int = 2
-    p_degree: int = 2
+    """FixedTauNPSolverParameters (include/np_solver_parameters.h:295-330):
+    degree 1 and tau 1 for every component by default, as there"""
+    V_degree: int = 1
+    n_degree: int = 1
+    p_degree: int = 1
     nonlinear_parameters: NonlinearSolverParameters = field(default_factory=NonlinearSolverParameters)
     V_tau: float = 1.0
     n_tau: float = 1.0

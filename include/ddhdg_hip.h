@@ -320,6 +320,22 @@ int  hdg_trace_plan_selftest(int dim, const int *n_cells, int dofs_per_face,
  * budget; no counterpart in the reference. */
 int  hdg_trace_plan_size(int dim, const int *n_cells, int dofs_per_face, double out[6]);
 
+/* The iterative branch of the reference's solve_linear_problem
+ * (src/np_solver.cpp:1326-1354, parameters->iterative_linear_solver): GMRES
+ * restarted every `restart` iterations (deal.II's default keeps 30 vectors,
+ * i.e. 28), no preconditioner, zero initial guess, stopping when the
+ * residual estimate is at most tol (the reference: 1e-10 |rhs|_2) or with
+ * -6 after max_it iterations (the reference: 10 n).  rowptr (n+1), colind,
+ * vals: a device CSR matrix of order n (hdg_csr_pattern layout); rhs, x:
+ * device vectors of n entries.  iterations and residual (host) receive the
+ * iteration count and the last residual estimate.  Synchronises stream;
+ * allocates a workspace of (restart + 2) n doubles for the call.  -5 if the
+ * residual is not a number. */
+int  hdg_gmres_solve(const long long *rowptr, const int *colind, const double *vals,
+                     long long n, const double *rhs, double *x, int restart,
+                     double tol, long long max_it, long long *iterations,
+                     double *residual, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
