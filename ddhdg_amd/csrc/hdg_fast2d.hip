@@ -703,6 +703,21 @@ namespace hdg
       for (int r = 0; r < NS; ++r)
         R[r] *= keep;
 
+      // The second colour adds its part of the diagonal block of an interior
+      // face to the first colour's.  A trace lane's column of that block
+      // lies in the rows of its own face only, so its TDF earlier values
+      // are loaded here, their latency hidden behind the elimination.
+      double     dold[TDF];
+      const bool dadd = !PASS2 && mode == 0 && isL && color == 1 && bidv[fl] < 0;
+      {
+        const int       rj0 = isL ? (lane - NS) % TDF : 0;
+        const long long b0  = blk[fl] * TDF * TDF + (long long)slot[fl * NFC + fl] * TDF + rj0;
+        const long long rs0 = (long long)nbk[fl] * TDF;
+#pragma unroll
+        for (int r = 0; r < TDF; ++r)
+          dold[r] = dadd ? p.vals[b0 + r * rs0] : 0.0;
+      }
+
       // ---- 3: Gauss-Jordan, scaled column pivoting
       if constexpr (!MIXED)
         gauss_jordan<NS>(R, isU, lane, pivs);
@@ -871,7 +886,7 @@ namespace hdg
               const int lt = lane - NS, gj = lt / TDF, rj = lt % TDF;
               const long long base = blk[g] * TDF * TDF + (long long)slot[g * NFC + gj] * TDF + rj;
               const long long rs   = (long long)nbk[g] * TDF;
-              const bool      addd = color == 1 && g == gj && bidv[g] < 0;
+              const bool      addd = dadd && g == gj;
 #pragma unroll
               for (int r = 0; r < TDF; ++r)
                 {
@@ -879,10 +894,7 @@ namespace hdg
                   const bool dir = isdir[g * 3 + (NE == 1 ? 0 : ci)];
                   const double v = dir ? (i == lt ? 1.0 : 0.0) : (dircol ? 0.0 : acc[r]);
                   const long long pos = base + r * rs;
-                  if (addd)
-                    p.vals[pos] += v;
-                  else
-                    p.vals[pos] = v;
+                  p.vals[pos] = addd ? dold[r] + v : v;
                 }
             }
           if (isR)
