@@ -167,11 +167,98 @@ def batched_sample(args, torch, stream):
     out["autotune"] = groups[0][0].info()
     if len(groups) > 1:
         out["autotune_degree3"] = groups[1][0].info()
+    if args.batch_newton:
+        del groups, bufs
+        # the trace factors are allocated by the library, outside torch's cache
+        torch.cuda.empty_cache()
+        out["newton"] = batched_newton(torch, sv)
     out["workload"] = (f"{len(sv)} 2D diodes from seed 20261203: sides uniform in 16..192 cells, "
                        "degree uniform in {1, 2, 3}, doping log-uniform 1e21-1e24, junction in "
                        "[0.3, 0.7], contact bias uniform in [0, 0.6] V; pass 1; degrees 1-2 in one "
                        "persistent launch per colour, degree 3 (pair kernel, FP64: no FP32 path "
                        "there) in another")
+    return out
+
+
+def batched_newton(torch, sv):
+    """the devices' Newton workflow (thermodynamic equilibrium, n and p,
+    the coupled system) with FP32 + refinement condensation: all devices
+    advanced together (BatchedNewton, one hipGraph per stage) against the
+    devices solved one after the other (each with its own captured Newton
+    iteration), from the same start; the final states are compared"""
+    from ddhdg_amd.batch import BatchedNewton, batched_run
+
+    for s in sv:
+        s.ctx.set_precision(1)
+    init = [(s.x_cell.clone(), s.x_trace.clone(), tuple(s.enabled)) for s in sv]
+
+    def workflow(batched):
+        rec = {}
+        stages = [("equilibrium", None), ("np", (False, True, True)),
+                  ("coupled", (True, True, True))]
+        for name, en in stages:
+            for s in sv:
+                if en is not None:
+                    s.set_enabled_components(*en)
+                if name == "equilibrium":
+                    s.set_enabled_components(True, False, False)
+                # the factors of one component set per device at a time
+                s.release_trace_solvers()
+            torch.cuda.empty_cache()
+            # the trace solvers of the stage (host plan, factor storage),
+            # built before the clock starts in both modes
+            # (and the charge-neutrality first guess, a host projection)
+            ts0 = time.perf_counter()
+            for s in sv:
+                s.trace_solver()
+                if name == "equilibrium":
+                    s.compute_local_charge_neutrality()
+            torch.cuda.synchronize()
+            setup = time.perf_counter() - ts0
+            t0 = time.perf_counter()
+            if name == "equilibrium":
+                nl = [s.parameters.nonlinear_parameters for s in sv]
+                r = (BatchedNewton(sv, True).run() if batched else
+                     [s._private_run(p.absolute_tolerance, p.relative_tolerance,
+                                     p.max_number_of_iterations, True) for s, p in zip(sv, nl)])
+            else:
+                r = batched_run(sv) if batched else [s.run() for s in sv]
+            torch.cuda.synchronize()
+            elapsed = time.perf_counter() - t0
+            if name == "equilibrium":
+                for s in sv:
+                    s.project_densities_from_potential(0.0, 0.0)
+            rec[name] = {"s": elapsed, "setup_s": setup,
+                         "iterations": [x.iterations for x in r],
+                         "converged": sum(1 for x in r if x.converged)}
+        return rec
+
+    batched = workflow(True)
+    final = [(s.x_cell.clone(), s.x_trace.clone()) for s in sv]
+    for s, (c, tr, en) in zip(sv, init):
+        s.set_enabled_components(*en)
+        s.x_cell.copy_(c)
+        s.x_trace.copy_(tr)
+        s.parameters.graph_newton = True
+    sequential = workflow(False)
+    same = all(torch.equal(s.x_cell, c) and torch.equal(s.x_trace, tr)
+               for s, (c, tr) in zip(sv, final))
+    for s in sv:
+        s.parameters.graph_newton = False
+        s.ctx.set_precision(0)
+    out = {"precision": "fp32_refine (degree 3: FP64, no FP32 path in the pair kernel)",
+           "states_bitwise_equal": bool(same)}
+    for name in batched:
+        b, q = batched[name], sequential[name]
+        out[name] = {"batched_s": b["s"], "sequential_s": q["s"],
+                     "trace_solver_setup_s": b["setup_s"],
+                     "speedup": q["s"] / b["s"] if b["s"] > 0 else None,
+                     "iterations_max": max(b["iterations"]),
+                     "iterations_total": sum(b["iterations"]),
+                     "iterations_equal": b["iterations"] == q["iterations"],
+                     "converged": b["converged"], "devices": len(sv)}
+    out["workflow_batched_s"] = sum(batched[n]["s"] for n in batched)
+    out["workflow_sequential_s"] = sum(sequential[n]["s"] for n in sequential)
     return out
 
 
@@ -353,6 +440,8 @@ def main():
                     help="cells per axis of the 3D degree-2 sample (0: none)")
     ap.add_argument("--batch", type=int, default=64,
                     help="devices condensed together beside the metric (0: none)")
+    ap.add_argument("--no-batch-newton", dest="batch_newton", action="store_false",
+                    help="skip the batched Newton workflow of the batched devices")
     ap.add_argument("--serve", type=int, default=16,
                     help="requests of the served stream served beside the metric (0: none)")
     ap.add_argument("--serve-rate", type=float, default=4.0)

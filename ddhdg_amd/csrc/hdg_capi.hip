@@ -652,6 +652,9 @@ struct hdg_batch
   long long              ncells = 0;
   hdg::KParams          *d_params = nullptr, *h_params = nullptr; // h_params pinned
   hipEvent_t             copied   = nullptr;
+  // the parameter blocks of the last upload: an unchanged call (the same
+  // buffers and data) skips the upload, which a stream capture cannot hold
+  std::vector<hdg::KParams> last;
   int                    grid     = 0; // WGs per launch; 0 until tuned
   float                  tuned_ms[3] = {0, 0, 0};
 };
@@ -775,12 +778,9 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
   if (thermodyn && B->deg3)
     return fail("hdg_batch_condense: degree-3 batches run the full operator only", -2);
   hipStream_t s = (hipStream_t)stream;
-  // the pinned parameter blocks may still feed the previous call's copy
-  hipError_t e = hipEventSynchronize(B->copied);
-  if (e != hipSuccess)
-    return hip_fail(e, "hdg_batch_condense");
-  const int n     = (int)B->ctx.size();
-  int       mixed = B->ctx[0]->precision;
+  const int   n     = (int)B->ctx.size();
+  int         mixed = B->ctx[0]->precision;
+  std::vector<hdg::KParams> np(n);
   for (int i = 0; i < n; ++i)
     {
       if (!x_cell[i] || !x_trace[i] || !csr_vals[i] || !rhs[i] || !dir_vals[i])
@@ -807,14 +807,37 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
       p.vals         = csr_vals[i];
       p.rhs          = rhs[i];
       p.dirout       = dir_vals[i];
-      B->h_params[i] = p;
+      std::memcpy((void *)&np[i], (const void *)&p, sizeof(p));
     }
-  e = hipMemcpyAsync(B->d_params, B->h_params, n * sizeof(hdg::KParams), hipMemcpyHostToDevice,
-                     s);
-  if (e == hipSuccess)
-    e = hipEventRecord(B->copied, s);
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  hipError_t             e   = hipStreamIsCapturing(s, &cap);
   if (e != hipSuccess)
-    return hip_fail(e, "hdg_batch_condense copy");
+    return hip_fail(e, "hdg_batch_condense");
+  const bool capturing = cap != hipStreamCaptureStatusNone;
+  const bool same      = B->last.size() == (size_t)n &&
+                    std::memcmp((const void *)B->last.data(), (const void *)np.data(),
+                                n * sizeof(hdg::KParams)) == 0;
+  if (!same)
+    {
+      if (capturing)
+        return fail("hdg_batch_condense: a captured call must use the buffers and data of the "
+                    "previous (eager) call",
+                    -2);
+      // the pinned parameter blocks may still feed the previous call's copy
+      e = hipEventSynchronize(B->copied);
+      if (e != hipSuccess)
+        return hip_fail(e, "hdg_batch_condense");
+      std::memcpy((void *)B->h_params, (const void *)np.data(), n * sizeof(hdg::KParams));
+      e = hipMemcpyAsync(B->d_params, B->h_params, n * sizeof(hdg::KParams),
+                         hipMemcpyHostToDevice, s);
+      if (e == hipSuccess)
+        e = hipEventRecord(B->copied, s);
+      if (e != hipSuccess)
+        return hip_fail(e, "hdg_batch_condense copy");
+      B->last = np;
+    }
+  if (B->grid == 0 && capturing)
+    return fail("hdg_batch_condense: the first call tunes the grid and cannot be captured", -2);
   if (B->grid == 0)
     {
       // autotuner: persistent grids of 1, 2 and 4 workgroups per CU, timed

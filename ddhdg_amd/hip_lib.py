@@ -154,6 +154,7 @@ class HdgContext:
         _check(self.L.hdg_ctx_create(ct.byref(mesh), ct.byref(phys), ct.byref(self.ctx)))
         self.mesh, self.phys = mesh, phys
         self.dim = mesh.dim
+        self.precision = 0
         self._refresh()
 
     def close(self):
@@ -182,6 +183,7 @@ class HdgContext:
         """0: FP64; 1: FP32 elimination + one FP64 refinement step (pass 1 of
         the register kernel)"""
         _check(self.L.hdg_set_precision(self.ctx, int(precision)))
+        self.precision = int(precision)
 
     def set_kernel_variant(self, variant):
         """0 automatic, 1 LDS workgroup-per-cell kernel, 2 register-resident."""

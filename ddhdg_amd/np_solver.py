@@ -300,6 +300,12 @@ class NPSolver:
             self._solver_cache[key] = TraceSolver(self.ctx)
         return self._solver_cache[key]
 
+    def release_trace_solvers(self, keep_current=True):
+        """free the device memory (the factors) of the cached trace solvers:
+        all of them, or all but the current component set's"""
+        for key in [k for k in self._solver_cache if not (keep_current and k == self.enabled)]:
+            self._solver_cache.pop(key).close()
+
     def _device_pattern(self):
         """the trace CSR pattern of the enabled components on the device
         (row pointers int64, column indices int32), checked on the host"""

@@ -139,8 +139,7 @@ class _Slot:
             elif self.stage >= 3:
                 configs.set_bias(s, self.req.biases[self.stage - 3])
             # a request holds the trace solver of its current stage only
-            for key in [k for k in s._solver_cache if k != s.enabled]:
-                s._solver_cache.pop(key).close()
+            s.release_trace_solvers()
             if self.stage >= 3:
                 # bias points: same operator and buffers, boundary data
                 # changed in place; the captured iteration is replayed
@@ -157,9 +156,7 @@ class _Slot:
         s = getattr(self, "solver", None)
         if s is None:
             return
-        for ts in s._solver_cache.values():
-            ts.close()
-        s._solver_cache.clear()
+        s.release_trace_solvers(keep_current=False)
         self.it = None
         s.ctx.close()
         self.solver = None

@@ -252,7 +252,10 @@ int hdg_max_abs4_noalloc(const double *const *vectors, const long long *lengths,
  * full operator only.  The
  * persistent grid is picked by timing 1, 2 and 4 workgroups per CU on the
  * first call (hdg_batch_info reports the choice).  Pointer arguments are
- * host arrays of n device pointers, in the order of ctxs.  No counterpart in
+ * host arrays of n device pointers, in the order of ctxs.  A call with the
+ * buffers and data of the previous call reuses its uploaded parameter
+ * blocks, so it can be captured into a HIP graph (after an eager first
+ * call); a captured call with other buffers returns -2.  No counterpart in
  * the reference (one mesh per NPSolver). */
 typedef struct hdg_batch hdg_batch;
 
