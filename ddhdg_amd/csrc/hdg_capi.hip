@@ -656,12 +656,15 @@ struct hdg_batch
   // buffers and data) skips the upload, which a stream capture cannot hold
   std::vector<hdg::KParams> last;
   int                    grid     = 0; // WGs per launch; 0 until tuned
-  float                  tuned_ms[3] = {0, 0, 0};
+  int                    tile = 4; // waves per workgroup (register batches)
+  // the autotuner's candidates: (tile, grid, ms of a two-colour pass)
+  int                    ncand = 0, cand_tile[8] = {}, cand_grid[8] = {};
+  float                  cand_ms[8] = {};
 };
 
 namespace
 {
-  int batch_launch(hdg_batch *B, int thermodyn, int mixed, int grid, hipStream_t s)
+  int batch_launch(hdg_batch *B, int thermodyn, int mixed, int grid, int tile, hipStream_t s)
   {
     for (int col = 0; col < 2; ++col)
       {
@@ -670,7 +673,7 @@ namespace
                                            B->nchunks[col], col, grid, s)
                   : hdg::launch_batch_condense(B->ne, thermodyn, mixed, B->d_params,
                                                B->d_chunk_dev[col], B->d_items[col],
-                                               B->nchunks[col], col, grid, s);
+                                               B->nchunks[col], col, grid, tile, s);
         if (e != hipSuccess)
           return hip_fail(e, "hdg_batch_condense launch");
       }
@@ -840,22 +843,34 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
     return fail("hdg_batch_condense: the first call tunes the grid and cannot be captured", -2);
   if (B->grid == 0)
     {
-      // autotuner: persistent grids of 1, 2 and 4 workgroups per CU, timed
-      // on this batch (complete two-colour passes: the outputs are the same
-      // for every grid); the fastest is kept for the batch
-      const int  cand[3] = {1, 2, 4};
+      // autotuner: tiles (waves per workgroup of the register kernel, 4 or
+      // 8; the pair kernel has one) and persistent grids (workgroups per
+      // CU), each timed on this batch over a complete two-colour pass (the
+      // outputs are the same for every candidate); the fastest is kept
+      const int tiles4[2] = {4, 8}, per_cu4[2][3] = {{1, 2, 4}, {1, 2, 0}};
+      const int ntile = B->deg3 ? 1 : 2;
+      const int mx    = std::max(1, std::max(B->nchunks[0], B->nchunks[1]));
+      B->ncand        = 0;
+      for (int ti = 0; ti < ntile; ++ti)
+        for (int k = 0; k < 3; ++k)
+          if (per_cu4[ti][k] > 0)
+            {
+              B->cand_tile[B->ncand] = B->deg3 ? hdg::batch_pairs_per_group() : tiles4[ti];
+              B->cand_grid[B->ncand] = std::max(1, std::min(per_cu4[ti][k] * B->ncu, mx));
+              ++B->ncand;
+            }
       hipEvent_t a, b;
       if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
         return fail("hdg_batch_condense: event creation failed", -3);
       float best = 1e30f;
-      for (int k = 0; k < 3; ++k)
+      for (int k = 0; k < B->ncand; ++k)
         {
-          const int g = std::max(1, std::min(cand[k] * B->ncu, std::max(B->nchunks[0], B->nchunks[1])));
-          int       rc = batch_launch(B, thermodyn, mixed, g, s);
+          const int g = B->cand_grid[k], tl = B->cand_tile[k];
+          int       rc = batch_launch(B, thermodyn, mixed, g, tl, s);
           if (rc == 0)
             {
               (void)hipEventRecord(a, s);
-              rc = batch_launch(B, thermodyn, mixed, g, s);
+              rc = batch_launch(B, thermodyn, mixed, g, tl, s);
               (void)hipEventRecord(b, s);
               (void)hipEventSynchronize(b);
             }
@@ -863,21 +878,23 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
             {
               (void)hipEventDestroy(a);
               (void)hipEventDestroy(b);
+              B->grid = 0;
               return rc;
             }
           float ms = 0;
           (void)hipEventElapsedTime(&ms, a, b);
-          B->tuned_ms[k] = ms;
+          B->cand_ms[k] = ms;
           if (ms < best)
             {
               best    = ms;
               B->grid = g;
+              B->tile = tl;
             }
         }
       (void)hipEventDestroy(a);
       (void)hipEventDestroy(b);
     }
-  return batch_launch(B, thermodyn, mixed, B->grid, s);
+  return batch_launch(B, thermodyn, mixed, B->grid, B->tile, s);
 }
 
 int hdg_batch_info(const hdg_batch *B, double out[6])
@@ -887,9 +904,27 @@ int hdg_batch_info(const hdg_batch *B, double out[6])
   out[0] = (double)B->ncells;
   out[1] = (double)B->grid;
   out[2] = (double)B->ncu;
-  for (int k = 0; k < 3; ++k)
-    out[3 + k] = B->tuned_ms[k];
+  out[3] = (double)B->tile;
+  out[4] = (double)B->ncand;
+  float best = 0;
+  for (int k = 0; k < B->ncand; ++k)
+    if (B->cand_tile[k] == B->tile && B->cand_grid[k] == B->grid)
+      best = B->cand_ms[k];
+  out[5] = best;
   return 0;
+}
+
+int hdg_batch_tuning(const hdg_batch *B, double *out, int cap)
+{
+  if (!B || (cap > 0 && !out) || cap < 0)
+    return fail("hdg_batch_tuning: invalid argument");
+  for (int k = 0; k < B->ncand && k < cap; ++k)
+    {
+      out[3 * k]     = (double)B->cand_tile[k];
+      out[3 * k + 1] = (double)B->cand_grid[k];
+      out[3 * k + 2] = (double)B->cand_ms[k];
+    }
+  return B->ncand;
 }
 
 int hdg_max_abs4(const double *const *vectors, const long long *lengths, double *scratch,

@@ -41,6 +41,9 @@ namespace hdg
   namespace
   {
     constexpr int WPB = 4; // waves per workgroup
+    // cells per chunk of a batched work list; a batched workgroup of 4 or
+    // 8 waves (the tile the autotuner picks) takes one chunk at a time
+    constexpr int BCH = 8;
 #ifndef WPE
 #define WPE 2
 #endif
@@ -105,10 +108,10 @@ namespace hdg
       static constexpr int iSLOT = iNBK + NFC;    // NFC*NFC
       static constexpr int iPIV  = iSLOT + NFC * NFC; // NS pivot lanes
       static constexpr int WINT  = iPIV + NS;
-      static constexpr size_t bytes_for(bool pass2)
+      static constexpr size_t bytes_for(bool pass2, int nw = WPB)
       {
-        return (size_t)(TAB + WPB * (pass2 ? WDBL2 : WDBL)) * sizeof(double) +
-               (size_t)WPB * WINT * sizeof(int);
+        return (size_t)(TAB + nw * (pass2 ? WDBL2 : WDBL)) * sizeof(double) +
+               (size_t)nw * WINT * sizeof(int);
       }
     };
 
@@ -187,37 +190,38 @@ namespace hdg
 
     // the reference-cell tables of one mesh into LDS (one workgroup)
     template <int DEG, int NE>
-    __device__ __forceinline__ void stage_tables(const DevTables &tb, double *tab0, int tid)
+    __device__ __forceinline__ void stage_tables(const DevTables &tb, double *tab0, int tid,
+                                                 int nthreads = 64 * WPB)
     {
       using S          = F2<DEG, NE>;
       constexpr int NB = S::NB, NF = S::NF, QC = S::QC, QF = S::QF, NFC = S::NFC;
-      for (int i = tid; i < QC * NB; i += 64 * WPB)
+      for (int i = tid; i < QC * NB; i += nthreads)
         {
           tab0[S::tPHIC + i] = tb.phic[i];
           tab0[S::tPMC + i]  = tb.PhiMinvC[i];
         }
-      for (int i = tid; i < QC * 2 * NB; i += 64 * WPB)
+      for (int i = tid; i < QC * 2 * NB; i += nthreads)
         {
           tab0[S::tLDC + i]   = tb.LDc[i];
           tab0[S::tGRADC + i] = tb.gradc[i];
         }
-      for (int i = tid; i < NFC * QC * NF; i += 64 * WPB)
+      for (int i = tid; i < NFC * QC * NF; i += nthreads)
         tab0[S::tLCC + i] = tb.LCc[i];
-      for (int i = tid; i < NFC * QF * NB; i += 64 * WPB)
+      for (int i = tid; i < NFC * QF * NB; i += nthreads)
         {
           tab0[S::tPHIF + i] = tb.phif[i];
           tab0[S::tLDF + i]  = tb.LDf[i];
           tab0[S::tPMF + i]  = tb.PhiMinvF[i];
         }
-      for (int i = tid; i < NFC * NFC * QF * NF; i += 64 * WPB)
+      for (int i = tid; i < NFC * NFC * QF * NF; i += nthreads)
         tab0[S::tLCF + i] = tb.LCf[i];
-      for (int i = tid; i < QF * NF; i += 64 * WPB)
+      for (int i = tid; i < QF * NF; i += nthreads)
         tab0[S::tXI + i] = tb.xi[i];
-      for (int i = tid; i < QC; i += 64 * WPB)
+      for (int i = tid; i < QC; i += nthreads)
         tab0[S::tJXWC + i] = tb.JxWc[i];
-      for (int i = tid; i < NFC * QF; i += 64 * WPB)
+      for (int i = tid; i < NFC * QF; i += nthreads)
         tab0[S::tJXWF + i] = tb.JxWf[i];
-      for (int i = tid; i < 2 * NF * NF; i += 64 * WPB)
+      for (int i = tid; i < 2 * NF * NF; i += nthreads)
         tab0[S::tMFI + i] = tb.Mfinv[i];
     }
 
@@ -227,7 +231,7 @@ namespace hdg
     template <int DEG, int NE, bool THERMO, bool PASS2, bool MIXED>
     __device__ __forceinline__ void cell_body(const KParams &p, double *smem, int wstr,
                                               long long cell, long long idx, int color,
-                                              int mode)
+                                              int mode, int nw = WPB)
     {
       using S              = F2<DEG, NE>;
       constexpr int NB     = S::NB, NF = S::NF, QC = S::QC, QF = S::QF, NFC = S::NFC;
@@ -237,7 +241,7 @@ namespace hdg
       const int     tid  = threadIdx.x;
       const int     lane = tid & 63, wv = tid >> 6;
       double       *ws   = smem + S::TAB + wv * wstr;
-      int *wi = reinterpret_cast<int *>(smem + S::TAB + WPB * wstr) + wv * S::WINT;
+      int *wi = reinterpret_cast<int *>(smem + S::TAB + nw * wstr) + wv * S::WINT;
       long long *blk   = reinterpret_cast<long long *>(ws + S::wBLK);
       long long *facev = reinterpret_cast<long long *>(ws + S::wFACE);
       const DevTables &tb = p.tb;
@@ -945,15 +949,18 @@ namespace hdg
 
 
     // Pass 1 over the cells of several meshes (devices) of degree 1 or 2 in
-    // one persistent launch per colour: the work list holds chunks of WPB
+    // one persistent launch per colour: the work list holds chunks of BCH
     // cells of one device (padded with -1 at device boundaries); a workgroup
-    // restages the reference-cell tables when its chunk's device changes.
-    // LDS is sized for degree 2; a degree-1 device uses the first part.
-    template <int NE, bool THERMO, bool MIXED>
-    __global__ void __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(WPE)))
+    // of NW waves (the tile: 4 or 8) takes one chunk at a time, wave w its
+    // cells w, w + NW, ..., and restages the reference-cell tables when its
+    // chunk's device changes.  LDS is sized for degree 2; a degree-1 device
+    // uses the first part.
+    template <int NE, bool THERMO, bool MIXED, int NW>
+    __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE)))
       batch_condense(const KParams *__restrict__ plist, const int *__restrict__ chunk_dev,
                      const int *__restrict__ items, int nchunks, int color)
     {
+      static_assert(BCH % NW == 0, "a chunk splits evenly over the waves");
       extern __shared__ double smem[];
       const int wv  = threadIdx.x >> 6;
       int       cur = -1, deg = 0;
@@ -966,42 +973,62 @@ namespace hdg
               __syncthreads();
               deg = p.deg;
               if (deg == 1)
-                stage_tables<1, NE>(p.tb, smem, threadIdx.x);
+                stage_tables<1, NE>(p.tb, smem, threadIdx.x, 64 * NW);
               else
-                stage_tables<2, NE>(p.tb, smem, threadIdx.x);
+                stage_tables<2, NE>(p.tb, smem, threadIdx.x, 64 * NW);
               __syncthreads();
               cur = dev;
             }
-          const int cell = items[c * WPB + wv];
-          if (cell < 0)
-            continue;
-          if (deg == 1)
-            cell_body<1, NE, THERMO, false, MIXED>(p, smem, F2<1, NE>::WDBL, cell, cell, color, 0);
-          else
-            cell_body<2, NE, THERMO, false, MIXED>(p, smem, F2<2, NE>::WDBL, cell, cell, color, 0);
+#pragma unroll 1
+          for (int w = wv; w < BCH; w += NW)
+            {
+              const int cell = items[c * BCH + w];
+              if (cell < 0)
+                continue;
+              if (deg == 1)
+                cell_body<1, NE, THERMO, false, MIXED>(p, smem, F2<1, NE>::WDBL, cell, cell,
+                                                       color, 0, NW);
+              else
+                cell_body<2, NE, THERMO, false, MIXED>(p, smem, F2<2, NE>::WDBL, cell, cell,
+                                                       color, 0, NW);
+            }
         }
     }
 
-    template <int NE, bool THERMO, bool MIXED>
-    hipError_t launch_batch_t(const KParams *plist, const int *chunk_dev, const int *items,
-                              int nchunks, int color, int grid, hipStream_t s)
+    template <int NE, bool THERMO, bool MIXED, int NW>
+    hipError_t launch_batch_nw(const KParams *plist, const int *chunk_dev, const int *items,
+                               int nchunks, int color, int grid, hipStream_t s)
     {
-      constexpr size_t bytes = F2<2, NE>::bytes_for(false);
-      static bool      attr  = false;
+      constexpr size_t bytes = F2<2, NE>::bytes_for(false, NW);
+      static_assert(bytes <= 160 * 1024, "LDS of one workgroup");
+      static bool attr = false;
       if (!attr)
         {
-          hipError_t e = hipFuncSetAttribute((const void *)batch_condense<NE, THERMO, MIXED>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)bytes);
+          hipError_t e =
+            hipFuncSetAttribute((const void *)batch_condense<NE, THERMO, MIXED, NW>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
           if (e != hipSuccess)
             return e;
           attr = true;
         }
       if (nchunks <= 0)
         return hipSuccess;
-      hipLaunchKernelGGL((batch_condense<NE, THERMO, MIXED>), dim3(grid), dim3(64 * WPB), bytes,
-                         s, plist, chunk_dev, items, nchunks, color);
+      hipLaunchKernelGGL((batch_condense<NE, THERMO, MIXED, NW>), dim3(grid), dim3(64 * NW),
+                         bytes, s, plist, chunk_dev, items, nchunks, color);
       return hipGetLastError();
+    }
+
+    template <int NE, bool THERMO, bool MIXED>
+    hipError_t launch_batch_t(const KParams *plist, const int *chunk_dev, const int *items,
+                              int nchunks, int color, int grid, int tile, hipStream_t s)
+    {
+      if (tile == 8)
+        return launch_batch_nw<NE, THERMO, MIXED, 8>(plist, chunk_dev, items, nchunks, color,
+                                                     grid, s);
+      if (tile == 4)
+        return launch_batch_nw<NE, THERMO, MIXED, 4>(plist, chunk_dev, items, nchunks, color,
+                                                     grid, s);
+      return hipErrorInvalidValue;
     }
 
     template <int DEG, int NE, bool THERMO, bool PASS2 = false, bool MIXED = false>
@@ -1062,20 +1089,24 @@ namespace hdg
 
   int batch_waves_per_group()
   {
-    return WPB;
+    return BCH;
   }
 
   hipError_t launch_batch_condense(int ne, int thermo, int mixed, const KParams *plist,
                                    const int *chunk_dev, const int *items, int nchunks,
-                                   int color, int grid, hipStream_t s)
+                                   int color, int grid, int tile, hipStream_t s)
   {
     if (thermo)
-      return mixed ? launch_batch_t<1, true, true>(plist, chunk_dev, items, nchunks, color, grid, s)
-                   : launch_batch_t<1, true, false>(plist, chunk_dev, items, nchunks, color, grid, s);
+      return mixed ? launch_batch_t<1, true, true>(plist, chunk_dev, items, nchunks, color, grid,
+                                                   tile, s)
+                   : launch_batch_t<1, true, false>(plist, chunk_dev, items, nchunks, color,
+                                                    grid, tile, s);
     if (ne != 3)
       return hipErrorInvalidValue;
-    return mixed ? launch_batch_t<3, false, true>(plist, chunk_dev, items, nchunks, color, grid, s)
-                 : launch_batch_t<3, false, false>(plist, chunk_dev, items, nchunks, color, grid, s);
+    return mixed ? launch_batch_t<3, false, true>(plist, chunk_dev, items, nchunks, color, grid,
+                                                  tile, s)
+                 : launch_batch_t<3, false, false>(plist, chunk_dev, items, nchunks, color, grid,
+                                                   tile, s);
   }
 
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s)

@@ -83,11 +83,11 @@ namespace hdg
   // pass 1 of several meshes in one persistent launch per colour (register
   // kernel, degree 1 or 2): plist (device) holds one parameter block per
   // mesh, chunk_dev / items (device) the work list of nchunks chunks of
-  // batch_waves_per_group() cells
+  // batch_waves_per_group() cells; tile: waves per workgroup (4 or 8)
   int        batch_waves_per_group();
   hipError_t launch_batch_condense(int ne, int thermo, int mixed, const KParams *plist,
                                    const int *chunk_dev, const int *items, int nchunks,
-                                   int color, int grid, hipStream_t s);
+                                   int color, int grid, int tile, hipStream_t s);
   // register-resident pass 1 for 3D degree 1 (hdg_fast3d.hip): two columns
   // per lane, all three components or the V-only equilibrium operator
   bool       fast3_supported(int dim, int deg, int ne, int thermo, const int en[3]);

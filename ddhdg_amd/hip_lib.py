@@ -30,7 +30,7 @@ EXPORTS = [
     "hdg_trace_plan_selftest", "hdg_terminal_current", "hdg_set_precision",
     "hdg_batch_create", "hdg_batch_destroy", "hdg_batch_condense", "hdg_batch_info",
     "hdg_max_abs4", "hdg_set_cell_face_tau", "hdg_max_abs4_noalloc", "hdg_trace_plan_size",
-    "hdg_local_system", "hdg_gmres_solve",
+    "hdg_local_system", "hdg_gmres_solve", "hdg_batch_tuning",
 ]
 
 (HDG_TS_FLOPS, HDG_TS_BYTES, HDG_TS_LEVELS, HDG_TS_MAX_FRONT, HDG_TS_NODES,
@@ -94,6 +94,7 @@ def lib():
             "hdg_batch_destroy": (None, [v]),
             "hdg_batch_condense": (i, [v, i, P(v), P(v), P(v), P(v), P(v), v]),
             "hdg_batch_info": (i, [v, dp]),
+            "hdg_batch_tuning": (i, [v, dp, i]),
             "hdg_max_abs4": (i, [P(v), lp, v, v, v]),
             "hdg_max_abs4_noalloc": (i, [P(v), lp, v, v]),
             "hdg_trace_plan_size": (i, [i, ip, i, dp]),
@@ -411,10 +412,17 @@ class HdgBatch:
             arr(dirv, "dir_vals", lambda c: c.ntrace), _stream()))
 
     def info(self):
+        """the autotuner's choice (after the first condense): cells, grid
+        (workgroups), CUs, tile (waves per workgroup, or pairs for degree 3)
+        and every candidate (tile, grid, ms of a two-colour pass)"""
         out = np.zeros(6)
         _check(self.L.hdg_batch_info(self.h, _dp(out)))
+        cand = np.zeros(3 * 8)
+        n = _check(self.L.hdg_batch_tuning(self.h, _dp(cand), 8))
         return {"cells": int(out[0]), "grid": int(out[1]), "cus": int(out[2]),
-                "tuned_ms": [float(x) for x in out[3:]]}
+                "tile": int(out[3]), "ms": float(out[5]),
+                "candidates": [{"tile": int(cand[3 * k]), "grid": int(cand[3 * k + 1]),
+                                "ms": float(cand[3 * k + 2])} for k in range(min(n, 8))]}
 
 
 def max_abs4(vectors, scratch, out):

@@ -249,10 +249,12 @@ int hdg_max_abs4_noalloc(const double *const *vectors, const long long *lengths,
  * components: degree 1 and 2 mixed, or all of degree 3 with V, n and p) in
  * one persistent launch per colour; each context's outputs are exactly
  * those of its own hdg_condense.  A degree-3 batch runs in FP64 and the
- * full operator only.  The
- * persistent grid is picked by timing 1, 2 and 4 workgroups per CU on the
- * first call (hdg_batch_info reports the choice).  Pointer arguments are
- * host arrays of n device pointers, in the order of ctxs.  A call with the
+ * full operator only.  The autotuner picks the tile (waves per workgroup
+ * of the degree 1-2 kernel: 4 or 8, one cell per wave) and the persistent
+ * grid (workgroups per CU) by timing each candidate on the first call
+ * (hdg_batch_info reports the choice, hdg_batch_tuning every candidate).
+ * Pointer arguments are host arrays of n device pointers, in the order of
+ * ctxs.  A call with the
  * buffers and data of the previous call reuses its uploaded parameter
  * blocks, so it can be captured into a HIP graph (after an eager first
  * call); a captured call with other buffers returns -2.  No counterpart in
@@ -266,9 +268,14 @@ int  hdg_batch_condense(hdg_batch *batch, int thermodyn,
                         const double *const *x_trace, double *const *csr_vals,
                         double *const *rhs, double *const *dir_vals,
                         void *stream);
-/* out = {cells, persistent workgroups chosen, CUs, ms of the three timed
- * candidates} */
+/* out = {cells, persistent workgroups chosen, CUs, tile chosen (waves per
+ * workgroup; pairs of waves for degree 3), candidates timed, ms of a
+ * two-colour pass with the choice} */
 int  hdg_batch_info(const hdg_batch *batch, double out[6]);
+/* The autotuner's candidates: out[3k..3k+2] = {tile, grid, ms} for k <
+ * min(cap, candidates); returns the number of candidates (0 before the
+ * first hdg_batch_condense). */
+int  hdg_batch_tuning(const hdg_batch *batch, double *out, int cap);
 
 /* ---- Global trace solve ------------------------------------------------
  * Sparse direct LU of the restricted trace system, on the GPU: a nested-
