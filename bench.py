@@ -32,6 +32,15 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PER_LAUNCH = {(256, 2): (101692.3 * 2.0 + 378050.0 * 1.0) * 1024}
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    """one line on stderr as each part of the run ends (stdout keeps the
+    single JSON line)"""
+    print(f"[bench {time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def flops_per_cell(N, T):
     """dense algorithmic count for pass 1: LU of the N x N cell block,
     solves for the T trace columns and the rhs, Schur product."""
@@ -171,6 +180,7 @@ def batched_sample(args, torch, stream):
         del groups, bufs
         # the trace factors are allocated by the library, outside torch's cache
         torch.cuda.empty_cache()
+        progress("batched pass 1")
         out["newton"] = batched_newton(torch, sv)
     out["workload"] = (f"{len(sv)} 2D diodes from seed 20261203: sides uniform in 16..192 cells, "
                        "degree uniform in {1, 2, 3}, doping log-uniform 1e21-1e24, junction in "
@@ -234,6 +244,7 @@ def batched_newton(torch, sv):
         return rec
 
     batched = workflow(True)
+    progress("batched Newton workflow")
     final = [(s.x_cell.clone(), s.x_trace.clone()) for s in sv]
     for s, (c, tr, en) in zip(sv, init):
         s.set_enabled_components(*en)
@@ -320,8 +331,10 @@ def sample_3d(args, torch, stream):
                                                                      False, dxc), reps)
     # the first Newton stage at full size: the thermodynamic equilibrium (V
     # only; its trace factors take 63 GB, a coupled step's would take 567 GB)
+    progress("3D degree-1 pass 1 and pass 2")
     s.set_enabled_components(True, False, False)
     out["newton_equilibrium"] = newton_phases(torch, stream, s, True, 2)
+    progress("3D degree-1 equilibrium Newton step")
     s.set_enabled_components(True, True, True)
     del s
     # a coupled Newton step of the same device at 24^3
@@ -504,6 +517,7 @@ def main():
     kernel_ms = ev0.elapsed_time(ev1) / args.steps       # two color launches per step
     ms_per_step = wall * 1e3 / args.steps
     cells = ctx.ncells
+    progress(f"FP64 pass 1: {ms_per_step:.3f} ms per step")
     value = cells * args.steps / wall
     dxt = torch.zeros_like(s.x_trace)
     dxc = torch.empty_like(s.x_cell)
@@ -536,6 +550,7 @@ def main():
     err_S = float(((Sm - Sd).norm(dim=1) / Sd.norm(dim=1)).max())
     err_g = float(((gm - gd).norm(dim=1) / gd.norm(dim=1).clamp_min(1e-300)).max())
     m_ms = e0m.elapsed_time(e1m) / args.steps / 2.0
+    progress(f"FP32+refine pass 1: {wall_m * 1e3 / args.steps:.3f} ms per step")
     # the dense count split by precision: LU and solves in FP32; the
     # refinement residual and the Schur product in FP64
     f32 = 2 * N ** 3 / 3 + 2 * N ** 2 * (T + 1)
@@ -587,6 +602,7 @@ def main():
         torch.cuda.synchronize()
         newton_ms = (time.perf_counter() - w0) * 1e3 / n_newton
         phase_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v) for k, v in phases.items()}
+        progress(f"Newton step: {newton_ms:.2f} ms")
 
     graph_step_ms = None
     if n_newton > 0:
@@ -644,11 +660,14 @@ def main():
                              "factor_TFLOPs": ts.flops / (phase_ms["factor"] * 1e-3) / 1e12}}
     if args.cells3d > 0:
         line["sample_3d"] = sample_3d(args, torch, stream)
+        progress("3D degree-1 sample")
     if args.cells3d_k2 > 0:
         line["sample_3d_k2"] = sample_3d_k2(args, torch, stream)
+        progress("3D degree-2 sample")
         torch.cuda.empty_cache()
     if args.batch > 0:
         line["batched"] = batched_sample(args, torch, stream)
+        progress("batched devices")
     if args.serve > 0:
         # the served workload: the first requests of the seeded stream
         # (Poisson arrivals, lognormal 2D meshes 32^2-256^2, 20 % 3D cubes
@@ -664,6 +683,7 @@ def main():
                            precision=prec)
             res, wall = srv.run(reqs)
             runs[prec] = (srv, res, wall)
+            progress(f"served stream, precision {prec}: {wall:.1f} s")
         keys = ("solves_per_s", "p50_latency_s", "p99_latency_s", "converged", "requests",
                 "wall_s", "newton_iterations")
 
@@ -703,7 +723,9 @@ def main():
         # all the cores this process may use (the box grants 16 per GPU), and one
         threads = max(1, min(host["usable"], 16))
         v, nc, t = cpu_baseline(args.cpu_sample, args.degree, threads, args.bias)
+        progress(f"CPU baseline, {threads} threads: {t:.2f} s per run")
         v1, nc1, t1 = cpu_baseline(args.cpu_sample_1core, args.degree, 1, args.bias)
+        progress(f"CPU baseline, one core: {t1:.2f} s per run")
         line["cpu_baseline"] = {
             "value": v, "unit": "cells/s", "cores": threads, "kind": "port",
             "sample": f"oracle pass 1 (full-inverse Gauss-Jordan + mmult, WorkStream threads, "
