@@ -8,8 +8,9 @@
 // level.  A front's eliminated block is taken in panels of at most 72 dofs
 // (whole faces): per panel, the LU of its diagonal block with partial
 // pivoting inside the panel (one workgroup per front, in LDS), the row
-// interchanges applied to the panel's rows right of it, the two batched
-// triangular solves (U12 = L11^-1 P F12, L21 = F21 U11^-1) and one rocBLAS
+// interchanges applied to the panel's rows right of it, the two
+// triangular solves (U12 = L11^-1 P F12, L21 = F21 U11^-1; blocked on the
+// matrix cores in groups of few fronts) and one rocBLAS
 // strided-batched gemm (F22 -= L21 U12) on the rest of the front.  That is
 // the elimination of a chain of nodes of at most 72 dofs each, with
 // pivoting inside each node as in any multifrontal code.  An extend-add of
@@ -34,6 +35,8 @@ namespace
   using hdg::lane_value;
   using hdg::leading_bits;
   using hdg::wave_max;
+  using hdg::dbl4;
+  using hdg::mfma_f64_16x16x4;
 
   struct DevGroup
   {
@@ -200,6 +203,109 @@ namespace
       info[b] = o + bad;
   }
 
+  // The same factorisation with four waves per front, for groups of few
+  // fronts (where one wave per front leaves the GPU idle).  Lane i of every
+  // wave is row i; wave w holds columns w, w+4, w+8, ... in SM/4 registers,
+  // loaded straight from the front (coalesced down each column).  Step k:
+  // the wave holding column k picks the pivot (DPP max over the unpivoted
+  // rows), stores its multipliers l_i = a_ik / a_pk in place and publishes
+  // them and p in LDS (two buffers, one barrier a step); then every wave
+  // updates its columns right of k, a_ij -= l_i a_pj (a_pj by readlane).
+  // The steps run four at a time in a loop that is not unrolled (the
+  // unrolled 64 steps outgrow the instruction cache): after each four, the
+  // finished column of every wave goes to LDS and its registers rotate by
+  // one, so that a wave's next column is always register 0.  Same pivots,
+  // multipliers, output rows and zero-pivot report as k_panel_lu.
+  template <int SM>
+  __global__ void __launch_bounds__(256)
+    k_panel_lu4(double *F, long long fbase, int n, int S, int *perm, long long pbase, int ps,
+                int o, int *info, int batch)
+  {
+    constexpr int NM = SM / 4, LC = 65;
+    __shared__ double lb[2][64];
+    __shared__ double Cs[SM * LC]; // finished columns, Cs[j * LC + i]
+    __shared__ int    pb[2], badk;
+    const int b = blockIdx.x;
+    if (b >= batch)
+      return;
+    const int tid = threadIdx.x, i = tid & 63, w = tid >> 6;
+    double   *Fb = F + fbase + (long long)b * n * n;
+    double    A[NM];
+    {
+      const int ic = i < S ? i : 0;
+#pragma unroll
+      for (int m = 0; m < NM; ++m)
+        {
+          const int    j = 4 * m + w;
+          const double v = Fb[ic + (long long)(j < S ? j : 0) * n];
+          A[m]           = (j < S && i < S) ? v : 0.0;
+        }
+    }
+    if (tid == 0)
+      badk = 0;
+    bool pivoted = i >= S;
+    int  myk     = 0;
+    __syncthreads();
+    const int nk4 = (S + 3) / 4;
+#pragma unroll 1
+    for (int k4 = 0; k4 < nk4; ++k4)
+      {
+        // this wave's register m holds column 4 (k4 + m) + w
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+          {
+            const int k = 4 * k4 + kk;
+            if (k >= S)
+              break;
+            if (w == kk)
+              {
+                const double   v   = A[0];
+                const unsigned key = wave_max(row_key(v, !pivoted, i));
+                const int      p   = 63 - (int)((key - 1u) & 63u);
+                const double   d   = lane_value(v, p);
+                const bool     upd = !pivoted && i != p;
+                const double   l   = (upd && d != 0.0) ? v / d : 0.0;
+                if (upd)
+                  A[0] = l;
+                lb[k & 1][i] = l;
+                if (i == 0)
+                  {
+                    pb[k & 1] = p;
+                    if (d == 0.0 && badk == 0)
+                      badk = k + 1;
+                  }
+              }
+            __syncthreads();
+            const int    p = __builtin_amdgcn_readfirstlane(pb[k & 1]);
+            const double l = lb[k & 1][i];
+            if (i == p)
+              {
+                pivoted = true;
+                myk     = k;
+              }
+#pragma unroll
+            for (int m = 0; m < NM; ++m)
+              if (4 * m + w > kk && m < NM - k4)
+                A[m] -= l * lane_value(A[m], p);
+          }
+        // column 4 k4 + w is final: out to LDS, then the rotation
+        Cs[(4 * k4 + w) * LC + i] = A[0];
+#pragma unroll
+        for (int m = 0; m + 1 < NM; ++m)
+          A[m] = A[m + 1];
+      }
+    __syncthreads();
+    if (i < S)
+      {
+        for (int j = w; j < S; j += 4)
+          Fb[myk + (long long)j * n] = Cs[j * LC + i];
+        if (w == 0)
+          perm[pbase + (long long)b * ps + myk] = i;
+      }
+    if (tid == 0 && badk && info[b] == 0)
+      info[b] = o + badk;
+  }
+
   // The off-diagonal blocks of one panel, one launch: blocks x < nbl (one
   // lane per column, the Ic columns from column c0 of the panel's trailing
   // front) form U12 = L11^-1 P F12 (the column's rows gathered in pivot
@@ -284,6 +390,167 @@ namespace
           for (int j = 0; j < PW_MAX; ++j)
             if (j < S)
               row[(long long)j * n] = y[j];
+      }
+  }
+
+  // The same blocks on the matrix cores, for groups of at most
+  // TRSM_MFMA_MAX_BATCH fronts (where one lane per column leaves the GPU
+  // idle), one launch:
+  // one workgroup (four waves) per tile of 64 columns of F12 (blocks x <
+  // nbl) or 64 rows of F21 of one front.  Both are a lower-triangular solve M Y = B with the panel's
+  // factor staged in LDS:
+  //   U12: M = L11 (unit lower), B = P F12, the rows gathered in pivot order;
+  //   L21: M = U11^T, B = F21^T (L21^T = U11^-T F21^T).
+  // Blocked by 16: wave 0 inverts the diagonal 16 x 16 blocks M_ii (lane =
+  // 16 i + column, a 16-step substitution); then wave w solves columns
+  // 16w..16w+15 of the tile block row by block row,
+  //   Y_i = M_ii^-1 (B_i - sum_{k<i} M_ik Y_k),
+  // every product on v_mfma_f64_16x16x4f64 (A[i][k] in lane i + 16k, B[k][j]
+  // in lane j + 16k, element e of the result = D[4e + lane/16][lane%16]).
+  // Element kk of a result is its B operand for the K step kk (rows
+  // 4kk..4kk+3), so the Y_k stay in registers.  The tile is loaded and stored
+  // coalesced through LDS.  SM >= S bounds the blocks; the factor is padded
+  // with the identity and the tile with zeros.
+  template <int SM>
+  __global__ void __launch_bounds__(256)
+    k_trsm_mfma(double *F, long long fbase, int n, int S, int Ic, int c0, int Ir,
+                const int *__restrict__ perm, long long pbase, int ps, int nbl, int batch,
+                int yoff)
+  {
+    constexpr int LF = SM + 1, LB = 65, NB = SM / 16, LD = 17;
+    extern __shared__ double sm[];
+    double       *Fs = sm;                 // SM x LF: the factored block, row-major
+    double       *Ds = Fs + SM * LF;       // NB x 16 x LD: inverses of the diagonal blocks
+    double       *Bs = Ds + NB * 16 * LD;  // SM x LB: B (row j, column c of the tile)
+    const int b = blockIdx.y + yoff;
+    if (b >= batch)
+      return;
+    const bool left = (int)blockIdx.x < nbl;
+    const int  tile = (left ? (int)blockIdx.x : (int)blockIdx.x - nbl) * 64;
+    const int  nvr  = (left ? Ic : Ir) - tile;
+    const int  nv   = nvr < 64 ? nvr : 64;
+    const int  tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    double    *Fb  = F + fbase + (long long)b * n * n;
+    // every load of a wave is issued before its LDS stores (one HBM round
+    // trip, not SM/4)
+    constexpr int NJ = SM / 4;
+    {
+      double v[NJ];
+      const int li = lane < S ? lane : 0;
+#pragma unroll
+      for (int jj = 0; jj < NJ; ++jj)
+        {
+          const int j = 4 * jj + w;
+          v[jj]       = Fb[li + (long long)(j < S ? j : 0) * n];
+        }
+      if (lane < SM)
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj)
+          {
+            const int j        = 4 * jj + w;
+            Fs[lane * LF + j] = (lane < S && j < S) ? v[jj] : (lane == j ? 1.0 : 0.0);
+          }
+    }
+    if (left)
+      {
+        // lane j: row perm[j] of column c (a permutation inside the column)
+        const int pj = lane < S ? perm[pbase + (long long)b * ps + lane] : 0;
+        double    v[16];
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc)
+          {
+            const int c = 4 * cc + w;
+            v[cc]       = Fb[pj + (long long)(c0 + tile + (c < nv ? c : 0)) * n];
+          }
+        if (lane < SM)
+#pragma unroll
+          for (int cc = 0; cc < 16; ++cc)
+            {
+              const int c       = 4 * cc + w;
+              Bs[lane * LB + c] = (lane < S && c < nv) ? v[cc] : 0.0;
+            }
+      }
+    else
+      {
+        // lane r: row S + tile + r of F21, column j (transposed into Bs)
+        const int ri = lane < nv ? lane : 0;
+        double    v[NJ];
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj)
+          {
+            const int j = 4 * jj + w;
+            v[jj]       = Fb[S + tile + ri + (long long)(j < S ? j : 0) * n];
+          }
+#pragma unroll
+        for (int jj = 0; jj < NJ; ++jj)
+          {
+            const int j      = 4 * jj + w;
+            Bs[j * LB + lane] = (lane < nv && j < S) ? v[jj] : 0.0;
+          }
+      }
+    __syncthreads();
+    // M[r][c] (r >= c) from the factored block
+    auto mval = [&](int r, int c) -> double {
+      return left ? (r == c ? 1.0 : Fs[r * LF + c]) : Fs[c * LF + r];
+    };
+    if (w == 0)
+      {
+        const int bi = lane >> 4, j = lane & 15, o = 16 * bi;
+        if (bi < NB)
+          {
+            double x[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+              {
+                double a = r == j ? 1.0 : 0.0;
+#pragma unroll
+                for (int m = 0; m < r; ++m)
+                  a = fma(-mval(o + r, o + m), x[m], a);
+                x[r] = a / mval(o + r, o + r);
+              }
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+              Ds[(o + r) * LD + j] = x[r];
+          }
+      }
+    __syncthreads();
+    const int q = lane >> 4, c16 = lane & 15;
+    dbl4      Y[NB];
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+      {
+        dbl4 acc;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          acc[e] = Bs[(16 * i + 4 * e + q) * LB + 16 * w + c16];
+#pragma unroll
+        for (int k = 0; k < i; ++k)
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk)
+            acc = mfma_f64_16x16x4(-mval(16 * i + c16, 16 * k + 4 * kk + q), Y[k][kk], acc);
+        Y[i] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk)
+          Y[i] = mfma_f64_16x16x4(Ds[(16 * i + c16) * LD + 4 * kk + q], acc[kk], Y[i]);
+      }
+    // each wave wrote and reads only its own 16 columns of Bs
+#pragma unroll
+    for (int i = 0; i < NB; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        Bs[(16 * i + 4 * e + q) * LB + 16 * w + c16] = Y[i][e];
+    __syncthreads();
+    if (left)
+      {
+        for (int c = w; c < nv; c += 4)
+          if (lane < S)
+            Fb[lane + (long long)(c0 + tile + c) * n] = Bs[lane * LB + c];
+      }
+    else
+      {
+        for (int j = w; j < S; j += 4)
+          if (lane < nv)
+            Fb[S + tile + lane + (long long)j * n] = Bs[j * LB + lane];
       }
   }
 
@@ -543,6 +810,18 @@ namespace
       }
   }
 
+  // dynamic LDS of k_trsm_mfma<SM>: the factor SM x (SM+1), the diagonal
+  // inverses SM x 17 and the tile SM x 65 doubles
+  constexpr size_t trsm_mfma_lds(int SM)
+  {
+    return (size_t)(SM * (SM + 1) + SM * 17 + SM * 65) * sizeof(double);
+  }
+
+  // groups of at most this many fronts take the triangular solves and the
+  // panel LU in four-wave workgroups (k_trsm_mfma, k_panel_lu4); larger
+  // groups keep one wave per front and panel (enough waves to fill the GPU)
+  constexpr int TRSM_MFMA_MAX_BATCH = 1024;
+
   constexpr int YCHUNK = 32768; // nodes per launch along grid y
   // a sweep's off-diagonal update stays in the one-wave panel kernel up to
   // this many trailing rows, and goes to a batched gemv above
@@ -653,6 +932,20 @@ int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
     T->pw = std::max(1, PW_MAX / std::max(1, P.tdf)) * std::max(1, P.tdf);
   if (rc == 0 && T->pw > PW_MAX)
     rc = hdg::set_error("hdg_trace_solver_create: more than 64 dofs per face", -4);
+  if (rc == 0)
+    {
+      // k_trsm_mfma<64> takes more than the default 64 KiB of dynamic LDS
+      static bool attr = false;
+      if (!attr)
+        {
+          if (hipFuncSetAttribute((const void *)k_trsm_mfma<64>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)trsm_mfma_lds(64)) != hipSuccess)
+            rc = hdg::set_error("hdg_trace_solver_create: LDS attribute", -3);
+          else
+            attr = true;
+        }
+    }
   if (rc == 0 && rocblas_create_handle(&T->h) != rocblas_status_success)
     rc = hdg::set_error("hdg_trace_solver_create: rocblas_create_handle failed", -3);
   if (rc != 0)
@@ -725,6 +1018,18 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
       const long long nn = (long long)G.n * G.n;
       auto lu = [&](int q0, int Sp) {
         const long long fb = G.fbase + (long long)q0 * G.n + q0;
+        if (G.batch <= TRSM_MFMA_MAX_BATCH)
+          {
+            if (Sp <= 32)
+              hipLaunchKernelGGL(k_panel_lu4<32>, dim3(G.batch), dim3(256), 0, s, T->d_F, fb,
+                                 G.n, Sp, T->d_perm, G.pbase + q0, G.S, q0, T->d_info + info_off,
+                                 G.batch);
+            else
+              hipLaunchKernelGGL(k_panel_lu4<PW_MAX>, dim3(G.batch), dim3(256), 0, s, T->d_F, fb,
+                                 G.n, Sp, T->d_perm, G.pbase + q0, G.S, q0, T->d_info + info_off,
+                                 G.batch);
+            return;
+          }
         for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
           {
             const dim3   grid(std::min(YCHUNK, G.batch - y0));
@@ -750,6 +1055,27 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
         const int       nbc = (Ic + 63) / 64, nbr = (Ir + 63) / 64;
         if (nbc + nbr == 0)
           return;
+        if (G.batch <= TRSM_MFMA_MAX_BATCH)
+          {
+            const dim3 gt(nbc + nbr, G.batch);
+            if (Sp <= 16)
+              hipLaunchKernelGGL(k_trsm_mfma<16>, gt, dim3(256), trsm_mfma_lds(16), s, T->d_F,
+                                 fb, G.n, Sp, Ic, c0, Ir, T->d_perm, G.pbase + q0, G.S, nbc,
+                                 G.batch, 0);
+            else if (Sp <= 32)
+              hipLaunchKernelGGL(k_trsm_mfma<32>, gt, dim3(256), trsm_mfma_lds(32), s, T->d_F,
+                                 fb, G.n, Sp, Ic, c0, Ir, T->d_perm, G.pbase + q0, G.S, nbc,
+                                 G.batch, 0);
+            else if (Sp <= 48)
+              hipLaunchKernelGGL(k_trsm_mfma<48>, gt, dim3(256), trsm_mfma_lds(48), s, T->d_F,
+                                 fb, G.n, Sp, Ic, c0, Ir, T->d_perm, G.pbase + q0, G.S, nbc,
+                                 G.batch, 0);
+            else
+              hipLaunchKernelGGL(k_trsm_mfma<64>, gt, dim3(256), trsm_mfma_lds(64), s, T->d_F,
+                                 fb, G.n, Sp, Ic, c0, Ir, T->d_perm, G.pbase + q0, G.S, nbc,
+                                 G.batch, 0);
+            return;
+          }
         for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
           hipLaunchKernelGGL(k_panel_trsm, dim3(nbc + nbr, std::min(YCHUNK, G.batch - y0)),
                              dim3(64), 0, s, T->d_F, fb, G.n, Sp, Ic, c0, Ir, T->d_perm,
