@@ -108,7 +108,7 @@ def cpu_baseline(n_sample, degree, threads, bias, reps=5):
 def batched_sample(args, torch, stream):
     """pass 1 of many devices of uneven sizes and degrees: the degree-1 and
     degree-2 devices in one persistent launch per colour, the degree-3
-    devices (pair kernel, FP64) in another (autotuned grids); against all
+    devices (pair kernel) in another (autotuned grids); against all
     devices condensed one after the other (two launches each).  FP64 and
     FP32 + refinement."""
     import numpy as np
@@ -185,8 +185,7 @@ def batched_sample(args, torch, stream):
     out["workload"] = (f"{len(sv)} 2D diodes from seed 20261203: sides uniform in 16..192 cells, "
                        "degree uniform in {1, 2, 3}, doping log-uniform 1e21-1e24, junction in "
                        "[0.3, 0.7], contact bias uniform in [0, 0.6] V; pass 1; degrees 1-2 in one "
-                       "persistent launch per colour, degree 3 (pair kernel, FP64: no FP32 path "
-                       "there) in another")
+                       "persistent launch per colour, degree 3 (pair kernel) in another")
     return out
 
 
@@ -257,7 +256,8 @@ def batched_newton(torch, sv):
     for s in sv:
         s.parameters.graph_newton = False
         s.ctx.set_precision(0)
-    out = {"precision": "fp32_refine (degree 3: FP64, no FP32 path in the pair kernel)",
+    out = {"precision": "fp32_refine (every degree; the equilibrium stage's degree-3 "
+                        "devices on the generic kernel in FP64)",
            "states_bitwise_equal": bool(same)}
     for name in batched:
         b, q = batched[name], sequential[name]

@@ -95,10 +95,10 @@ namespace
            hdg::fast3_supported(C->d, C->k, C->ne, thermodyn, C->en);
   }
 
-  // pass 2 on the register-resident kernels (2D degree 1-2, 3D degree 1)
+  // pass 2 on the register-resident kernels (2D degree 1-3, 3D degree 1)
   bool use_fast_pass2(const hdg_ctx *C, int thermodyn)
   {
-    return use_fast(C, thermodyn) && ((C->d == 2 && C->k <= 2) || (C->d == 3 && C->k == 1));
+    return use_fast(C, thermodyn) && ((C->d == 2 && C->k <= 3) || (C->d == 3 && C->k == 1));
   }
 
   hipError_t launch_pass1(const hdg_ctx *C, bool fast, const hdg::KParams &p, const int *cells,
@@ -564,11 +564,12 @@ int hdg_reconstruct(hdg_ctx *C, const double *x_cell, const double *x_trace,
   p.xt  = x_trace;
   p.dxt = dx_trace;
   p.dxc = dx_cell;
-  hipError_t e = use_fast_pass2(C, thermodyn)
-                   ? (C->d == 3 ? hdg::launch_fast3_reconstruct(p, (int)C->ncells, (hipStream_t)stream)
-                                : hdg::launch_fast_reconstruct(C->k, p, (int)C->ncells,
-                                                               (hipStream_t)stream))
-                   : hdg::launch_reconstruct(C->d, C->k, p, (int)C->ncells, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t  e = !use_fast_pass2(C, thermodyn)
+                    ? hdg::launch_reconstruct(C->d, C->k, p, (int)C->ncells, s)
+                  : C->d == 3 ? hdg::launch_fast3_reconstruct(p, (int)C->ncells, s)
+                  : C->k == 3 ? hdg::launch_fast2p_reconstruct(p, (int)C->ncells, s)
+                              : hdg::launch_fast_reconstruct(C->k, p, (int)C->ncells, s);
   return e == hipSuccess ? 0 : hip_fail(e, "hdg_reconstruct launch");
 }
 
@@ -646,7 +647,7 @@ struct hdg_batch
   std::vector<hdg_ctx *> ctx;
   int                    ne = 0, thermo_ok = 0, full_ok = 0, wpb = 4, ncu = 256;
   int                    en[3] = {0, 0, 0}; // enabled components at creation
-  int                    deg3 = 0; // degree-3 meshes: the pair kernel, FP64
+  int                    deg3 = 0; // degree-3 meshes: the pair kernel
   int                   *d_chunk_dev[2] = {nullptr, nullptr}, *d_items[2] = {nullptr, nullptr};
   int                    nchunks[2] = {0, 0};
   long long              ncells = 0;
@@ -670,7 +671,7 @@ namespace
       {
         hipError_t e =
           B->deg3 ? hdg::launch_batch_pair(B->d_params, B->d_chunk_dev[col], B->d_items[col],
-                                           B->nchunks[col], col, grid, s)
+                                           B->nchunks[col], col, grid, mixed, s)
                   : hdg::launch_batch_condense(B->ne, thermodyn, mixed, B->d_params,
                                                B->d_chunk_dev[col], B->d_items[col],
                                                B->nchunks[col], col, grid, tile, s);

@@ -926,8 +926,8 @@ namespace hdg
 
     // PASS2: the reconstruction pass (cells = all cells in order, columns =
     // scalar dofs + rhs with the trace update folded in, output dx_cell)
-    // MIXED (pass 1 only): the elimination of the scalar block in FP32 with one
-    // FP64 refinement step; assembly, residual, refinement residual and Schur
+    // MIXED: the elimination of the scalar block in FP32 with one FP64
+    // refinement step; assembly, residual, refinement residual and Schur
     // product in FP64
     template <int DEG, int NE, bool THERMO, bool PASS2, bool MIXED = false>
     __global__ void __launch_bounds__(64 * WPB)
@@ -1111,6 +1111,14 @@ namespace hdg
 
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s)
   {
+    if (p.mixed)
+      {
+        if (p.thermo)
+          return deg == 1 ? launch_fast<1, 1, true, true, true>(p, nullptr, ncells, 0, 2, s)
+                          : launch_fast<2, 1, true, true, true>(p, nullptr, ncells, 0, 2, s);
+        return deg == 1 ? launch_fast<1, 3, false, true, true>(p, nullptr, ncells, 0, 2, s)
+                        : launch_fast<2, 3, false, true, true>(p, nullptr, ncells, 0, 2, s);
+      }
     if (p.thermo)
       return deg == 1 ? launch_fast<1, 1, true, true>(p, nullptr, ncells, 0, 2, s)
                       : launch_fast<2, 1, true, true>(p, nullptr, ncells, 0, 2, s);

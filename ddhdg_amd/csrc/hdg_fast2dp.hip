@@ -22,8 +22,9 @@
 //      deterministic)
 // A workgroup holds two pairs (two cells) and both run every barrier; in
 // the last round a pair without a cell of its own repeats its partner's
-// with its stores switched off.  Pass 1 in FP64; pass 2 of these cells runs
-// on the generic kernel.
+// with its stores switched off.  Pass 2 runs the same cell body with the
+// trace update folded into the rhs column and writes dx_cell.  Both passes
+// in FP64 or with the elimination in FP32 and one FP64 refinement step.
 #include <type_traits>
 #include <utility>
 
@@ -101,7 +102,8 @@ namespace hdg
       static constexpr int iBID  = iNEU + NFC * 3;
       static constexpr int iNBK  = iBID + NFC;
       static constexpr int iSLOT = iNBK + NFC;
-      static constexpr int CINT  = iSLOT + NFC * NFC;
+      static constexpr int iPIV  = iSLOT + NFC * NFC; // NS pivot lanes (FP32 + refinement)
+      static constexpr int CINT  = iPIV + NS;
       static constexpr size_t bytes()
       {
         return (size_t)(TAB + PPB * CDBL) * sizeof(double) + (size_t)PPB * CINT * sizeof(int);
@@ -156,7 +158,10 @@ namespace hdg
     }
 
     // one cell (one pair of waves), pass 1 (mode 0: CSR scatter; mode 1:
-    // S, g of cell idx out); live = false: compute, store nothing
+    // S, g of cell idx out) or pass 2 (PASS2: dx_cell of the cell out);
+    // live = false: compute, store nothing.  MIXED: the elimination in FP32
+    // with one FP64 refinement step
+    template <bool PASS2, bool MIXED>
     __device__ __forceinline__ void cell_body_p(const KParams &p, double *smem, long long cell,
                                                 long long idx, int color, int mode, bool live)
     {
@@ -164,7 +169,7 @@ namespace hdg
       constexpr int NB     = S::NB, NF = S::NF, QC = S::QC, QF = S::QF, NFC = S::NFC;
       constexpr int NS     = S::NS, TDF = S::TDF, T = S::T, NE = S::NE;
       constexpr int K1     = S::K1, DEG = S::DEG;
-      constexpr bool THERMO = false, PASS2 = false;
+      constexpr bool THERMO = false;
       const int      tid    = threadIdx.x;
       // the wave's index is uniform: readfirstlane makes that visible, so
       // branches on half stay scalar
@@ -613,15 +618,27 @@ namespace hdg
       // max, and that lane writes its column to the ring; after a workgroup
       // barrier both waves read the column back (lane r holds entry r) and
       // apply the same update, each entry broadcast by readlane where it is
-      // used.
-      int myk = 0; // the equation this scalar lane was the pivot of
-      {
-        double cs = 0;
+      // used.  MIXED: that elimination in FP32 (the pivot lanes recorded),
+      // the FP64 residual r = b - A' x0 of every column (the scalar lanes
+      // keep their FP64 columns of A' in R and publish them through the ring
+      // in pivot order), the same FP32 elimination with the recorded pivots
+      // on r, and x = x0 + dx.
+      int  myk  = 0; // the equation this scalar lane was the pivot of
+      int *pivs = wi + S::iPIV;
+      // one elimination pass over Rr (double or float): SEARCH picks the
+      // pivots (and records them), otherwise they are read from pivs
+      auto elim = [&](auto &Rr, auto search) __attribute__((always_inline)) {
+        using Real            = std::remove_reference_t<decltype(Rr[0])>;
+        constexpr bool SEARCH = decltype(search)::value;
+        Real           cs     = 0;
+        bool           pivoted = !isU;
+        if constexpr (SEARCH)
+          {
 #pragma unroll
-        for (int r = 0; r < NS; ++r)
-          cs = fmax(cs, fabs(R[r]));
-        cs           = cs > 0.0 ? 1.0 / cs : 0.0;
-        bool pivoted = !isU;
+            for (int r = 0; r < NS; ++r)
+              cs = fmax(cs, fabs(Rr[r]));
+            cs = cs > Real(0) ? Real(1) / cs : Real(0);
+          }
         // the steps in groups of GS, expanded at compile time so that every
         // step index is a constant
         auto group = [&](auto gc) __attribute__((always_inline)) {
@@ -635,31 +652,137 @@ namespace hdg
                   double *col = ring + (k & 1) * NS;
                   if (half == 0)
                     {
-                      const unsigned key =
-                        wave_max(pivot_key_p(fabs(R[k]) * cs, !pivoted, lane));
-                      const int pl = 63 - (int)((key - 1u) & 63u);
-                      if (lane == pl)
+                      int pl;
+                      if constexpr (SEARCH)
                         {
-                          pivoted = true;
-                          myk     = k;
-#pragma unroll
-                          for (int r = 0; r < NS; ++r)
-                            col[r] = R[r];
+                          const unsigned key =
+                            wave_max(pivot_key_p(fabs(Rr[k]) * cs, !pivoted, lane));
+                          pl = 63 - (int)((key - 1u) & 63u);
+                          if (lane == pl)
+                            {
+                              pivoted = true;
+                              myk     = k;
+                            }
+                          if (lane == 0)
+                            pivs[k] = pl;
                         }
+                      else
+                        pl = __builtin_amdgcn_readfirstlane(pivs[k]);
+                      if (lane == pl)
+#pragma unroll
+                        for (int r = 0; r < NS; ++r)
+                          col[r] = (double)Rr[r];
                     }
                   __syncthreads();
-                  const double v = col[lane < NS ? lane : 0];
-                  const double f = R[k] / lane_value(v, k);
+                  const Real v = (Real)col[lane < NS ? lane : 0];
+                  const Real f = Rr[k] / lane_value(v, k);
 #pragma unroll
                   for (int r = 0; r < NS; ++r)
                     if (r != k)
-                      R[r] -= lane_value(v, r) * f;
-                  R[k] = f;
+                      Rr[r] -= lane_value(v, r) * f;
+                  Rr[k] = f;
                 }
             }
         };
         for_each_group(group, std::make_integer_sequence<int, S::NG>{});
-      }
+      };
+      if constexpr (!MIXED)
+        elim(R, std::true_type{});
+      else
+        {
+          float R32[NS], X0[NS];
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            R32[r] = (float)R[r];
+          elim(R32, std::true_type{});
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            X0[r] = R32[r];
+          // r = b - A' x0: step k takes the FP64 column of the pivot lane
+          // of equation k through the ring
+          const double mres = isU ? 0.0 : 1.0;
+          auto resid = [&](auto gc) __attribute__((always_inline)) {
+            constexpr int G = decltype(gc)::value;
+#pragma unroll
+            for (int j = 0; j < S::GS; ++j)
+              {
+                const int k = G * S::GS + j;
+                if (k < NS)
+                  {
+                    double *col = ring + (k & 1) * NS;
+                    if (half == 0)
+                      {
+                        const int pl = __builtin_amdgcn_readfirstlane(pivs[k]);
+                        if (lane == pl)
+#pragma unroll
+                          for (int r = 0; r < NS; ++r)
+                            col[r] = R[r];
+                      }
+                    __syncthreads();
+                    const double v  = col[lane < NS ? lane : 0];
+                    const double xk = mres * (double)X0[k];
+#pragma unroll
+                    for (int r = 0; r < NS; ++r)
+                      R[r] -= lane_value(v, r) * xk;
+                  }
+              }
+          };
+          for_each_group(resid, std::make_integer_sequence<int, S::NG>{});
+#pragma unroll
+          for (int r = 0; r < NS; ++r)
+            R32[r] = (float)R[r];
+          elim(R32, std::false_type{});
+#pragma unroll
+          for (int k = 0; k < NS; ++k)
+            R[k] = (double)X0[k] + (double)R32[k];
+        }
+
+      if constexpr (PASS2)
+        {
+          // the rhs lane holds u(p_k) in row k: u reaches the scalar lanes
+          // through the ring (lane p reads row myk), then W = M^-1 (rhs_q +
+          // D_a u - sum_f C_{a,f} lambda_upd,f) and u go out lane-parallel
+          // (reference template.h:2688-2697)
+          __syncthreads();
+          if (isR)
+#pragma unroll
+            for (int k = 0; k < NS; ++k)
+              ring[k] = R[k];
+          __syncthreads();
+          double *sol = ring + NS;
+          if (isU)
+            sol[plane] = ring[myk];
+          __syncthreads();
+          for (int e = plane; e < S::NCD; e += 128)
+            {
+              const int c = e / (3 * NB), r = e % (3 * NB), blkc = r / NB, i = r % NB;
+              double    v = 0;
+              const double *u = sol + c * NB;
+              if (blkc == 2)
+                v = u[i];
+              else
+                {
+                  const int a = blkc;
+#pragma unroll
+                  for (int l = 0; l < NB; ++l)
+                    v += tb.Minv[i * NB + l] * rhsq[(c * 2 + a) * NB + l] +
+                         tb.MinvD[(a * NB + i) * NB + l] * u[l];
+#pragma unroll
+                  for (int sd2 = 0; sd2 < 2; ++sd2)
+                    {
+                      const int f = 2 * a + sd2;
+#pragma unroll
+                      for (int m = 0; m < NF; ++m)
+                        v -= tb.MinvC[(f * NB + i) * NF + m] * lupd[(f * 3 + c) * NF + m];
+                    }
+                }
+              if (live)
+                p.dxc[cell * S::NCD + e] = v;
+            }
+          // the next cell rewrites the ring and the pair's scratch
+          __syncthreads();
+          return;
+        }
 
       // ---- 3: [S | g] = [TT' | t'] - TC' X, face by face.  The product
       // runs on the matrix cores (v_mfma_f64_16x16x4f64, K = 4 pivot rows per
@@ -783,6 +906,7 @@ namespace hdg
         }
     }
 
+    template <bool MIXED>
     __global__ void __launch_bounds__(64 * 2 * PPB) __attribute__((amdgpu_waves_per_eu(2)))
       fast_condense_p(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
     {
@@ -796,14 +920,32 @@ namespace hdg
         {
           const int  idx  = base + pr;
           const bool live = idx < ncell;
-          cell_body_p(p, smem, (long long)cells[live ? idx : base], live ? idx : base, color,
-                      mode, live);
+          cell_body_p<false, MIXED>(p, smem, (long long)cells[live ? idx : base],
+                                    live ? idx : base, color, mode, live);
+        }
+    }
+    // pass 2 over all cells in order (dx_cell out)
+    template <bool MIXED>
+    __global__ void __launch_bounds__(64 * 2 * PPB) __attribute__((amdgpu_waves_per_eu(2)))
+      reconstruct_p(KParams p, int ncell)
+    {
+      extern __shared__ double smem[];
+      stage_tables_p(p.tb, smem, threadIdx.x);
+      __syncthreads();
+      const int pr = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 7);
+      for (int base = blockIdx.x * PPB; base < ncell; base += gridDim.x * PPB)
+        {
+          const int  idx  = base + pr;
+          const bool live = idx < ncell;
+          const int  use  = live ? idx : base;
+          cell_body_p<true, MIXED>(p, smem, use, use, 0, 0, live);
         }
     }
     // pass 1 over the cells of several degree-3 meshes in one persistent
     // launch per colour: the work list holds chunks of PPB cells of one
     // device (padded with -1 at device boundaries; a chunk's first cell is
     // real); a workgroup restages the tables when its chunk's device changes
+    template <bool MIXED>
     __global__ void __launch_bounds__(64 * 2 * PPB) __attribute__((amdgpu_waves_per_eu(2)))
       batch_condense_p(const KParams *__restrict__ plist, const int *__restrict__ chunk_dev,
                        const int *__restrict__ items, int nchunks, int color)
@@ -825,8 +967,31 @@ namespace hdg
           const int  cell = items[c * PPB + pr];
           const bool live = cell >= 0;
           const int  use  = live ? cell : items[c * PPB];
-          cell_body_p(p, smem, use, use, color, 0, live);
+          cell_body_p<false, MIXED>(p, smem, use, use, color, 0, live);
         }
+    }
+
+    // the kernel's dynamic LDS attribute, once per instantiation
+    template <class K>
+    hipError_t lds_attr(K kernel, bool &done)
+    {
+      if (done)
+        return hipSuccess;
+      hipError_t e = hipFuncSetAttribute((const void *)kernel,
+                                         hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)FP::bytes());
+      if (e == hipSuccess)
+        done = true;
+      return e;
+    }
+
+    int persistent_grid(int ncell)
+    {
+      int dev = 0, ncu = 256;
+      if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+      const int want = (ncell + PPB - 1) / PPB;
+      return want < 2 * ncu ? want : 2 * ncu;
     }
   } // namespace
 
@@ -836,22 +1001,21 @@ namespace hdg
   }
 
   hipError_t launch_batch_pair(const KParams *plist, const int *chunk_dev, const int *items,
-                               int nchunks, int color, int grid, hipStream_t s)
+                               int nchunks, int color, int grid, int mixed, hipStream_t s)
   {
-    constexpr size_t bytes = FP::bytes();
-    static bool      attr  = false;
-    if (!attr)
-      {
-        hipError_t e = hipFuncSetAttribute((const void *)batch_condense_p,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess)
-          return e;
-        attr = true;
-      }
+    static bool a0 = false, a1 = false;
+    hipError_t  e = mixed ? lds_attr(batch_condense_p<true>, a1)
+                          : lds_attr(batch_condense_p<false>, a0);
+    if (e != hipSuccess)
+      return e;
     if (nchunks <= 0)
       return hipSuccess;
-    hipLaunchKernelGGL(batch_condense_p, dim3(grid), dim3(64 * 2 * PPB), bytes, s, plist,
-                       chunk_dev, items, nchunks, color);
+    if (mixed)
+      hipLaunchKernelGGL(batch_condense_p<true>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s,
+                         plist, chunk_dev, items, nchunks, color);
+    else
+      hipLaunchKernelGGL(batch_condense_p<false>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s,
+                         plist, chunk_dev, items, nchunks, color);
     return hipGetLastError();
   }
 
@@ -864,25 +1028,39 @@ namespace hdg
   hipError_t launch_fast2p_condense(const KParams &p, const int *cells, int ncell, int color,
                                     int mode, hipStream_t s)
   {
-    constexpr size_t bytes = FP::bytes();
-    static bool      attr  = false;
-    if (!attr)
-      {
-        hipError_t e = hipFuncSetAttribute((const void *)fast_condense_p,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-        if (e != hipSuccess)
-          return e;
-        attr = true;
-      }
+    static bool a0 = false, a1 = false;
+    hipError_t  e = p.mixed ? lds_attr(fast_condense_p<true>, a1)
+                            : lds_attr(fast_condense_p<false>, a0);
+    if (e != hipSuccess)
+      return e;
     if (ncell <= 0)
       return hipSuccess;
-    int dev = 0, ncu = 256;
-    if (hipGetDevice(&dev) == hipSuccess)
-      (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    const int want = (ncell + PPB - 1) / PPB;
-    const int grid = want < 2 * ncu ? want : 2 * ncu;
-    hipLaunchKernelGGL(fast_condense_p, dim3(grid), dim3(64 * 2 * PPB), bytes, s, p, cells, ncell,
-                       color, mode);
+    const int grid = persistent_grid(ncell);
+    if (p.mixed)
+      hipLaunchKernelGGL(fast_condense_p<true>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s,
+                         p, cells, ncell, color, mode);
+    else
+      hipLaunchKernelGGL(fast_condense_p<false>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s,
+                         p, cells, ncell, color, mode);
+    return hipGetLastError();
+  }
+
+  hipError_t launch_fast2p_reconstruct(const KParams &p, int ncells, hipStream_t s)
+  {
+    static bool a0 = false, a1 = false;
+    hipError_t  e = p.mixed ? lds_attr(reconstruct_p<true>, a1)
+                            : lds_attr(reconstruct_p<false>, a0);
+    if (e != hipSuccess)
+      return e;
+    if (ncells <= 0)
+      return hipSuccess;
+    const int grid = persistent_grid(ncells);
+    if (p.mixed)
+      hipLaunchKernelGGL(reconstruct_p<true>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s, p,
+                         ncells);
+    else
+      hipLaunchKernelGGL(reconstruct_p<false>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s, p,
+                         ncells);
     return hipGetLastError();
   }
 } // namespace hdg

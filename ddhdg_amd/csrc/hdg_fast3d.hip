@@ -709,7 +709,40 @@ namespace hdg
             {
               if constexpr (PASS2)
                 {
-                  gauss_jordan_rec<NS>(R, isU, lane, pivs, myk, gjr);
+                  if constexpr (!MIXED)
+                    gauss_jordan_rec<NS>(R, isU, lane, pivs, myk, gjr);
+                  else
+                    {
+                      // the FP32 elimination, the FP64 residual of the rhs
+                      // column (the scalar lanes keep their FP64 columns of
+                      // A' in R), the FP32 elimination again on it
+                      float R32[NS], X0[NS];
+#pragma unroll
+                      for (int r = 0; r < NS; ++r)
+                        R32[r] = (float)R[r];
+                      gauss_jordan_rec<NS>(R32, isU, lane, pivs, myk, gjr32);
+#pragma unroll
+                      for (int r = 0; r < NS; ++r)
+                        X0[r] = R32[r];
+                      WAVE_SYNC3();
+                      const double mres = isU ? 0.0 : 1.0;
+#pragma unroll
+                      for (int k = 0; k < NS; ++k)
+                        {
+                          const int    pl = __builtin_amdgcn_readfirstlane(pivs[k]);
+                          const double xk = mres * (double)X0[k];
+#pragma unroll
+                          for (int r = 0; r < NS; ++r)
+                            R[r] -= lane_value(R[r], pl) * xk;
+                        }
+#pragma unroll
+                      for (int r = 0; r < NS; ++r)
+                        R32[r] = (float)R[r];
+                      gauss_jordan_replay<NS>(R32, gjr32);
+#pragma unroll
+                      for (int k = 0; k < NS; ++k)
+                        R[k] = (double)X0[k] + (double)R32[k];
+                    }
                   WAVE_SYNC3();
                   // the rhs lane holds u(p_k) in row k: publish u, then
                   // W = M^-1 (rhs_q + D_a u - sum_f C_{a,f} lambda_upd,f)
@@ -1033,6 +1066,9 @@ namespace hdg
 
   hipError_t launch_fast3_reconstruct(const KParams &p, int ncells, hipStream_t s)
   {
+    if (p.mixed)
+      return p.thermo ? launch_fast3<1, true, true, true>(p, nullptr, ncells, 0, 0, s)
+                      : launch_fast3<3, false, true, true>(p, nullptr, ncells, 0, 0, s);
     if (p.thermo)
       return launch_fast3<1, true, false, true>(p, nullptr, ncells, 0, 0, s);
     return launch_fast3<3, false, false, true>(p, nullptr, ncells, 0, 0, s);

@@ -98,11 +98,14 @@ namespace hdg
   bool       fast2p_supported(int dim, int deg, int ne, int thermo, const int en[3]);
   hipError_t launch_fast2p_condense(const KParams &p, const int *cells, int ncell, int color,
                                     int mode, hipStream_t s);
-  // the same over several degree-3 meshes in one persistent launch per
-  // colour (work list in chunks of batch_pairs_per_group() cells), FP64
+  // its pass 2 (all cells in order, dx_cell out)
+  hipError_t launch_fast2p_reconstruct(const KParams &p, int ncells, hipStream_t s);
+  // the same pass 1 over several degree-3 meshes in one persistent launch
+  // per colour (work list in chunks of batch_pairs_per_group() cells), FP64
+  // or FP32 + refinement (mixed)
   int        batch_pairs_per_group();
   hipError_t launch_batch_pair(const KParams *plist, const int *chunk_dev, const int *items,
-                               int nchunks, int color, int grid, hipStream_t s);
+                               int nchunks, int color, int grid, int mixed, hipStream_t s);
   // register-resident pass 2 (same shapes as launch_fast_condense)
   hipError_t launch_fast_reconstruct(int deg, const KParams &p, int ncells, hipStream_t s);
   // the uncondensed local system of listed cells (hdg_local.hip); G: a

@@ -126,12 +126,12 @@ int hdg_set_enabled(hdg_ctx *ctx, int V, int n, int p);
  * reference (one CPU code path). */
 int hdg_set_kernel_variant(hdg_ctx *ctx, int variant);
 
-/* Precision of pass 1's local elimination: 0 = FP64 (default); 1 = the
- * scalar block eliminated in FP32 with one FP64 iterative-refinement step
- * (assembly, the residual, the refinement residual and the Schur product
- * in FP64).  Applies to the register-resident kernels of 2D degree 1-2 and
- * 3D degree 1; the other kernels and pass 2 always run in FP64.  No
- * counterpart in the reference (FP64 throughout). */
+/* Precision of the local elimination of both passes: 0 = FP64 (default);
+ * 1 = the scalar block eliminated in FP32 with one FP64 iterative-
+ * refinement step (assembly, the residual, the refinement residual and the
+ * Schur product in FP64).  Applies to the register-resident kernels (2D
+ * degree 1-3, 3D degree 1), pass 1 and pass 2; the generic kernel runs in
+ * FP64.  No counterpart in the reference (FP64 throughout). */
 int hdg_set_precision(hdg_ctx *ctx, int precision);
 
 /* Restricted trace matrix pattern, host arrays: rowptr (NTRACE+1),
@@ -248,8 +248,8 @@ int hdg_max_abs4_noalloc(const double *const *vectors, const long long *lengths,
  * Pass 1 (hdg_condense) of several contexts (2D with the same enabled
  * components: degree 1 and 2 mixed, or all of degree 3 with V, n and p) in
  * one persistent launch per colour; each context's outputs are exactly
- * those of its own hdg_condense.  A degree-3 batch runs in FP64 and the
- * full operator only.  The autotuner picks the tile (waves per workgroup
+ * those of its own hdg_condense, in the contexts' precision (they must
+ * share it).  A degree-3 batch runs the full operator only.  The autotuner picks the tile (waves per workgroup
  * of the degree 1-2 kernel: 4 or 8, one cell per wave) and the persistent
  * grid (workgroups per CU) by timing each candidate on the first call
  * (hdg_batch_info reports the choice, hdg_batch_tuning every candidate).

@@ -64,13 +64,15 @@ def test_batch_equals_per_mesh(thermodyn, precision):
         assert t.equal(v, v1) and t.equal(r, r1) and t.equal(d, d1)
 
 
-def test_degree3_batch_equals_per_mesh():
-    """degree-3 meshes in one persistent launch per colour of the pair kernel:
-    every mesh's outputs bitwise equal to its own hdg_condense"""
+@pytest.mark.parametrize("precision", [0, 1])
+def test_degree3_batch_equals_per_mesh(precision):
+    """degree-3 meshes in one persistent launch per colour of the pair kernel
+    (FP64 and FP32 + refinement): every mesh's outputs bitwise equal to its
+    own hdg_condense"""
     t = _torch()
-    print(f"case degree-3 batch of {len(MESHES3)} meshes", flush=True)
+    print(f"case degree-3 batch of {len(MESHES3)} meshes precision={precision}", flush=True)
     from ddhdg_amd.hip_lib import HdgBatch
-    sv = _solvers(False, 0, MESHES3)
+    sv = _solvers(False, precision, MESHES3)
     bufs = []
     for s in sv:
         c = s.ctx

@@ -20,7 +20,12 @@ def _torch():
     return torch
 
 
-@pytest.mark.parametrize("dim,n_cells,k", [(2, 4, 1), (2, 4, 2), (3, 3, 1)])
+# (2, 3, 3): the pair kernel (V/n/p only: its equilibrium runs the generic
+# kernel in FP64)
+MIXED_CASES = [(2, 4, 1), (2, 4, 2), (2, 3, 3), (3, 3, 1)]
+
+
+@pytest.mark.parametrize("dim,n_cells,k", MIXED_CASES)
 @pytest.mark.parametrize("thermodyn", [False, True])
 def test_mixed_cell_systems_error(dim, n_cells, k, thermodyn):
     t = _torch()
@@ -51,16 +56,47 @@ def test_mixed_cell_systems_error(dim, n_cells, k, thermodyn):
     assert eS < MIXED_BAR and eg < MIXED_BAR
 
 
-def test_mixed_newton_reaches_fp64_solution():
+@pytest.mark.parametrize("dim,n_cells,k", MIXED_CASES)
+@pytest.mark.parametrize("thermodyn", [False, True])
+def test_mixed_reconstruct_error(dim, n_cells, k, thermodyn):
+    """pass 2 with the FP32 elimination and one FP64 refinement step against
+    the FP64 oracle's pass 2 (error stated per case)"""
+    t = _torch()
+    print(f"case mixed reconstruct dim={dim} n={n_cells} k={k} thermo={thermodyn}", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(dim, n_cells, k)
+    s = NPSolver(prob, params, adim)
+    enabled = (True, False, False) if thermodyn else (True, True, True)
+    s.set_enabled_components(*enabled)
+    s.ctx.set_precision(1)
+    o = H.oracle_for(prob, params, adim, enabled)
+    cell, tr = H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=41 + k)
+    s.set_state(cell, tr)
+    o.set_state(cell, tr)
+    rng = np.random.default_rng(6)
+    upd = 1e-2 * rng.standard_normal(s.x_trace.numel())
+    if thermodyn:
+        upd.reshape(-1, 3, s.ctx.nf)[:, 1:] = 0
+    dxt = t.as_tensor(upd, device="cuda")
+    dxc = t.empty_like(s.x_cell)
+    s.ctx.reconstruct(s.x_cell, s.x_trace, dxt, thermodyn, dxc)
+    t.cuda.synchronize()
+    err = H.rel_fro(dxc.cpu().numpy(), o.pass2(upd, thermodyn))
+    print(f"  FP32+refine pass 2 vs FP64 oracle: rel err {err:.2e}")
+    assert err < MIXED_BAR
+
+
+@pytest.mark.parametrize("n_cells,k", [(8, 2), (6, 3)])
+def test_mixed_newton_reaches_fp64_solution(n_cells, k):
     """the diode workflow (equilibrium, n/p, coupled, 0.1 V, 0.3 V) with the mixed
-    elimination converges to the FP64 run's solution (relative L-infinity
-    1e-10)"""
+    elimination (both passes) converges to the FP64 run's solution (relative
+    L-infinity 1e-10)"""
     _torch()
-    print("case mixed Newton diode 2D n=8 k=2", flush=True)
+    print(f"case mixed Newton diode 2D n={n_cells} k={k}", flush=True)
     from ddhdg_amd.np_solver import NPSolver
     states, its = [], []
     for prec in (0, 1):
-        prob, params, adim = configs.diode(2, 8, 2)
+        prob, params, adim = configs.diode(2, n_cells, k)
         s = NPSolver(prob, params, adim)
         s.ctx.set_precision(prec)
         s.set_enabled_components(True, False, False)
