@@ -423,11 +423,31 @@ def sample_3d_k2(args, torch, stream):
     p2 = timed(lambda: ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc))
     fm = timed(lambda: ts.factor(vals))
     sm = timed(lambda: ts.solve(rhs, sol))
+    # the same passes with the FP32 elimination and one FP64 refinement
+    # step, and its error against FP64 on a sample of cells
+    sample = torch.arange(0, ctx.ncells, max(1, ctx.ncells // 256), dtype=torch.int32,
+                          device="cuda")
+    T = ctx.T
+    blocks = {}
+    for prec in (0, 1):
+        ctx.set_precision(prec)
+        S_ = torch.empty(sample.numel() * T * T, dtype=torch.float64, device="cuda")
+        g_ = torch.empty(sample.numel() * T, dtype=torch.float64, device="cuda")
+        ctx.cell_systems(s.x_cell, s.x_trace, False, sample, S_, g_)
+        blocks[prec] = (S_.view(-1, T * T), g_.view(-1, T))
+    p1m = timed(lambda: ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv))
+    p2m = timed(lambda: ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc))
+    ctx.set_precision(0)
+    (Sd, gd), (Sm, gm) = blocks[0], blocks[1]
+    err_S = float(((Sm - Sd).norm(dim=1) / Sd.norm(dim=1)).max())
     return {"workload": f"3D diode {n}^3 hexahedra, degree 2, V/n/p (324 cell dofs, 162 trace "
                         "dofs), step junction N_D = N_A = 1e22 m^-3; generic kernels, FP64",
             "cells": int(ctx.ncells),
             "pass1": {"value": ctx.ncells / (p1 * 1e-3), "unit": "cells/s", "ms_per_pass": p1},
             "pass2_ms": p2, "factor_ms": fm, "solve_ms": sm,
+            "fp32_refine": {"pass1_ms": p1m, "pass2_ms": p2m, "vs_fp64": p1 / p1m,
+                            "max_rel_err_S_vs_fp64": err_S,
+                            "error_sample_cells": int(sample.numel())},
             "factor_GFLOP": ts.flops / 1e9, "factor_TFLOPs": ts.flops / (fm * 1e-3) / 1e12,
             "max_front": int(ts.max_front)}
 

@@ -704,8 +704,8 @@ namespace hdg
     // row's multiplier of step k (rows not yet pivoted) stored in A[i][k]:
     // afterwards row piv[k] holds [L(k, 0..k-1) | U(k, k..)] in its own
     // place.  Pivot choice and update order are those of eliminate().
-    template <int DIM, int DEG>
-    __device__ __forceinline__ void lu_rows(double *A, int ld, double *sm, int *si, int nr, int nc)
+    template <int DIM, int DEG, typename Real = double>
+    __device__ __forceinline__ void lu_rows(Real *A, int ld, double *sm, int *si, int nr, int nc)
     {
       using S   = Sh<DIM, DEG>;
       static_assert(2 * S::MAXNS <= NT, "one update lane pair per row");
@@ -716,7 +716,7 @@ namespace hdg
         {
           double mx = 0;
           for (int j = 0; j < nr; ++j)
-            mx = fmax(mx, fabs(A[i * ld + j]));
+            mx = fmax(mx, fabs((double)A[i * ld + j]));
           rs[i]   = mx > 0 ? 1.0 / mx : 1.0;
           used[i] = 0;
         }
@@ -734,7 +734,7 @@ namespace hdg
               for (int i = tid; i < nr; i += 64)
                 if (!used[i])
                   {
-                    const double sc = fabs(A[i * ld + k]) * rs[i];
+                    const double sc = fabs((double)A[i * ld + k]) * rs[i];
                     if (sc > best || bi < 0) // a NaN column still picks a row
                       {
                         best = sc;
@@ -761,12 +761,12 @@ namespace hdg
           __syncthreads();
           // rows still to pivot (the pivot rows of this and earlier steps
           // keep used = 2)
-          const int    pr = piv[k];
-          const double pv = A[pr * ld + k];
+          const int  pr = piv[k];
+          const Real pv = A[pr * ld + k];
           if (ui < nr && used[ui] == 0)
             {
-              const double m = A[ui * ld + k] / pv;
-              if (m != 0.0)
+              const Real m = A[ui * ld + k] / pv;
+              if (m != Real(0))
                 for (int j = k + 1 + uh; j < nc; j += 2)
                   A[ui * ld + j] -= m * A[pr * ld + j];
               if (uh == 0)
@@ -775,6 +775,121 @@ namespace hdg
           __syncthreads();
         }
     }
+
+    // FP32 elimination with one FP64 refinement step, one column (one
+    // thread): x = A'^-1 b for b in col[i * LD], i < ns, overwritten by x.
+    // A32 holds lu_rows' FP32 factors (row piv[k]: [L(k, 0..k-1) | U(k,
+    // k..)], leading dimension al); Ad is A' in FP64 (leading dimension LD).
+    //   x0 = U^-1 L^-1 P b          FP32
+    //   r  = b - A' x0              FP64 (written over b)
+    //   x  = x0 + U^-1 L^-1 P r     the correction in FP32
+    // NSS >= ns bounds the register arrays (ns is uniform).
+    template <int NSS, int LD>
+    __device__ __forceinline__ void mixed_col_solve(double *col, const double *Ad,
+                                                    const float *A32, int al, const int *piv,
+                                                    int ns)
+    {
+      auto lu_solve = [&](float (&y)[NSS]) __attribute__((always_inline)) {
+        static_for(
+          [&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            if (k < ns)
+              {
+                const float *a   = A32 + piv[k] * al;
+                float        acc = (float)col[piv[k] * LD];
+#pragma unroll
+                for (int l = 0; l < k; ++l)
+                  acc -= a[l] * y[l];
+                y[k] = acc;
+              }
+          },
+          std::make_integer_sequence<int, NSS>{});
+        static_for(
+          [&](auto kc) {
+            constexpr int k = NSS - 1 - decltype(kc)::value;
+            if (k < ns)
+              {
+                const float *a   = A32 + piv[k] * al;
+                float        acc = y[k];
+#pragma unroll
+                for (int l = k + 1; l < NSS; ++l)
+                  if (l < ns)
+                    acc -= a[l] * y[l];
+                y[k] = acc / a[k];
+              }
+          },
+          std::make_integer_sequence<int, NSS>{});
+      };
+      // the two solves share one expansion of lu_solve (a loop that is not
+      // unrolled: the expanded solve is long)
+      float x0[NSS];
+#pragma unroll 1
+      for (int pass = 0; pass < 2; ++pass)
+        {
+          float y[NSS];
+          lu_solve(y);
+          if (pass == 0)
+            {
+#pragma unroll
+              for (int k = 0; k < NSS; ++k)
+                x0[k] = y[k];
+              for (int i = 0; i < ns; ++i)
+                {
+                  const double *a   = Ad + i * LD;
+                  double        acc = col[i * LD];
+#pragma unroll
+                  for (int k = 0; k < NSS; ++k)
+                    if (k < ns)
+                      acc -= a[k] * (double)x0[k];
+                  col[i * LD] = acc;
+                }
+            }
+          else
+#pragma unroll
+            for (int k = 0; k < NSS; ++k)
+              if (k < ns)
+                col[k * LD] = (double)x0[k] + (double)y[k];
+        }
+    }
+
+    // the same for the shape (one instantiation at its largest ns: a
+    // smaller ns skips the steps past it at uniform branches)
+    template <int DIM, int DEG>
+    __device__ __forceinline__ void mixed_col(double *col, const double *Ad, const float *A32,
+                                              int al, const int *piv, int ns)
+    {
+      using S = Sh<DIM, DEG>;
+      mixed_col_solve<S::MAXNS, S::LD>(col, Ad, A32, al, piv, ns);
+    }
+
+    // where the FP32 factors live: the G region of an LDS layout (dead after
+    // the reduced system is built), the scalar-block area of a global one
+    template <int DIM, int DEG>
+    __device__ __forceinline__ float *mixed_a32(double *rg, double *sm, int &al)
+    {
+      using S = Sh<DIM, DEG>;
+      if constexpr (S::BIG)
+        {
+          al = S::AL;
+          return reinterpret_cast<float *>(sm + S::oAW);
+        }
+      else
+        {
+          static_assert((S::MAXNS * (S::MAXNS + 1) + 1) / 2 <= S::SZ * S::LD,
+                        "the FP32 factors fit the G region");
+          al = S::MAXNS + 1;
+          return reinterpret_cast<float *>(rg + S::oG);
+        }
+    }
+
+    // pass 1 with the FP32 elimination and FP64 refinement:
+    //   X = A'^-1 [CT' | c']   one thread per column (mixed_col)
+    //   [S | g] = [TT' | t'] - TC' X   in FP64, written over the trace rows
+    template <int DIM, int DEG>
+    __device__ void mixed_schur(double *rg, double *sm, int *si, int ns, int T);
+    template <int DIM, int DEG>
+    __device__ __forceinline__ void schur_product(double *R, double *A, const int *rowof, int ns,
+                                                  int T);
 
     // The two triangular solves of schur_big for NSS scalar rows (rows of A
     // in place, leading dimensions LD for R and AL for A):
@@ -867,6 +982,24 @@ namespace hdg
       else
         tri_solves<S::NB, LD, AL>(R, A, piv, tid, T);
       __syncthreads();
+      schur_product<DIM, DEG>(R, A, piv, ns, T);
+    }
+
+    // [S | g] -= M Y over the trace rows of R (global or LDS), in passes of
+    // 16 RA rows with KC-deep chunks of M and Y staged in LDS at A: M in
+    // R's trace rows, columns 0..ns (its column k pairs with Y's row k);
+    // Y's row k at R row rowof[k] (rowof null: row k), columns ns..ns+T
+    template <int DIM, int DEG>
+    __device__ __forceinline__ void schur_product(double *R, double *A, const int *rowof, int ns,
+                                                  int T)
+    {
+      using S             = Sh<DIM, DEG>;
+      constexpr int LD    = S::LD;
+      constexpr int KC    = 9;                      // k per staged chunk
+      constexpr int RA    = 6;                      // row slots per pass
+      constexpr int CB    = (S::MAXT + 1 + 15) / 16; // column slots
+      const int     tid   = threadIdx.x;
+      const int     W     = T + 1;
       constexpr int WP = 16 * CB;       // staged Y width (zero beyond W)
       double   *Ms = A;                // [16 RA][KC], rows of this pass
       double   *Ys = A + 16 * RA * KC; // [KC][WP]
@@ -893,7 +1026,8 @@ namespace hdg
               for (int e = tid; e < KC * WP; e += NT)
                 {
                   const int kk = e / WP, c = e % WP;
-                  Ys[e]        = c < W ? R[piv[k0 + kk] * LD + ns + c] : 0.0;
+                  const int yr = rowof ? rowof[k0 + kk] : k0 + kk;
+                  Ys[e]        = c < W ? R[yr * LD + ns + c] : 0.0;
                 }
               __syncthreads();
 #pragma unroll
@@ -927,6 +1061,49 @@ namespace hdg
     }
 
     template <int DIM, int DEG>
+    __device__ void mixed_schur(double *rg, double *sm, int *si, int ns, int T)
+    {
+      using S          = Sh<DIM, DEG>;
+      constexpr int LD = S::LD;
+      double       *R   = rg + S::oR;
+      const int    *piv = si + S::iPIV;
+      const int     tid = threadIdx.x;
+      int           al;
+      float        *A32 = mixed_a32<DIM, DEG>(rg, sm, al);
+      for (int e = tid; e < ns * ns; e += NT)
+        {
+          const int i = e / ns, j = e % ns;
+          A32[i * al + j] = (float)R[i * LD + j];
+        }
+      __syncthreads();
+      lu_rows<DIM, DEG, float>(A32, al, sm, si, ns, ns);
+      if (tid < T + 1)
+        mixed_col<DIM, DEG>(R + ns + tid, R, A32, al, piv, ns);
+      __syncthreads();
+      if constexpr (S::BIG)
+        schur_product<DIM, DEG>(R, sm + S::oAW, nullptr, ns, T);
+      else
+        {
+          // every (trace row, column) entry: its dot product over the X
+          // rows (rows below ns and columns from ns are written only here)
+          const int W = T + 1;
+          for (int e = tid; e < T * W; e += NT)
+            {
+              const int     t   = e / W, c = e % W;
+              const double *m   = R + (ns + t) * LD;
+              double        acc = m[ns + c];
+              for (int k = 0; k < ns; ++k)
+                acc -= m[k] * R[k * LD + ns + c];
+              R[(ns + t) * LD + ns + c] = acc;
+            }
+          __syncthreads();
+        }
+    }
+
+    // MIXED: the elimination in FP32 with one FP64 refinement step
+    // (mixed_schur); a kernel of its own, so that the FP64 kernel's
+    // registers are its own
+    template <int DIM, int DEG, bool MIXED>
     __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2)))
       condense_kernel(KParams p, const int *cells, int ncell, int color, int mode)
     {
@@ -970,7 +1147,9 @@ namespace hdg
           R[r * LD + ncol - 1] -= acc;
         }
       __syncthreads();
-      if constexpr (S::BIG)
+      if constexpr (MIXED)
+        mixed_schur<DIM, DEG>(rg, sm, si, ns, T);
+      else if constexpr (S::BIG)
         schur_big<DIM, DEG>(R, sm, si, ns, T);
       else
         eliminate<DIM, DEG>(rg, sm, si, ns, nrow, ncol);
@@ -1024,7 +1203,7 @@ namespace hdg
       }
     }
 
-    template <int DIM, int DEG>
+    template <int DIM, int DEG, bool MIXED>
     __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) reconstruct_kernel(KParams p, int ncells)
     {
       using S = Sh<DIM, DEG>;
@@ -1041,35 +1220,57 @@ namespace hdg
       double       *lupd = sm + S::oLUPD;
       int          *piv  = si + S::iPIV;
       const int     tid  = threadIdx.x;
-      // U and the reduced rhs: R itself, or for R in global memory a copy
-      // of the ns x (ns + 1) system factorised in LDS
-      const double *U  = R;
-      int           lu = LD;
-      if constexpr (S::BIG)
+      if constexpr (MIXED)
         {
-          double *A = sm + S::oAW;
-          for (int e = tid; e < ns * (ns + 1); e += NT)
+          // u = A'^-1 c' in FP32 with one FP64 refinement step
+          int    al;
+          float *A32 = mixed_a32<DIM, DEG>(rg, sm, al);
+          for (int e = tid; e < ns * ns; e += NT)
             {
-              const int i = e / (ns + 1), j = e % (ns + 1);
-              A[i * S::AL + j] = R[i * LD + j];
+              const int i = e / ns, j = e % ns;
+              A32[i * al + j] = (float)R[i * LD + j];
             }
           __syncthreads();
-          lu_rows<DIM, DEG>(A, S::AL, sm, si, ns, ns + 1);
-          U  = A;
-          lu = S::AL;
+          lu_rows<DIM, DEG, float>(A32, al, sm, si, ns, ns);
+          if (tid == 0)
+            mixed_col<DIM, DEG>(R + ns, R, A32, al, piv, ns);
+          __syncthreads();
+          for (int k = tid; k < ns; k += NT)
+            sol[k] = R[k * LD + ns];
+          __syncthreads();
         }
       else
-        eliminate<DIM, DEG>(rg, sm, si, ns, ns, ns + 1);
-      if (tid == 0)
-        for (int k = ns - 1; k >= 0; --k)
-          {
-            const int pr = piv[k];
-            double    v  = U[pr * lu + ns];
-            for (int j = k + 1; j < ns; ++j)
-              v -= U[pr * lu + j] * sol[j];
-            sol[k] = v / U[pr * lu + k];
-          }
-      __syncthreads();
+        {
+          // U and the reduced rhs: R itself, or for R in global memory a
+          // copy of the ns x (ns + 1) system factorised in LDS
+          const double *U  = R;
+          int           lu = LD;
+          if constexpr (S::BIG)
+            {
+              double *A = sm + S::oAW;
+              for (int e = tid; e < ns * (ns + 1); e += NT)
+                {
+                  const int i = e / (ns + 1), j = e % (ns + 1);
+                  A[i * S::AL + j] = R[i * LD + j];
+                }
+              __syncthreads();
+              lu_rows<DIM, DEG>(A, S::AL, sm, si, ns, ns + 1);
+              U  = A;
+              lu = S::AL;
+            }
+          else
+            eliminate<DIM, DEG>(rg, sm, si, ns, ns, ns + 1);
+          if (tid == 0)
+            for (int k = ns - 1; k >= 0; --k)
+              {
+                const int pr = piv[k];
+                double    v  = U[pr * lu + ns];
+                for (int j = k + 1; j < ns; ++j)
+                  v -= U[pr * lu + j] * sol[j];
+                sol[k] = v / U[pr * lu + k];
+              }
+          __syncthreads();
+        }
       // W = M^-1 (rhs_q + D_a u - sum_f C_{a,f} lambda_f), written with u
       double *out = p.dxc + cell * S::NCD;
       for (int e = tid; e < S::NCD; e += NT)
@@ -1105,48 +1306,68 @@ namespace hdg
       }
     }
 
+    // the dynamic-LDS attribute of the four kernels of a shape, once
+    template <int DIM, int DEG>
+    hipError_t generic_attrs()
+    {
+      using S          = Sh<DIM, DEG>;
+      static bool attr = false;
+      if (attr)
+        return hipSuccess;
+      for (const void *k : {(const void *)condense_kernel<DIM, DEG, false>,
+                            (const void *)condense_kernel<DIM, DEG, true>,
+                            (const void *)reconstruct_kernel<DIM, DEG, false>,
+                            (const void *)reconstruct_kernel<DIM, DEG, true>})
+        {
+          hipError_t e =
+            hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)S::bytes);
+          if (e != hipSuccess)
+            return e;
+        }
+      attr = true;
+      return hipSuccess;
+    }
+
     template <int DIM, int DEG>
     hipError_t launch_c(const KParams &p, const int *cells, int ncell, int color, int mode,
                         hipStream_t s)
     {
-      using S = Sh<DIM, DEG>;
-      static bool attr = false;
-      if (!attr)
-        {
-          hipError_t e = hipFuncSetAttribute((const void *)condense_kernel<DIM, DEG>,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             (int)S::bytes);
-          if (e != hipSuccess)
-            return e;
-          e = hipFuncSetAttribute((const void *)reconstruct_kernel<DIM, DEG>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)S::bytes);
-          if (e != hipSuccess)
-            return e;
-          attr = true;
-        }
+      using S      = Sh<DIM, DEG>;
+      hipError_t e = generic_attrs<DIM, DEG>();
+      if (e != hipSuccess)
+        return e;
       if (ncell <= 0)
         return hipSuccess;
       if (S::BIG && !p.rgscr)
         return hipErrorInvalidValue;
       const int grid = S::BIG ? std::min(ncell, GRID_BIG) : ncell;
-      hipLaunchKernelGGL((condense_kernel<DIM, DEG>), dim3(grid), dim3(NT), S::bytes, s, p,
-                         cells, ncell, color, mode);
+      if (p.mixed)
+        hipLaunchKernelGGL((condense_kernel<DIM, DEG, true>), dim3(grid), dim3(NT), S::bytes, s,
+                           p, cells, ncell, color, mode);
+      else
+        hipLaunchKernelGGL((condense_kernel<DIM, DEG, false>), dim3(grid), dim3(NT), S::bytes, s,
+                           p, cells, ncell, color, mode);
       return hipGetLastError();
     }
 
     template <int DIM, int DEG>
     hipError_t launch_r(const KParams &p, int ncells, hipStream_t s)
     {
-      using S = Sh<DIM, DEG>;
-      hipError_t e = launch_c<DIM, DEG>(p, nullptr, 0, 0, 0, s); // sets attributes
+      using S      = Sh<DIM, DEG>;
+      hipError_t e = generic_attrs<DIM, DEG>();
       if (e != hipSuccess)
         return e;
+      if (ncells <= 0)
+        return hipSuccess;
       if (S::BIG && !p.rgscr)
         return hipErrorInvalidValue;
       const int grid = S::BIG ? std::min(ncells, GRID_BIG) : ncells;
-      hipLaunchKernelGGL((reconstruct_kernel<DIM, DEG>), dim3(grid), dim3(NT), S::bytes, s,
-                         p, ncells);
+      if (p.mixed)
+        hipLaunchKernelGGL((reconstruct_kernel<DIM, DEG, true>), dim3(grid), dim3(NT), S::bytes,
+                           s, p, ncells);
+      else
+        hipLaunchKernelGGL((reconstruct_kernel<DIM, DEG, false>), dim3(grid), dim3(NT),
+                           S::bytes, s, p, ncells);
       return hipGetLastError();
     }
   } // namespace

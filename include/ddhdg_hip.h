@@ -129,9 +129,8 @@ int hdg_set_kernel_variant(hdg_ctx *ctx, int variant);
 /* Precision of the local elimination of both passes: 0 = FP64 (default);
  * 1 = the scalar block eliminated in FP32 with one FP64 iterative-
  * refinement step (assembly, the residual, the refinement residual and the
- * Schur product in FP64).  Applies to the register-resident kernels (2D
- * degree 1-3, 3D degree 1), pass 1 and pass 2; the generic kernel runs in
- * FP64.  No counterpart in the reference (FP64 throughout). */
+ * Schur product in FP64), in every kernel (register-resident and generic),
+ * pass 1 and pass 2.  No counterpart in the reference (FP64 throughout). */
 int hdg_set_precision(hdg_ctx *ctx, int precision);
 
 /* Restricted trace matrix pattern, host arrays: rowptr (NTRACE+1),

@@ -109,3 +109,71 @@ def test_mixed_newton_reaches_fp64_solution(n_cells, k):
     print(f"  iterations FP64 {its[0]} FP32+refine {its[1]}")
     for a, b in zip(states[0], states[1]):
         assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
+
+
+# the generic kernel (forced; it also runs 3D degree 2 and the non-default
+# operator variants): (3, 2, 2) keeps its reduced systems in global scratch
+GENERIC_CASES = [(2, 4, 1), (2, 4, 2), (2, 3, 3), (3, 2, 1), (3, 2, 2)]
+
+
+@pytest.mark.parametrize("dim,n_cells,k", GENERIC_CASES)
+@pytest.mark.parametrize("thermodyn", [False, True])
+def test_mixed_generic_kernel_error(dim, n_cells, k, thermodyn):
+    """the generic kernel with the FP32 elimination and one FP64 refinement
+    step: per-cell S, g and pass 2 against the FP64 oracle"""
+    t = _torch()
+    print(f"case mixed generic dim={dim} n={n_cells} k={k} thermo={thermodyn}", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(dim, n_cells, k)
+    s = NPSolver(prob, params, adim)
+    enabled = (True, False, False) if thermodyn else (True, True, True)
+    s.set_enabled_components(*enabled)
+    s.ctx.set_kernel_variant(1)
+    s.ctx.set_precision(1)
+    o = H.oracle_for(prob, params, adim, enabled)
+    cell, tr = H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=51 + k)
+    s.set_state(cell, tr)
+    o.set_state(cell, tr)
+    ctx = s.ctx
+    cells = t.arange(ctx.ncells, dtype=t.int32, device="cuda")
+    S = t.empty(ctx.ncells * ctx.T * ctx.T, dtype=t.float64, device="cuda")
+    g = t.empty(ctx.ncells * ctx.T, dtype=t.float64, device="cuda")
+    ctx.cell_systems(s.x_cell, s.x_trace, thermodyn, cells, S, g)
+    rng = np.random.default_rng(7)
+    upd = 1e-2 * rng.standard_normal(s.x_trace.numel())
+    if thermodyn:
+        upd.reshape(-1, 3, s.ctx.nf)[:, 1:] = 0
+    dxt = t.as_tensor(upd, device="cuda")
+    dxc = t.empty_like(s.x_cell)
+    ctx.reconstruct(s.x_cell, s.x_trace, dxt, thermodyn, dxc)
+    t.cuda.synchronize()
+    S = S.cpu().numpy().reshape(ctx.ncells, ctx.T, ctx.T)
+    g = g.cpu().numpy().reshape(ctx.ncells, ctx.T)
+    eS = eg = 0.0
+    for c in range(ctx.ncells):
+        So, go, *_ = o.condensed_cell(c, thermodyn)
+        eS, eg = max(eS, H.rel_fro(S[c], So)), max(eg, H.rel_fro(g[c], go))
+    e2 = H.rel_fro(dxc.cpu().numpy(), o.pass2(upd, thermodyn))
+    print(f"  FP32+refine vs FP64 oracle: worst rel S {eS:.2e} g {eg:.2e}, pass 2 {e2:.2e}")
+    assert eS < MIXED_BAR and eg < MIXED_BAR and e2 < MIXED_BAR
+
+
+def test_mixed_generic_variant_matches_fp64():
+    """a non-default operator variant (use_trace flux with per-cell-face
+    tau, only on the generic kernel) with the FP32 elimination: the global
+    system against the same kernel in FP64"""
+    t = _torch()
+    print("case mixed generic variant use_trace + cellface tau 2D k=2", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(2, 5, 2, dd_flux_type="use_trace")
+    out = []
+    for prec in (0, 1):
+        s = NPSolver(prob, params, adim)
+        s.ctx.set_precision(prec)
+        s.set_state(*H.perturbed(*H.equilibrium_guess(prob, adim, 2), seed=61))
+        A, b, _, _ = s.assemble_system()
+        out.append((A.toarray(), b))
+    eA = H.rel_fro(out[1][0], out[0][0])
+    eb = H.rel_fro(out[1][1], out[0][1])
+    print(f"  FP32+refine vs FP64 global system: rel {eA:.2e}, rhs {eb:.2e}")
+    assert eA < MIXED_BAR and eb < MIXED_BAR
