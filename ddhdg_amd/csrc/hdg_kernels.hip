@@ -776,18 +776,27 @@ namespace hdg
         }
     }
 
-    // FP32 elimination with one FP64 refinement step, one column (one
+    // FP32 elimination with FP64 iterative refinement, one column (one
     // thread): x = A'^-1 b for b in col[i * LD], i < ns, overwritten by x.
-    // A32 holds lu_rows' FP32 factors (row piv[k]: [L(k, 0..k-1) | U(k,
-    // k..)], leading dimension al); Ad is A' in FP64 (leading dimension LD).
-    //   x0 = U^-1 L^-1 P b          FP32
-    //   r  = b - A' x0              FP64 (written over b)
-    //   x  = x0 + U^-1 L^-1 P r     the correction in FP32
+    // A32 holds lu_rows' FP32 factors of the equilibrated block Dr A' Dc
+    // (row piv[k]: [L(k, 0..k-1) | U(k, k..)], leading dimension al); rsc,
+    // csc the diagonals of Dr, Dc (powers of two); Ad is A' in FP64 (leading
+    // dimension LD), anorm its infinity norm.  As LAPACK's dsgesv:
+    //   x = 0, r = b;  repeat:  d = Dc U^-1 L^-1 P Dr r  (FP32 factors),
+    //   x += d,  r -= A' d  (FP64),  until ||r|| <= ||x|| ||A'|| eps sqrt(ns)
+    //   and ||d|| <= 2^-44 ||x||
+    // (at most MIXED_IT solves; three where the block is well conditioned).
+    // The second condition bounds the forward error too: the local blocks of
+    // the carriers are ill conditioned where the doping is high, and a
+    // solution with an FP64-level residual alone could still differ from
+    // the FP64 one by about 1e-11 relative, enough to add a Newton step.
     // NSS >= ns bounds the register arrays (ns is uniform).
+    constexpr int MIXED_IT = 8;
     template <int NSS, int LD>
     __device__ __forceinline__ void mixed_col_solve(double *col, const double *Ad,
                                                     const float *A32, int al, const int *piv,
-                                                    int ns)
+                                                    const double *rsc, const double *csc,
+                                                    double anorm, int ns)
     {
       auto lu_solve = [&](float (&y)[NSS]) __attribute__((always_inline)) {
         static_for(
@@ -796,7 +805,7 @@ namespace hdg
             if (k < ns)
               {
                 const float *a   = A32 + piv[k] * al;
-                float        acc = (float)col[piv[k] * LD];
+                float        acc = (float)(col[piv[k] * LD] * rsc[piv[k]]);
 #pragma unroll
                 for (int l = 0; l < k; ++l)
                   acc -= a[l] * y[l];
@@ -820,46 +829,102 @@ namespace hdg
           },
           std::make_integer_sequence<int, NSS>{});
       };
-      // the two solves share one expansion of lu_solve (a loop that is not
-      // unrolled: the expanded solve is long)
-      float x0[NSS];
+      const double tol = anorm * 0x1p-52 * sqrt((double)ns);
+      double       x[NSS];
+#pragma unroll
+      for (int k = 0; k < NSS; ++k)
+        x[k] = 0.0;
+      // one expansion of lu_solve (the loop is not unrolled)
 #pragma unroll 1
-      for (int pass = 0; pass < 2; ++pass)
+      for (int it = 0; it < MIXED_IT; ++it)
         {
           float y[NSS];
           lu_solve(y);
-          if (pass == 0)
+          double xmax = 0.0, dmax = 0.0;
+#pragma unroll
+          for (int k = 0; k < NSS; ++k)
+            if (k < ns)
+              {
+                const double d = (double)y[k] * csc[k];
+                x[k] += d;
+                xmax = fmax(xmax, fabs(x[k]));
+                dmax = fmax(dmax, fabs(d));
+              }
+          double rmax = 0.0;
+          for (int i = 0; i < ns; ++i)
             {
+              const double *a   = Ad + i * LD;
+              double        acc = col[i * LD];
 #pragma unroll
               for (int k = 0; k < NSS; ++k)
-                x0[k] = y[k];
-              for (int i = 0; i < ns; ++i)
-                {
-                  const double *a   = Ad + i * LD;
-                  double        acc = col[i * LD];
-#pragma unroll
-                  for (int k = 0; k < NSS; ++k)
-                    if (k < ns)
-                      acc -= a[k] * (double)x0[k];
-                  col[i * LD] = acc;
-                }
+                if (k < ns)
+                  acc -= a[k] * ((double)y[k] * csc[k]);
+              col[i * LD] = acc;
+              rmax        = fmax(rmax, fabs(acc));
             }
-          else
-#pragma unroll
-            for (int k = 0; k < NSS; ++k)
-              if (k < ns)
-                col[k * LD] = (double)x0[k] + (double)y[k];
+          if (!(rmax > xmax * tol) && !(dmax > xmax * 0x1p-44))
+            break;
         }
+#pragma unroll
+      for (int k = 0; k < NSS; ++k)
+        if (k < ns)
+          col[k * LD] = x[k];
     }
 
-    // the same for the shape (one instantiation at its largest ns: a
-    // smaller ns skips the steps past it at uniform branches)
+    // rsum: the row sums of |A'| (mixed_factor), for its infinity norm
     template <int DIM, int DEG>
     __device__ __forceinline__ void mixed_col(double *col, const double *Ad, const float *A32,
-                                              int al, const int *piv, int ns)
+                                              int al, const int *piv, const double *rsc,
+                                              const double *csc, const double *rsum, int ns)
     {
-      using S = Sh<DIM, DEG>;
-      mixed_col_solve<S::MAXNS, S::LD>(col, Ad, A32, al, piv, ns);
+      using S      = Sh<DIM, DEG>;
+      double anorm = 0.0;
+      for (int i = 0; i < ns; ++i)
+        anorm = fmax(anorm, rsum[i]);
+      mixed_col_solve<S::MAXNS, S::LD>(col, Ad, A32, al, piv, rsc, csc, anorm, ns);
+    }
+
+    // the FP32 copy of the equilibrated scalar block: Dr A' Dc with Dr, Dc
+    // powers of two (exact) that bring every row's, then every column's,
+    // largest entry into [1, 2): the local blocks of the carriers mix
+    // densities many orders of magnitude apart, beyond what one refinement
+    // step of a plain FP32 factorisation recovers.  Then lu_rows in FP32.
+    // rsc, csc, rsum (the row sums of |A'|): ns doubles each (LDS)
+    template <int DIM, int DEG>
+    __device__ __forceinline__ void mixed_factor(const double *R, float *A32, int al, double *sm,
+                                                 int *si, double *rsc, double *csc,
+                                                 double *rsum, int ns)
+    {
+      using S          = Sh<DIM, DEG>;
+      constexpr int LD = S::LD;
+      const int     tid = threadIdx.x;
+      for (int i = tid; i < ns; i += NT)
+        {
+          double m = 0, sum = 0;
+          for (int j = 0; j < ns; ++j)
+            {
+              m = fmax(m, fabs(R[i * LD + j]));
+              sum += fabs(R[i * LD + j]);
+            }
+          rsc[i]  = m > 0 ? ldexp(1.0, -ilogb(m)) : 1.0;
+          rsum[i] = sum;
+        }
+      __syncthreads();
+      for (int j = tid; j < ns; j += NT)
+        {
+          double m = 0;
+          for (int i = 0; i < ns; ++i)
+            m = fmax(m, fabs(R[i * LD + j] * rsc[i]));
+          csc[j] = m > 0 ? ldexp(1.0, -ilogb(m)) : 1.0;
+        }
+      __syncthreads();
+      for (int e = tid; e < ns * ns; e += NT)
+        {
+          const int i = e / ns, j = e % ns;
+          A32[i * al + j] = (float)(R[i * LD + j] * rsc[i] * csc[j]);
+        }
+      __syncthreads();
+      lu_rows<DIM, DEG, float>(A32, al, sm, si, ns, ns);
     }
 
     // where the FP32 factors live: the G region of an LDS layout (dead after
@@ -1070,15 +1135,11 @@ namespace hdg
       const int     tid = threadIdx.x;
       int           al;
       float        *A32 = mixed_a32<DIM, DEG>(rg, sm, al);
-      for (int e = tid; e < ns * ns; e += NT)
-        {
-          const int i = e / ns, j = e % ns;
-          A32[i * al + j] = (float)R[i * LD + j];
-        }
-      __syncthreads();
-      lu_rows<DIM, DEG, float>(A32, al, sm, si, ns, ns);
+      double       *rsc = sm + S::oMUL, *csc = rsc + S::MAXNS, *rsum = sm + S::oSOL;
+      static_assert(2 * S::MAXNS <= S::MAXR, "both scales fit the multiplier area");
+      mixed_factor<DIM, DEG>(R, A32, al, sm, si, rsc, csc, rsum, ns);
       if (tid < T + 1)
-        mixed_col<DIM, DEG>(R + ns + tid, R, A32, al, piv, ns);
+        mixed_col<DIM, DEG>(R + ns + tid, R, A32, al, piv, rsc, csc, rsum, ns);
       __syncthreads();
       if constexpr (S::BIG)
         schur_product<DIM, DEG>(R, sm + S::oAW, nullptr, ns, T);
@@ -1223,17 +1284,13 @@ namespace hdg
       if constexpr (MIXED)
         {
           // u = A'^-1 c' in FP32 with one FP64 refinement step
-          int    al;
-          float *A32 = mixed_a32<DIM, DEG>(rg, sm, al);
-          for (int e = tid; e < ns * ns; e += NT)
-            {
-              const int i = e / ns, j = e % ns;
-              A32[i * al + j] = (float)R[i * LD + j];
-            }
-          __syncthreads();
-          lu_rows<DIM, DEG, float>(A32, al, sm, si, ns, ns);
+          int     al;
+          float  *A32 = mixed_a32<DIM, DEG>(rg, sm, al);
+          // the row sums go to sol, which receives u only afterwards
+          double *rsc = sm + S::oMUL, *csc = rsc + S::MAXNS, *rsum = sol;
+          mixed_factor<DIM, DEG>(R, A32, al, sm, si, rsc, csc, rsum, ns);
           if (tid == 0)
-            mixed_col<DIM, DEG>(R + ns, R, A32, al, piv, ns);
+            mixed_col<DIM, DEG>(R + ns, R, A32, al, piv, rsc, csc, rsum, ns);
           __syncthreads();
           for (int k = tid; k < ns; k += NT)
             sol[k] = R[k * LD + ns];
