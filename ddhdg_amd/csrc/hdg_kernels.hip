@@ -781,15 +781,17 @@ namespace hdg
     // A32 holds lu_rows' FP32 factors of the equilibrated block Dr A' Dc
     // (row piv[k]: [L(k, 0..k-1) | U(k, k..)], leading dimension al); rsc,
     // csc the diagonals of Dr, Dc (powers of two); Ad is A' in FP64 (leading
-    // dimension LD), anorm its infinity norm.  As LAPACK's dsgesv:
+    // dimension LD), anorm its infinity norm.  Refinement as in LAPACK's
+    // dsgesv:
     //   x = 0, r = b;  repeat:  d = Dc U^-1 L^-1 P Dr r  (FP32 factors),
     //   x += d,  r -= A' d  (FP64),  until ||r|| <= ||x|| ||A'|| eps sqrt(ns)
     //   and ||d|| <= 2^-44 ||x||
     // (at most MIXED_IT solves; three where the block is well conditioned).
     // The second condition bounds the forward error too: the local blocks of
     // the carriers are ill conditioned where the doping is high, and a
-    // solution with an FP64-level residual alone could still differ from
-    // the FP64 one by about 1e-11 relative, enough to add a Newton step.
+    // solution with an FP64-level residual alone (or one predicted from the
+    // contraction of the corrections) differs from the FP64 one by enough
+    // to add Newton steps (measured on the 12^3 Gaussian cube).
     // NSS >= ns bounds the register arrays (ns is uniform).
     constexpr int MIXED_IT = 8;
     template <int NSS, int LD>
