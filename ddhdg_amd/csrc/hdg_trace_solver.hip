@@ -24,7 +24,13 @@
 #include <unordered_map>
 #include <vector>
 
+// the solution-index GEMM interface (rocblas_gemm_strided_batched_ex_get_solutions)
+#define ROCBLAS_BETA_FEATURES_API
 #include <rocblas/rocblas.h>
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "hdg_ctx.h"
 #include "hdg_nd_plan.h"
@@ -833,6 +839,118 @@ namespace
   // many entries (64 MB)
   constexpr long long SP_MIN_ENTRIES = 8ll << 20;
 
+  // C -= A B over the batch (leading dimension ld, batch stride st): with
+  // the rocBLAS solution sol from the tuning below, or rocBLAS's own choice
+  // (sol 0, or a solution that refuses the shape)
+  rocblas_status strided_gemm(rocblas_handle h, int sol, int M, int N, int K, const double *A,
+                              const double *B, double *C, int ld, long long st, int batch)
+  {
+    const double one = 1.0, mone = -1.0;
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+    if (sol != 0)
+      {
+        const rocblas_status r = rocblas_gemm_strided_batched_ex(
+          h, rocblas_operation_none, rocblas_operation_none, M, N, K, &mone, A,
+          rocblas_datatype_f64_r, ld, st, B, rocblas_datatype_f64_r, ld, st, &one, C,
+          rocblas_datatype_f64_r, ld, st, C, rocblas_datatype_f64_r, ld, st, batch,
+          rocblas_datatype_f64_r, rocblas_gemm_algo_solution_index, sol, rocblas_gemm_flags_none);
+        if (r == rocblas_status_success)
+          return r;
+      }
+#pragma clang diagnostic pop
+    return rocblas_dgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, M, N,
+                                         K, &mone, A, ld, st, B, ld, st, &one, C, ld, st, batch);
+  }
+
+  // groups whose factorisation holds at least this many FLOPs have their
+  // main trailing GEMM tuned
+  constexpr double GEMM_TUNE_MIN_FLOP = 5e9;
+  // wall time one solver may spend tuning (it is created once per mesh and
+  // component set; a served request creates one per stage)
+  constexpr double GEMM_TUNE_BUDGET_S = 0.3;
+
+  // The fastest rocBLAS solution for one GEMM shape, timed on scratch
+  // operands (A, B, C inside a front buffer; values do not matter): every
+  // solution rocBLAS lists for the shape against rocBLAS's default, one
+  // warm-up and one timed call each, on at most 2^26 / (M N) of the batch.
+  // 0 (the default) unless a solution is at least 5 % faster.  Results are
+  // kept per shape for the process.
+  int tune_gemm(rocblas_handle h, int M, int N, int K, const double *A, const double *B,
+                double *C, int ld, long long st, int batch_all)
+  {
+    const int batch = (int)std::max<long long>(
+      1, std::min<long long>(batch_all, (1ll << 26) / std::max<long long>(1, (long long)M * N)));
+    typedef std::tuple<int, int, int, int, long long, int> Key;
+    static std::map<Key, int> cache;
+    static std::mutex         mu;
+    const Key                 key{M, N, K, ld, st, batch};
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto                        it = cache.find(key);
+      if (it != cache.end())
+        return it->second;
+    }
+    const double one = 1.0, mone = -1.0;
+    rocblas_int  nsol = 0;
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wdeprecated-declarations"
+    auto list = [&](rocblas_int *arr) {
+      return rocblas_gemm_strided_batched_ex_get_solutions(
+        h, rocblas_operation_none, rocblas_operation_none, M, N, K, &mone, A,
+        rocblas_datatype_f64_r, ld, st, B, rocblas_datatype_f64_r, ld, st, &one, C,
+        rocblas_datatype_f64_r, ld, st, C, rocblas_datatype_f64_r, ld, st, batch,
+        rocblas_datatype_f64_r, rocblas_gemm_algo_solution_index, rocblas_gemm_flags_none, arr,
+        &nsol);
+    };
+#pragma clang diagnostic pop
+    int best = 0;
+    if (list(nullptr) == rocblas_status_success && nsol > 0)
+      {
+        std::vector<rocblas_int> sols((size_t)nsol);
+        if (list(sols.data()) == rocblas_status_success)
+          {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            rocblas_set_stream(h, nullptr);
+            if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+              {
+                auto ms = [&](int sol) -> float {
+                  if (strided_gemm(h, sol, M, N, K, A, B, C, ld, st, batch) !=
+                      rocblas_status_success)
+                    return -1.0f;
+                  (void)hipEventRecord(e0, nullptr);
+                  (void)strided_gemm(h, sol, M, N, K, A, B, C, ld, st, batch);
+                  (void)hipEventRecord(e1, nullptr);
+                  float t = -1.0f;
+                  if (hipEventSynchronize(e1) != hipSuccess ||
+                      hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+                    return -1.0f;
+                  return t;
+                };
+                const float tdef = ms(0);
+                float       tb   = tdef > 0 ? 0.95f * tdef : 0.0f;
+                for (rocblas_int sol : sols)
+                  {
+                    const float t = ms(sol);
+                    if (t > 0 && t < tb)
+                      {
+                        tb   = t;
+                        best = sol;
+                      }
+                  }
+              }
+            if (e0)
+              (void)hipEventDestroy(e0);
+            if (e1)
+              (void)hipEventDestroy(e1);
+          }
+      }
+    (void)hipGetLastError();
+    std::lock_guard<std::mutex> lk(mu);
+    cache[key] = best;
+    return best;
+  }
+
   int grid_for(long long work)
   {
     const long long g = (work + TPB - 1) / TPB;
@@ -865,6 +983,7 @@ struct hdg_trace_solver
   double               *d_part = nullptr; // backward sweep: chunk partial sums
   int                  *d_perm = nullptr; // row order of the small fronts' factored blocks
   long long            *d_map = nullptr, *d_pad = nullptr, *d_wmap = nullptr;
+  std::vector<int>      gemm_sol; // per group: the tuned rocBLAS solution (0: rocBLAS's)
 };
 
 extern "C" {
@@ -948,6 +1067,41 @@ int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
     }
   if (rc == 0 && rocblas_create_handle(&T->h) != rocblas_status_success)
     rc = hdg::set_error("hdg_trace_solver_create: rocblas_create_handle failed", -3);
+  if (rc == 0)
+    {
+      // the main trailing GEMM of the groups with enough work, tuned on the
+      // front buffer (its contents are written by each factorisation): the
+      // largest groups first, within GEMM_TUNE_BUDGET_S of wall time
+      T->gemm_sol.assign(P.groups.size(), 0);
+      std::vector<std::pair<double, size_t>> order;
+      for (size_t gi = 0; gi < P.groups.size(); ++gi)
+        {
+          const hdg::NDGroup &G = P.groups[gi];
+          const double        S = G.S, I = G.I;
+          const double flop = G.batch * (2.0 / 3.0 * S * S * S + 2.0 * S * S * I + 2.0 * S * I * I);
+          if (flop >= GEMM_TUNE_MIN_FLOP)
+            order.emplace_back(-flop, gi);
+        }
+      std::sort(order.begin(), order.end());
+      const auto t0 = std::chrono::steady_clock::now();
+      for (const auto &og : order)
+        {
+          if (std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >
+              GEMM_TUNE_BUDGET_S)
+            break;
+          const size_t        gi = og.second;
+          const hdg::NDGroup &G  = P.groups[gi];
+          const long long rest = (long long)G.n * G.n * G.batch;
+          const int       SB   = std::min((rest >= SP_MIN_ENTRIES ? SP : 1) * T->pw, G.S);
+          const int       M    = G.n - SB;
+          if (M <= 0)
+            continue;
+          double *Fg = T->d_F + G.fbase;
+          const long long nn = (long long)G.n * G.n;
+          T->gemm_sol[gi] = tune_gemm(T->h, M, M, SB, Fg + SB, Fg + (long long)SB * G.n,
+                                      Fg + SB + (long long)SB * G.n, G.n, nn, G.batch);
+        }
+    }
   if (rc != 0)
     {
       hdg_trace_solver_destroy(T);
@@ -1003,8 +1157,7 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
   if ((long long)T->P.pad.size() > 0)
     hipLaunchKernelGGL(k_set_one, dim3(grid_for((long long)T->P.pad.size())), dim3(TPB), 0, s, T->d_F, T->d_pad,
                        (long long)T->P.pad.size());
-  const double one = 1.0, mone = -1.0;
-  long long    info_off = 0;
+  long long info_off = 0;
   for (size_t gi = 0; gi < T->P.groups.size(); ++gi)
     {
       const hdg::NDGroup &G = T->P.groups[gi];
@@ -1086,12 +1239,9 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
         if (M <= 0 || N <= 0 || K <= 0)
           return true;
         double *Fg = T->d_F + G.fbase;
-        return rocblas_dgemm_strided_batched(
-                 T->h, rocblas_operation_none, rocblas_operation_none, M, N, K, &mone,
-                 Fg + r0 + (long long)k0 * G.n, G.n, (rocblas_stride)nn,
-                 Fg + k0 + (long long)c0 * G.n, G.n, (rocblas_stride)nn, &one,
-                 Fg + r0 + (long long)c0 * G.n, G.n, (rocblas_stride)nn,
-                 G.batch) == rocblas_status_success;
+        return strided_gemm(T->h, T->gemm_sol[gi], M, N, K, Fg + r0 + (long long)k0 * G.n,
+                            Fg + k0 + (long long)c0 * G.n, Fg + r0 + (long long)c0 * G.n, G.n,
+                            nn, G.batch) == rocblas_status_success;
       };
       bool ok = true;
       for (int o = 0, SBo = 0; o < G.S && ok; o += SBo)

@@ -34,7 +34,10 @@ def main(argv=None):
     rhs = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
     dirv = torch.zeros(ctx.ntrace, dtype=torch.float64, device="cuda")
     ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv)
+    import time
+    t0 = time.perf_counter()
     ts = s.trace_solver()
+    create_s = time.perf_counter() - t0
     sol = torch.empty(ctx.ntrace, dtype=torch.float64, device="cuda")
     for _ in range(2):
         ts.factor(vals)
@@ -55,6 +58,7 @@ def main(argv=None):
     s_ms = timed(lambda: ts.solve(rhs, sol))
     print(json.dumps({"dim": args.dim, "cells_per_axis": args.cells, "degree": args.degree,
                       "trace_dofs": int(ctx.ntrace), "factor_ms": f_ms, "solve_ms": s_ms,
+                      "create_s": create_s,
                       "factor_GFLOP": ts.flops / 1e9,
                       "factor_TFLOPs": ts.flops / (f_ms * 1e-3) / 1e12,
                       "groups": int(ts.levels), "max_front": int(ts.max_front),
