@@ -159,14 +159,16 @@ def batched_sample(args, torch, stream):
     cells = sum(s.ctx.ncells for s in sv)
     out = {"devices": len(sv), "cells": int(cells),
            "degrees": {str(k): sum(1 for s in sv if s.degree == k) for k in (1, 2, 3)}}
-    for prec in (0, 1):
+    for prec in (0, 1, 2):
         for s in sv:
             s.ctx.set_precision(prec)
         batched()                      # first call autotunes the grid
         torch.cuda.synchronize()
         wb, gb = timed(batched, reps)
         ws, gs = timed(sequential, reps)
-        key = "fp64" if prec == 0 else "fp32_refine"
+        # precision 1: FP32 + refinement in the one-wave kernels (degree 1-2),
+        # degree 3 in FP64; precision 2: in the degree-3 pair kernel too
+        key = ("fp64", "fp32_refine", "fp32_refine_every_kernel")[prec]
         out[key] = {"value": cells / wb, "unit": "cells/s", "gpu_ms": gb,
                     "sequential_value": cells / ws, "sequential_gpu_ms": gs}
     for s in sv:
@@ -256,8 +258,8 @@ def batched_newton(torch, sv):
     for s in sv:
         s.parameters.graph_newton = False
         s.ctx.set_precision(0)
-    out = {"precision": "fp32_refine (every degree; the equilibrium stage's degree-3 "
-                        "devices on the generic kernel in FP64)",
+    out = {"precision": "fp32_refine, precision 1 (the one-wave register kernels; degree 3 "
+                        "and the generic kernel's stages in FP64)",
            "states_bitwise_equal": bool(same)}
     for name in batched:
         b, q = batched[name], sequential[name]
@@ -423,13 +425,14 @@ def sample_3d_k2(args, torch, stream):
     p2 = timed(lambda: ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc))
     fm = timed(lambda: ts.factor(vals))
     sm = timed(lambda: ts.solve(rhs, sol))
-    # the same passes with the FP32 elimination and one FP64 refinement
-    # step, and its error against FP64 on a sample of cells
+    # the same passes with the generic kernel's FP32 elimination and FP64
+    # refinement (precision 2), and its error against FP64 on a sample of
+    # cells
     sample = torch.arange(0, ctx.ncells, max(1, ctx.ncells // 256), dtype=torch.int32,
                           device="cuda")
     T = ctx.T
     blocks = {}
-    for prec in (0, 1):
+    for prec in (0, 2):
         ctx.set_precision(prec)
         S_ = torch.empty(sample.numel() * T * T, dtype=torch.float64, device="cuda")
         g_ = torch.empty(sample.numel() * T, dtype=torch.float64, device="cuda")
@@ -438,14 +441,15 @@ def sample_3d_k2(args, torch, stream):
     p1m = timed(lambda: ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv))
     p2m = timed(lambda: ctx.reconstruct(s.x_cell, s.x_trace, dxt, False, dxc))
     ctx.set_precision(0)
-    (Sd, gd), (Sm, gm) = blocks[0], blocks[1]
+    (Sd, gd), (Sm, gm) = blocks[0], blocks[2]
     err_S = float(((Sm - Sd).norm(dim=1) / Sd.norm(dim=1)).max())
     return {"workload": f"3D diode {n}^3 hexahedra, degree 2, V/n/p (324 cell dofs, 162 trace "
                         "dofs), step junction N_D = N_A = 1e22 m^-3; generic kernels, FP64",
             "cells": int(ctx.ncells),
             "pass1": {"value": ctx.ncells / (p1 * 1e-3), "unit": "cells/s", "ms_per_pass": p1},
             "pass2_ms": p2, "factor_ms": fm, "solve_ms": sm,
-            "fp32_refine": {"pass1_ms": p1m, "pass2_ms": p2m, "vs_fp64": p1 / p1m,
+            "fp32_refine": {"precision": 2, "pass1_ms": p1m, "pass2_ms": p2m,
+                            "vs_fp64": p1 / p1m,
                             "max_rel_err_S_vs_fp64": err_S,
                             "error_sample_cells": int(sample.numel())},
             "factor_GFLOP": ts.flops / 1e9, "factor_TFLOPs": ts.flops / (fm * 1e-3) / 1e12,

@@ -363,8 +363,9 @@ int hdg_set_precision(hdg_ctx *C, int precision)
 {
   if (!C)
     return fail("hdg_set_precision: null context");
-  if (precision < 0 || precision > 1)
-    return fail("hdg_set_precision: precision must be 0 (FP64) or 1 (FP32 + refinement)", -2);
+  if (precision < 0 || precision > 2)
+    return fail("hdg_set_precision: precision must be 0 (FP64), 1 (FP32 + refinement in the "
+                "register kernels) or 2 (in every kernel)", -2);
   C->precision = precision;
   return 0;
 }

@@ -1003,14 +1003,17 @@ namespace hdg
   hipError_t launch_batch_pair(const KParams *plist, const int *chunk_dev, const int *items,
                                int nchunks, int color, int grid, int mixed, hipStream_t s)
   {
+    // FP32 + refinement at precision 2 only: three barrier passes of the
+    // pair where FP64 has one (slower than FP64)
     static bool a0 = false, a1 = false;
-    hipError_t  e = mixed ? lds_attr(batch_condense_p<true>, a1)
-                          : lds_attr(batch_condense_p<false>, a0);
+    const bool  mx = mixed == 2;
+    hipError_t  e  = mx ? lds_attr(batch_condense_p<true>, a1)
+                        : lds_attr(batch_condense_p<false>, a0);
     if (e != hipSuccess)
       return e;
     if (nchunks <= 0)
       return hipSuccess;
-    if (mixed)
+    if (mx)
       hipLaunchKernelGGL(batch_condense_p<true>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s,
                          plist, chunk_dev, items, nchunks, color);
     else
@@ -1029,14 +1032,15 @@ namespace hdg
                                     int mode, hipStream_t s)
   {
     static bool a0 = false, a1 = false;
-    hipError_t  e = p.mixed ? lds_attr(fast_condense_p<true>, a1)
-                            : lds_attr(fast_condense_p<false>, a0);
+    const bool  mx = p.mixed == 2;
+    hipError_t  e  = mx ? lds_attr(fast_condense_p<true>, a1)
+                        : lds_attr(fast_condense_p<false>, a0);
     if (e != hipSuccess)
       return e;
     if (ncell <= 0)
       return hipSuccess;
     const int grid = persistent_grid(ncell);
-    if (p.mixed)
+    if (mx)
       hipLaunchKernelGGL(fast_condense_p<true>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s,
                          p, cells, ncell, color, mode);
     else
@@ -1048,14 +1052,15 @@ namespace hdg
   hipError_t launch_fast2p_reconstruct(const KParams &p, int ncells, hipStream_t s)
   {
     static bool a0 = false, a1 = false;
-    hipError_t  e = p.mixed ? lds_attr(reconstruct_p<true>, a1)
-                            : lds_attr(reconstruct_p<false>, a0);
+    const bool  mx = p.mixed == 2;
+    hipError_t  e  = mx ? lds_attr(reconstruct_p<true>, a1)
+                        : lds_attr(reconstruct_p<false>, a0);
     if (e != hipSuccess)
       return e;
     if (ncells <= 0)
       return hipSuccess;
     const int grid = persistent_grid(ncells);
-    if (p.mixed)
+    if (mx)
       hipLaunchKernelGGL(reconstruct_p<true>, dim3(grid), dim3(64 * 2 * PPB), FP::bytes(), s, p,
                          ncells);
     else

@@ -102,7 +102,7 @@ namespace hdg
   hipError_t launch_fast2p_reconstruct(const KParams &p, int ncells, hipStream_t s);
   // the same pass 1 over several degree-3 meshes in one persistent launch
   // per colour (work list in chunks of batch_pairs_per_group() cells), FP64
-  // or FP32 + refinement (mixed)
+  // or, at precision 2 (mixed == 2), FP32 + refinement
   int        batch_pairs_per_group();
   hipError_t launch_batch_pair(const KParams *plist, const int *chunk_dev, const int *items,
                                int nchunks, int color, int grid, int mixed, hipStream_t s);

@@ -1400,7 +1400,9 @@ namespace hdg
       if (S::BIG && !p.rgscr)
         return hipErrorInvalidValue;
       const int grid = S::BIG ? std::min(ncell, GRID_BIG) : ncell;
-      if (p.mixed)
+      // precision 2 only: the generic kernel's FP32 form is slower than its
+      // FP64 one (precision 1 keeps FP32 to the register kernels)
+      if (p.mixed == 2)
         hipLaunchKernelGGL((condense_kernel<DIM, DEG, true>), dim3(grid), dim3(NT), S::bytes, s,
                            p, cells, ncell, color, mode);
       else
@@ -1421,7 +1423,7 @@ namespace hdg
       if (S::BIG && !p.rgscr)
         return hipErrorInvalidValue;
       const int grid = S::BIG ? std::min(ncells, GRID_BIG) : ncells;
-      if (p.mixed)
+      if (p.mixed == 2)
         hipLaunchKernelGGL((reconstruct_kernel<DIM, DEG, true>), dim3(grid), dim3(NT), S::bytes,
                            s, p, ncells);
       else

@@ -181,8 +181,9 @@ class HdgContext:
         self._refresh()
 
     def set_precision(self, precision):
-        """0: FP64; 1: FP32 elimination + one FP64 refinement step (pass 1 of
-        the register kernel)"""
+        """0: FP64; 1: FP32 elimination with FP64 refinement in the one-wave
+        register kernels (2D degree 1-2, 3D degree 1; both passes), FP64
+        elsewhere; 2: FP32 with FP64 refinement in every kernel"""
         _check(self.L.hdg_set_precision(self.ctx, int(precision)))
         self.precision = int(precision)
 

@@ -127,10 +127,12 @@ int hdg_set_enabled(hdg_ctx *ctx, int V, int n, int p);
 int hdg_set_kernel_variant(hdg_ctx *ctx, int variant);
 
 /* Precision of the local elimination of both passes: 0 = FP64 (default);
- * 1 = the scalar block eliminated in FP32 with one FP64 iterative-
- * refinement step (assembly, the residual, the refinement residual and the
- * Schur product in FP64), in every kernel (register-resident and generic),
- * pass 1 and pass 2.  No counterpart in the reference (FP64 throughout). */
+ * 1 = the scalar block eliminated in FP32 with FP64 iterative refinement
+ * (assembly, the residual, the refinement residuals and the Schur product
+ * in FP64) in the one-wave register kernels (2D degree 1-2, 3D degree 1),
+ * FP64 elsewhere; 2 = the same in every kernel, also the degree-3 pair
+ * kernel and the generic kernel, whose FP32 forms run slower than their
+ * FP64 ones.  No counterpart in the reference (FP64 throughout). */
 int hdg_set_precision(hdg_ctx *ctx, int precision);
 
 /* Restricted trace matrix pattern, host arrays: rowptr (NTRACE+1),

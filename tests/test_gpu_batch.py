@@ -35,7 +35,7 @@ def _solvers(thermodyn, precision=0, meshes=MESHES):
 
 
 @pytest.mark.parametrize("thermodyn", [False, True])
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_batch_equals_per_mesh(thermodyn, precision):
     t = _torch()
     print(f"case batch of {len(MESHES)} meshes thermo={thermodyn} precision={precision}",
@@ -64,7 +64,7 @@ def test_batch_equals_per_mesh(thermodyn, precision):
         assert t.equal(v, v1) and t.equal(r, r1) and t.equal(d, d1)
 
 
-@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1, 2])
 def test_degree3_batch_equals_per_mesh(precision):
     """degree-3 meshes in one persistent launch per colour of the pair kernel
     (FP64 and FP32 + refinement): every mesh's outputs bitwise equal to its

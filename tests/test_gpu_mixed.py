@@ -1,6 +1,9 @@
-"""FP32 local elimination with one FP64 refinement step (GPU tests): the
-per-cell condensed blocks against the FP64 oracle, with the error stated per
-case, and Newton on the diode converging to the FP64 solution."""
+"""FP32 local elimination with FP64 refinement (GPU tests): the per-cell
+condensed blocks and pass 2 against the FP64 oracle, with the error stated
+per case, and Newton on the diode converging to the FP64 solution.
+Precision 1 runs FP32 in the one-wave register kernels only; precision 2 in
+every kernel (the per-cell tests use it, so that the degree-3 pair kernel
+and the generic kernel are covered too)."""
 import numpy as np
 import pytest
 
@@ -35,7 +38,7 @@ def test_mixed_cell_systems_error(dim, n_cells, k, thermodyn):
     s = NPSolver(prob, params, adim)
     enabled = (True, False, False) if thermodyn else (True, True, True)
     s.set_enabled_components(*enabled)
-    s.ctx.set_precision(1)
+    s.ctx.set_precision(2)
     o = H.oracle_for(prob, params, adim, enabled)
     cell, tr = H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=31 + k)
     s.set_state(cell, tr)
@@ -68,7 +71,7 @@ def test_mixed_reconstruct_error(dim, n_cells, k, thermodyn):
     s = NPSolver(prob, params, adim)
     enabled = (True, False, False) if thermodyn else (True, True, True)
     s.set_enabled_components(*enabled)
-    s.ctx.set_precision(1)
+    s.ctx.set_precision(2)
     o = H.oracle_for(prob, params, adim, enabled)
     cell, tr = H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=41 + k)
     s.set_state(cell, tr)
@@ -86,8 +89,8 @@ def test_mixed_reconstruct_error(dim, n_cells, k, thermodyn):
     assert err < MIXED_BAR
 
 
-@pytest.mark.parametrize("n_cells,k", [(8, 2), (6, 3)])
-def test_mixed_newton_reaches_fp64_solution(n_cells, k):
+@pytest.mark.parametrize("n_cells,k,prec32", [(8, 2, 1), (6, 3, 2)])
+def test_mixed_newton_reaches_fp64_solution(n_cells, k, prec32):
     """the diode workflow (equilibrium, n/p, coupled, 0.1 V, 0.3 V) with the mixed
     elimination (both passes) converges to the FP64 run's solution (relative
     L-infinity 1e-10)"""
@@ -95,7 +98,7 @@ def test_mixed_newton_reaches_fp64_solution(n_cells, k):
     print(f"case mixed Newton diode 2D n={n_cells} k={k}", flush=True)
     from ddhdg_amd.np_solver import NPSolver
     states, its = [], []
-    for prec in (0, 1):
+    for prec in (0, prec32):
         prob, params, adim = configs.diode(2, n_cells, k)
         s = NPSolver(prob, params, adim)
         s.ctx.set_precision(prec)
@@ -112,7 +115,8 @@ def test_mixed_newton_reaches_fp64_solution(n_cells, k):
 
 
 # the generic kernel (forced; it also runs 3D degree 2 and the non-default
-# operator variants): (3, 2, 2) keeps its reduced systems in global scratch
+# operator variants) at precision 2 (precision 1 leaves it in FP64): (3, 2,
+# 2) keeps its reduced systems in global scratch
 GENERIC_CASES = [(2, 4, 1), (2, 4, 2), (2, 3, 3), (3, 2, 1), (3, 2, 2)]
 
 
@@ -129,7 +133,7 @@ def test_mixed_generic_kernel_error(dim, n_cells, k, thermodyn):
     enabled = (True, False, False) if thermodyn else (True, True, True)
     s.set_enabled_components(*enabled)
     s.ctx.set_kernel_variant(1)
-    s.ctx.set_precision(1)
+    s.ctx.set_precision(2)
     o = H.oracle_for(prob, params, adim, enabled)
     cell, tr = H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=51 + k)
     s.set_state(cell, tr)
@@ -158,16 +162,40 @@ def test_mixed_generic_kernel_error(dim, n_cells, k, thermodyn):
     assert eS < MIXED_BAR and eg < MIXED_BAR and e2 < MIXED_BAR
 
 
-def test_mixed_generic_variant_matches_fp64():
-    """a non-default operator variant (use_trace flux with per-cell-face
-    tau, only on the generic kernel) with the FP32 elimination: the global
-    system against the same kernel in FP64"""
+@pytest.mark.parametrize("k,variant", [(2, 1), (3, 0)])
+def test_precision_1_leaves_other_kernels_fp64(k, variant):
+    """precision 1 runs the generic kernel (variant 1) and the degree-3 pair
+    kernel in FP64: their blocks equal the FP64 ones bitwise"""
     t = _torch()
-    print("case mixed generic variant use_trace + cellface tau 2D k=2", flush=True)
+    print(f"case precision 1 k={k} variant={variant}", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(2, 4, k)
+    s = NPSolver(prob, params, adim)
+    s.ctx.set_kernel_variant(variant)
+    s.set_state(*H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=71))
+    ctx = s.ctx
+    cells = t.arange(ctx.ncells, dtype=t.int32, device="cuda")
+    out = []
+    for prec in (0, 1):
+        ctx.set_precision(prec)
+        S = t.empty(ctx.ncells * ctx.T * ctx.T, dtype=t.float64, device="cuda")
+        g = t.empty(ctx.ncells * ctx.T, dtype=t.float64, device="cuda")
+        ctx.cell_systems(s.x_cell, s.x_trace, False, cells, S, g)
+        out.append((S, g))
+    t.cuda.synchronize()
+    assert t.equal(out[0][0], out[1][0]) and t.equal(out[0][1], out[1][1])
+
+
+def test_mixed_generic_variant_matches_fp64():
+    """a non-default operator variant (use_trace flux, only on the generic
+    kernel) with the FP32 elimination (precision 2): the global system
+    against the same kernel in FP64"""
+    t = _torch()
+    print("case mixed generic variant use_trace 2D k=2", flush=True)
     from ddhdg_amd.np_solver import NPSolver
     prob, params, adim = configs.diode(2, 5, 2, dd_flux_type="use_trace")
     out = []
-    for prec in (0, 1):
+    for prec in (0, 2):
         s = NPSolver(prob, params, adim)
         s.ctx.set_precision(prec)
         s.set_state(*H.perturbed(*H.equilibrium_guess(prob, adim, 2), seed=61))
