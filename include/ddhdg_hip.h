@@ -299,6 +299,9 @@ enum
   HDG_TS_NINFO
 };
 
+/* Builds the plan and its index maps on the device, then times rocBLAS's
+ * GEMM solutions for the largest Schur updates (at most about 0.3 s of wall
+ * time per process for shapes not met before; the fastest is kept). */
 int  hdg_trace_solver_create(const hdg_ctx *ctx, hdg_trace_solver **out);
 void hdg_trace_solver_destroy(hdg_trace_solver *solver);
 int  hdg_trace_solver_info(const hdg_trace_solver *solver, double out[HDG_TS_NINFO]);
