@@ -10,14 +10,15 @@
 // uniform scalars: the terms of include/assembling_methods.template.h:
 // 269-2118 after the flux unknowns are eliminated (hdg_tables.h).
 //   1. scalar-test rows [A' | CT' | c'] in registers
-//   2. trace-test rows of the scalar columns, TC', staged in LDS
+//   2. (no storage of the trace-test rows: step 4 forms them face by face)
 //   3. Gauss-Jordan on the scalar rows with scaled column pivoting: equation k
 //      picks the unknown (lane) with the largest scaled |entry| (a DPP
 //      max-reduction), the pivot column reaches the other lanes by readlane;
 //      afterwards row k of a trace/rhs lane holds X(p_k, m),
 //      X = A'^-1 [CT' | c']
-//   4. trace/rhs lanes form their trace-test rows TT', t' face by face and
-//      subtract TC' X: [S | g] = [TT' - TC' A'^-1 CT' | t' - TC' A'^-1 c'],
+//   4. every lane forms its trace-test rows of one face at a time (TT', t'
+//      on the trace/rhs lanes, TC' on the scalar lanes, published in LDS
+//      for the matrix-core product) and the trace/rhs lanes subtract TC' X: [S | g] = [TT' - TC' A'^-1 CT' | t' - TC' A'^-1 c'],
 //      written straight into the trace CSR (two colour launches,
 //      deterministic)
 // Workgroups of 4 waves are persistent.  Tables read at lane-dependent
@@ -95,8 +96,8 @@ namespace hdg
       static constexpr int wBC   = wNEU + NFC * NE * NF;
       static constexpr int wBLK  = wBC + 64;      // 4 long long (blkptr of the faces)
       static constexpr int wFACE = wBLK + NFC;    // 4 long long (face index)
-      static constexpr int wTCL  = wFACE + NFC;   // NS*T: TC' column by column (pass 1)
-      static constexpr int wXCH  = wTCL + NS * T; // X rows / Schur rows to and from the MFMA layouts
+      static constexpr int wTCL  = wFACE + NFC;   // NS*TDF: TC' rows of one face (pass 1)
+      static constexpr int wXCH  = wTCL + NS * TDF; // X rows / Schur rows to and from the MFMA layouts
       static constexpr int XCH   = (TDF > 4 ? TDF : 4) * (T + 1);
       static constexpr int WDBL  = wXCH + XCH;    // per-wave doubles, pass 1
       static constexpr int WDBL2 = wTCL;          // pass 2 needs no TC'
@@ -663,10 +664,6 @@ namespace hdg
       for (int g = 0; g < NFC; ++g)
         {
           const int a = g >> 1, side = g & 1;
-          double    acc[TDF];
-#pragma unroll
-          for (int r = 0; r < TDF; ++r)
-            acc[r] = 0.0;
 #pragma unroll 1
           for (int s = 0; s < QF; ++s)
             {
@@ -675,29 +672,13 @@ namespace hdg
               face_flux(g, s, F);
 #pragma unroll
               for (int ci = 0; ci < NE; ++ci)
-                {
 #pragma unroll
-                  for (int t = 0; t < K1; ++t)
-                    {
-                      const int i = a == 0 ? side * DEG + K1 * t : t + K1 * side * DEG;
-                      R[ci * NB + i] += Bz[(QC * 3 + fi) * NB + i] * F[ci];
-                    }
-                  if (!PASS2)
-#pragma unroll
-                    for (int i = 0; i < NF; ++i)
-                      acc[ci * NF + i] += Bt[fi * NF + i] * F[ci];
-                }
+                for (int t = 0; t < K1; ++t)
+                  {
+                    const int i = a == 0 ? side * DEG + K1 * t : t + K1 * side * DEG;
+                    R[ci * NB + i] += Bz[(QC * 3 + fi) * NB + i] * F[ci];
+                  }
             }
-          // ---- 2: TC' (trace-test rows of the scalar columns) into LDS
-          if (!PASS2 && isU)
-#pragma unroll
-            for (int ci = 0; ci < NE; ++ci)
-              {
-                const double keepr = isdir[g * 3 + (NE == 1 ? 0 : ci)] ? 0.0 : 1.0;
-#pragma unroll
-                for (int i = 0; i < NF; ++i)
-                  tcl[lane * T + g * TDF + ci * NF + i] = keepr * acc[ci * NF + i];
-              }
         }
       // LocalCondenser::condense_ct_matrix: the Dirichlet update is in the
       // rhs bundle already; the Dirichlet columns are dropped
@@ -840,8 +821,15 @@ namespace hdg
 #pragma unroll 1
       for (int g = 0; g < NFC; ++g)
         {
+          // the trace-test rows of face g: TT', t' on the trace and rhs
+          // lanes, TC' on the scalar lanes (published for the A operand)
           double acc[TDF];
           trace_rows(g, acc);
+          if (isU)
+#pragma unroll
+            for (int r = 0; r < TDF; ++r)
+              tcl[lane * TDF + r] = acc[r];
+          WAVE_SYNC();
           dbl4 dm[NCT];
 #pragma unroll
           for (int tj = 0; tj < NCT; ++tj)
@@ -850,7 +838,7 @@ namespace hdg
           for (int kk = 0; kk < NKK; ++kk)
             {
               const bool   ok = 4 * kk + q16 < NS && c16 < TDF;
-              const double v  = tcl[pka[kk] * T + g * TDF + (ok ? c16 : 0)];
+              const double v  = tcl[pka[kk] * TDF + (ok ? c16 : 0)];
               const double a  = ok ? v : 0.0;
 #pragma unroll
               for (int tj = 0; tj < NCT; ++tj)

@@ -94,14 +94,23 @@ namespace
     double       *P  = F + pf[b];
     const long long np = pn[b];
     const int    *ps = pos + (long long)b * I;
-    const long long II = (long long)I * I;
-    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < II;
-         e += (long long)gridDim.x * blockDim.x)
+    // one column per wave (its parent column index is uniform), the rows
+    // over the lanes: coalesced reads of U and writes of P, no per-entry
+    // index division
+    const int wpb = blockDim.x >> 6, l = threadIdx.x & 63;
+    for (int c = blockIdx.x * wpb + (threadIdx.x >> 6); c < I; c += gridDim.x * wpb)
       {
-        const int r = (int)(e % I), c = (int)(e / I);
-        const int pr = ps[r], pc = ps[c];
-        if (pr >= 0 && pc >= 0)
-          P[pr + pc * np] += U[r + (long long)c * n];
+        const int pc = __builtin_amdgcn_readfirstlane(ps[c]);
+        if (pc < 0)
+          continue;
+        const double *Uc = U + (long long)c * n;
+        double       *Pc = P + (long long)pc * np;
+        for (int r = l; r < I; r += 64)
+          {
+            const int pr = ps[r];
+            if (pr >= 0)
+              Pc[pr] += Uc[r];
+          }
       }
   }
 
@@ -1286,7 +1295,7 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
             hipLaunchKernelGGL(
               k_extend_add,
-              dim3(std::max(1, std::min(4096, (int)(((long long)G.I * G.I + TPB - 1) / TPB))),
+              dim3(std::max(1, std::min(1024, (G.I + TPB / 64 - 1) / (TPB / 64))),
                    std::min(YCHUNK, G.batch - y0)),
               dim3(TPB), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, G.batch, D.d_pf, D.d_pn,
               D.d_slot, D.d_pos, want, y0);
