@@ -13,13 +13,17 @@
 // matrix cores in groups of few fronts) and one rocBLAS
 // strided-batched gemm (F22 -= L21 U12) on the rest of the front.  That is
 // the elimination of a chain of nodes of at most 72 dofs each, with
-// pivoting inside each node as in any multifrontal code.  An extend-add of
-// the update blocks into the parents follows, one launch per child slot so
-// that no two writes meet (deterministic sums, no atomics).  The fronts stay
+// pivoting inside each node as in any multifrontal code.  Before a level is
+// factored its fronts are assembled in one pass (pull form of the
+// extend-add): every entry is written once with the sum of its children's
+// update-block entries (slot 0, then slot 1; deterministic, no atomics, no
+// zero fill), then the level's matrix entries and padded diagonals are
+// added.  The fronts stay
 // in HBM as the factors for the forward and backward sweeps, which run the
 // same panels.
 #include <algorithm>
 #include <cstdint>
+#include <limits>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -48,6 +52,15 @@ namespace
   {
     long long *d_pf = nullptr, *d_pw = nullptr, *d_edof = nullptr, *d_idof = nullptr;
     int       *d_pn = nullptr, *d_slot = nullptr, *d_pos = nullptr;
+    // assembly of the group's fronts from its nodes' children (null for a
+    // group of leaves): per node and child slot, the child's Schur block
+    // (front offset, leading dimension) and, per front index of the node,
+    // the child's interface index there (-1: none)
+    long long *d_cbase = nullptr;
+    int       *d_cld = nullptr, *d_inv = nullptr;
+    // the group's matrix entries and padded unit diagonals: ranges of the
+    // solver's grouped lists
+    long long csr0 = 0, ncsr = 0, pad0 = 0, npad = 0;
   };
 
   constexpr int TPB = 256;
@@ -66,12 +79,65 @@ namespace
       a[i] = v;
   }
 
-  __global__ void k_scatter(const double *__restrict__ vals, const long long *__restrict__ map,
-                            long long nnz, double *__restrict__ F)
+  // the matrix entries of one group added into its fronts (entry src[k]
+  // of the CSR values at front offset dst[k]; no two share an offset)
+  __global__ void k_scatter_add(const double *__restrict__ vals, const int *__restrict__ src,
+                                const long long *__restrict__ dst, long long n,
+                                double *__restrict__ F)
   {
-    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < nnz;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
          k += (long long)gridDim.x * blockDim.x)
-      F[map[k]] = vals[k];
+      F[dst[k]] += vals[src[k]];
+  }
+
+  // every entry of the fronts of one group, written once: the sum of its
+  // children's Schur-complement entries that land there (child slot 0,
+  // then slot 1), zero where none does.  One column per wave, the rows over
+  // the lanes.  cbase null: a group of leaves (zeros).
+  __global__ void k_assemble(double *F, long long fbase, int n, int batch,
+                             const long long *__restrict__ cbase, const int *__restrict__ cld,
+                             const int *__restrict__ inv, int yoff)
+  {
+    const int b = blockIdx.y + yoff;
+    if (b >= batch)
+      return;
+    double         *Fb  = F + fbase + (long long)b * n * n;
+    const int       wpb = blockDim.x >> 6, l = threadIdx.x & 63;
+    const long long c0  = cbase ? cbase[2 * b] : -1, c1 = cbase ? cbase[2 * b + 1] : -1;
+    const int       ld0 = cbase ? cld[2 * b] : 0, ld1 = cbase ? cld[2 * b + 1] : 0;
+    const int      *i0  = cbase ? inv + (long long)(2 * b) * n : nullptr;
+    const int      *i1  = cbase ? i0 + n : nullptr;
+    for (int c = blockIdx.x * wpb + (threadIdx.x >> 6); c < n; c += gridDim.x * wpb)
+      {
+        const int k0 = c0 >= 0 ? __builtin_amdgcn_readfirstlane(i0[c]) : -1;
+        const int k1 = c1 >= 0 ? __builtin_amdgcn_readfirstlane(i1[c]) : -1;
+        double   *Fc = Fb + (long long)c * n;
+        if (k0 < 0 && k1 < 0)
+          {
+            for (int r = l; r < n; r += 64)
+              Fc[r] = 0.0;
+            continue;
+          }
+        const double *U0 = F + (k0 >= 0 ? c0 + (long long)k0 * ld0 : 0);
+        const double *U1 = F + (k1 >= 0 ? c1 + (long long)k1 * ld1 : 0);
+        for (int r = l; r < n; r += 64)
+          {
+            double v = 0.0;
+            if (k0 >= 0)
+              {
+                const int r0 = i0[r];
+                if (r0 >= 0)
+                  v = U0[r0];
+              }
+            if (k1 >= 0)
+              {
+                const int r1 = i1[r];
+                if (r1 >= 0)
+                  v += U1[r1];
+              }
+            Fc[r] = v;
+          }
+      }
   }
 
   __global__ void k_set_one(double *F, const long long *__restrict__ idx, long long n)
@@ -79,39 +145,6 @@ namespace
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
          k += (long long)gridDim.x * blockDim.x)
       F[idx[k]] = 1.0;
-  }
-
-  // F22 of each node of one group (child slot want) added into its parent
-  __global__ void k_extend_add(double *F, long long fbase, int n, int S, int I, int batch,
-                               const long long *__restrict__ pf, const int *__restrict__ pn,
-                               const int *__restrict__ slot, const int *__restrict__ pos,
-                               int want, int yoff)
-  {
-    const int b = blockIdx.y + yoff;
-    if (b >= batch || slot[b] != want || pf[b] < 0)
-      return;
-    const double *U  = F + fbase + (long long)b * n * n + S + (long long)S * n;
-    double       *P  = F + pf[b];
-    const long long np = pn[b];
-    const int    *ps = pos + (long long)b * I;
-    // one column per wave (its parent column index is uniform), the rows
-    // over the lanes: coalesced reads of U and writes of P, no per-entry
-    // index division
-    const int wpb = blockDim.x >> 6, l = threadIdx.x & 63;
-    for (int c = blockIdx.x * wpb + (threadIdx.x >> 6); c < I; c += gridDim.x * wpb)
-      {
-        const int pc = __builtin_amdgcn_readfirstlane(ps[c]);
-        if (pc < 0)
-          continue;
-        const double *Uc = U + (long long)c * n;
-        double       *Pc = P + (long long)pc * np;
-        for (int r = l; r < I; r += 64)
-          {
-            const int pr = ps[r];
-            if (pr >= 0)
-              Pc[pr] += Uc[r];
-          }
-      }
   }
 
   // ---------------------------------------------------------------------
@@ -992,8 +1025,143 @@ struct hdg_trace_solver
   double               *d_part = nullptr; // backward sweep: chunk partial sums
   int                  *d_perm = nullptr; // row order of the small fronts' factored blocks
   long long            *d_map = nullptr, *d_pad = nullptr, *d_wmap = nullptr;
+  int                  *d_csrc = nullptr; // CSR entry of each d_map offset (both grouped)
   std::vector<int>      gemm_sol; // per group: the tuned rocBLAS solution (0: rocBLAS's)
 };
+
+namespace
+{
+  // The tables that assemble each group's fronts just before the group is
+  // factored (hdg_trace_factor): per node, its children's Schur blocks and
+  // index maps (pull form of the extend-add: every front entry is written
+  // once, no zero fill, no read-modify-write of the parent), and the
+  // matrix entries and padded diagonals of each group as contiguous ranges.
+  int build_assembly(hdg_trace_solver *T)
+  {
+    const hdg::NDPlan &P  = T->P;
+    const int          ng = (int)P.groups.size();
+    std::vector<std::pair<long long, int>> by_base(ng);
+    for (int gi = 0; gi < ng; ++gi)
+      by_base[gi] = {P.groups[gi].fbase, gi};
+    std::sort(by_base.begin(), by_base.end());
+    // group and node of a front entry's offset
+    auto locate = [&](long long off, int &gi, long long &b, long long &e) {
+      auto it = std::upper_bound(by_base.begin(), by_base.end(),
+                                 std::make_pair(off, std::numeric_limits<int>::max()));
+      if (it == by_base.begin())
+        return false;
+      gi = std::prev(it)->second;
+      const hdg::NDGroup &G  = P.groups[gi];
+      const long long     nn = (long long)G.n * G.n;
+      b = (off - G.fbase) / nn;
+      e = (off - G.fbase) % nn;
+      return b >= 0 && b < G.batch;
+    };
+    std::vector<std::vector<long long>> cbase(ng);
+    std::vector<std::vector<int>>       cld(ng), inv(ng);
+    for (int gc = 0; gc < ng; ++gc)
+      {
+        const hdg::NDGroup &C = P.groups[gc];
+        for (int b = 0; b < C.batch; ++b)
+          {
+            if (C.pf[b] < 0 || C.I == 0)
+              continue;
+            int       gp;
+            long long bp, e;
+            if (!locate(C.pf[b], gp, bp, e) || e != 0 || C.slot[b] < 0 || C.slot[b] > 1)
+              return hdg::set_error("hdg_trace_solver_create: a parent front outside the plan", -4);
+            const hdg::NDGroup &G = P.groups[gp];
+            if (cbase[gp].empty())
+              {
+                cbase[gp].assign(2 * (size_t)G.batch, -1);
+                cld[gp].assign(2 * (size_t)G.batch, 0);
+                inv[gp].assign(2 * (size_t)G.batch * G.n, -1);
+              }
+            const long long k = 2 * bp + C.slot[b];
+            if (cbase[gp][k] >= 0)
+              return hdg::set_error("hdg_trace_solver_create: two children in one slot", -4);
+            cbase[gp][k] = C.fbase + (long long)b * C.n * C.n + C.S + (long long)C.S * C.n;
+            cld[gp][k]   = C.n;
+            for (int r = 0; r < C.I; ++r)
+              {
+                const int q = C.pos[(long long)b * C.I + r];
+                if (q < -1 || q >= G.n)
+                  return hdg::set_error("hdg_trace_solver_create: interface index out of range", -4);
+                if (q >= 0)
+                  inv[gp][k * G.n + q] = r;
+              }
+          }
+      }
+    // the matrix entries and the padded diagonals, grouped (stable)
+    auto group_lists = [&](const std::vector<long long> &offs, std::vector<long long> &out,
+                           std::vector<int> *src, bool pads) {
+      std::vector<long long> count(ng + 1, 0);
+      std::vector<int>       gof(offs.size());
+      for (size_t k = 0; k < offs.size(); ++k)
+        {
+          int       gi;
+          long long b, e;
+          if (!locate(offs[k], gi, b, e))
+            return false;
+          gof[k] = gi;
+          ++count[gi + 1];
+          if (pads)
+            {
+              // a padded diagonal receives nothing from the children
+              const hdg::NDGroup &G = P.groups[gi];
+              const int           j = (int)(e % G.n);
+              if (!inv[gi].empty())
+                for (int sl = 0; sl < 2; ++sl)
+                  if (inv[gi][(2 * b + sl) * G.n + j] >= 0)
+                    return false;
+            }
+        }
+      for (int gi = 0; gi < ng; ++gi)
+        count[gi + 1] += count[gi];
+      out.assign(offs.size(), 0);
+      if (src)
+        src->assign(offs.size(), 0);
+      std::vector<long long> at(count.begin(), count.end() - 1);
+      for (size_t k = 0; k < offs.size(); ++k)
+        {
+          const long long q = at[gof[k]]++;
+          out[q]            = offs[k];
+          if (src)
+            (*src)[q] = (int)k;
+        }
+      for (int gi = 0; gi < ng; ++gi)
+        {
+          DevGroup &D = T->dg[gi];
+          (pads ? D.pad0 : D.csr0) = count[gi];
+          (pads ? D.npad : D.ncsr) = count[gi + 1] - count[gi];
+        }
+      return true;
+    };
+    if (P.map.size() >= (size_t)std::numeric_limits<int>::max())
+      return hdg::set_error("hdg_trace_solver_create: more than 2^31 matrix entries", -4);
+    std::vector<long long> dst, pad;
+    std::vector<int>       src;
+    if (!group_lists(P.map, dst, &src, false))
+      return hdg::set_error("hdg_trace_solver_create: a matrix entry outside the fronts", -4);
+    if (!group_lists(P.pad, pad, nullptr, true))
+      return hdg::set_error("hdg_trace_solver_create: a padded diagonal outside the fronts or under a child", -4);
+    int rc = upload(&T->d_map, dst, "trace solver map");
+    if (rc == 0)
+      rc = upload(&T->d_csrc, src, "trace solver map");
+    if (rc == 0)
+      rc = upload(&T->d_pad, pad, "trace solver padding");
+    for (int gi = 0; gi < ng && rc == 0; ++gi)
+      {
+        DevGroup &D = T->dg[gi];
+        rc          = upload(&D.d_cbase, cbase[gi], "trace solver assembly");
+        if (rc == 0)
+          rc = upload(&D.d_cld, cld[gi], "trace solver assembly");
+        if (rc == 0)
+          rc = upload(&D.d_inv, inv[gi], "trace solver assembly");
+      }
+    return rc;
+  }
+} // namespace
 
 extern "C" {
 
@@ -1032,9 +1200,7 @@ int hdg_trace_solver_create(const hdg_ctx *C, hdg_trace_solver **out)
         rc = upload(&D.d_idof, G.idof, "trace solver tables");
     }
   if (rc == 0)
-    rc = upload(&T->d_map, P.map, "trace solver map");
-  if (rc == 0)
-    rc = upload(&T->d_pad, P.pad, "trace solver padding");
+    rc = build_assembly(T);
   if (rc == 0)
     rc = upload(&T->d_wmap, P.wmap, "trace solver rhs map");
   if (rc == 0)
@@ -1126,12 +1292,13 @@ void hdg_trace_solver_destroy(hdg_trace_solver *T)
     return;
   for (DevGroup &D : T->dg)
     for (void *p : {(void *)D.d_pf, (void *)D.d_pw, (void *)D.d_pn, (void *)D.d_slot,
-                    (void *)D.d_pos, (void *)D.d_edof, (void *)D.d_idof})
+                    (void *)D.d_pos, (void *)D.d_edof, (void *)D.d_idof, (void *)D.d_cbase,
+                    (void *)D.d_cld, (void *)D.d_inv})
       if (p)
         (void)hipFree(p);
   for (void *p : {(void *)T->d_F, (void *)T->d_w, (void *)T->d_part, (void *)T->d_perm,
                   (void *)T->d_info,
-                  (void *)T->d_map, (void *)T->d_pad, (void *)T->d_wmap})
+                  (void *)T->d_map, (void *)T->d_pad, (void *)T->d_wmap, (void *)T->d_csrc})
     if (p)
       (void)hipFree(p);
   if (T->h)
@@ -1158,19 +1325,27 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
   hipStream_t s = (hipStream_t)stream;
   rocblas_set_stream(T->h, s);
   rocblas_set_pointer_mode(T->h, rocblas_pointer_mode_host);
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(T->P.Ftot)), dim3(TPB), 0, s, T->d_F, T->P.Ftot, 0.0);
   hipLaunchKernelGGL(k_fill_int, dim3(grid_for(T->P.nnodes)), dim3(TPB), 0, s,
                      T->d_info, T->P.nnodes, 0);
-  hipLaunchKernelGGL(k_scatter, dim3(grid_for(T->P.nnz)), dim3(TPB), 0, s, csr_vals, T->d_map,
-                     T->P.nnz, T->d_F);
-  if ((long long)T->P.pad.size() > 0)
-    hipLaunchKernelGGL(k_set_one, dim3(grid_for((long long)T->P.pad.size())), dim3(TPB), 0, s, T->d_F, T->d_pad,
-                       (long long)T->P.pad.size());
   long long info_off = 0;
   for (size_t gi = 0; gi < T->P.groups.size(); ++gi)
     {
       const hdg::NDGroup &G = T->P.groups[gi];
       const DevGroup     &D = T->dg[gi];
+      // the group's fronts: its children's Schur blocks (every child group
+      // is factored by now), then its matrix entries and padded diagonals
+      for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
+        hipLaunchKernelGGL(k_assemble,
+                           dim3(std::max(1, std::min(1024, (G.n + TPB / 64 - 1) / (TPB / 64))),
+                                std::min(YCHUNK, G.batch - y0)),
+                           dim3(TPB), 0, s, T->d_F, G.fbase, G.n, G.batch, D.d_cbase, D.d_cld,
+                           D.d_inv, y0);
+      if (D.ncsr > 0)
+        hipLaunchKernelGGL(k_scatter_add, dim3(grid_for(D.ncsr)), dim3(TPB), 0, s, csr_vals,
+                           T->d_csrc + D.csr0, T->d_map + D.csr0, D.ncsr, T->d_F);
+      if (D.npad > 0)
+        hipLaunchKernelGGL(k_set_one, dim3(grid_for(D.npad)), dim3(TPB), 0, s, T->d_F,
+                           T->d_pad + D.pad0, D.npad);
       // the eliminated block in panels of at most pw dofs (whole faces),
       // SP panels to a super-panel: within a super-panel each panel's LU,
       // its triangular solves and a GEMM update of the super-panel's own
@@ -1290,15 +1465,6 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
       if (!ok)
         return hdg::set_error("hdg_trace_factor: rocblas gemm failed", -3);
       info_off += G.batch;
-      if (G.I > 0)
-        for (int want = 0; want < 2; ++want)
-          for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(
-              k_extend_add,
-              dim3(std::max(1, std::min(1024, (G.I + TPB / 64 - 1) / (TPB / 64))),
-                   std::min(YCHUNK, G.batch - y0)),
-              dim3(TPB), 0, s, T->d_F, G.fbase, G.n, G.S, G.I, G.batch, D.d_pf, D.d_pn,
-              D.d_slot, D.d_pos, want, y0);
     }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : hdg::set_error(std::string("hdg_trace_factor: ") + hipGetErrorString(e), -3);
