@@ -120,22 +120,34 @@ namespace
           }
         const double *U0 = F + (k0 >= 0 ? c0 + (long long)k0 * ld0 : 0);
         const double *U1 = F + (k1 >= 0 ? c1 + (long long)k1 * ld1 : 0);
-        for (int r = l; r < n; r += 64)
+        // four rows per lane at a time: their index and value loads are all
+        // in flight together
+        constexpr int RU = 4;
+        for (int rb = 0; rb < n; rb += 64 * RU)
           {
-            double v = 0.0;
-            if (k0 >= 0)
+            int r0[RU], r1[RU];
+#pragma unroll
+            for (int u = 0; u < RU; ++u)
               {
-                const int r0 = i0[r];
-                if (r0 >= 0)
-                  v = U0[r0];
+                const int  r  = rb + l + 64 * u;
+                const bool in = r < n;
+                r0[u]         = (in && k0 >= 0) ? i0[r] : -1;
+                r1[u]         = (in && k1 >= 0) ? i1[r] : -1;
               }
-            if (k1 >= 0)
+            double v0[RU], v1[RU];
+#pragma unroll
+            for (int u = 0; u < RU; ++u)
               {
-                const int r1 = i1[r];
-                if (r1 >= 0)
-                  v += U1[r1];
+                v0[u] = r0[u] >= 0 ? U0[r0[u]] : 0.0;
+                v1[u] = r1[u] >= 0 ? U1[r1[u]] : 0.0;
               }
-            Fc[r] = v;
+#pragma unroll
+            for (int u = 0; u < RU; ++u)
+              {
+                const int r = rb + l + 64 * u;
+                if (r < n)
+                  Fc[r] = r1[u] >= 0 ? v0[u] + v1[u] : v0[u];
+              }
           }
       }
   }
