@@ -81,13 +81,20 @@ namespace
 
   // the matrix entries of one group added into its fronts (entry src[k]
   // of the CSR values at front offset dst[k]; no two share an offset)
+  // (ADD false: the fronts hold zeros there, a group without children)
+  template <bool ADD>
   __global__ void k_scatter_add(const double *__restrict__ vals, const int *__restrict__ src,
                                 const long long *__restrict__ dst, long long n,
                                 double *__restrict__ F)
   {
     for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n;
          k += (long long)gridDim.x * blockDim.x)
-      F[dst[k]] += vals[src[k]];
+      {
+        if (ADD)
+          F[dst[k]] += vals[src[k]];
+        else
+          F[dst[k]] = vals[src[k]];
+      }
   }
 
   // every entry of the fronts of one group, written once: the sum of its
@@ -1352,9 +1359,12 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
                                 std::min(YCHUNK, G.batch - y0)),
                            dim3(TPB), 0, s, T->d_F, G.fbase, G.n, G.batch, D.d_cbase, D.d_cld,
                            D.d_inv, y0);
-      if (D.ncsr > 0)
-        hipLaunchKernelGGL(k_scatter_add, dim3(grid_for(D.ncsr)), dim3(TPB), 0, s, csr_vals,
-                           T->d_csrc + D.csr0, T->d_map + D.csr0, D.ncsr, T->d_F);
+      if (D.ncsr > 0 && D.d_cbase)
+        hipLaunchKernelGGL(k_scatter_add<true>, dim3(grid_for(D.ncsr)), dim3(TPB), 0, s,
+                           csr_vals, T->d_csrc + D.csr0, T->d_map + D.csr0, D.ncsr, T->d_F);
+      else if (D.ncsr > 0)
+        hipLaunchKernelGGL(k_scatter_add<false>, dim3(grid_for(D.ncsr)), dim3(TPB), 0, s,
+                           csr_vals, T->d_csrc + D.csr0, T->d_map + D.csr0, D.ncsr, T->d_F);
       if (D.npad > 0)
         hipLaunchKernelGGL(k_set_one, dim3(grid_for(D.npad)), dim3(TPB), 0, s, T->d_F,
                            T->d_pad + D.pad0, D.npad);
