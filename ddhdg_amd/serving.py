@@ -70,14 +70,17 @@ class IVResult:
 @functools.lru_cache(maxsize=256)
 def _trace_solver_bytes(dim, n_cells, tdf):
     """device bytes of the trace solver of a mesh: fronts, work vectors,
-    pivots, the CSR-to-front map and the groups' index lists, from the plan's
-    exact sizes (built on the host without its maps)"""
+    pivots and the groups' index lists, from the plan's exact sizes (built
+    on the host without its maps); the front-assembly maps are bounded by
+    two ints per front index (8 bytes per work entry) and 24 bytes per
+    node"""
     from .hip_lib import HdgError, trace_plan_size
     try:
         z = trace_plan_size(dim, n_cells, tdf)
     except HdgError:
         return 1 << 62          # no plan (a front beyond the solver's limits)
-    return 8 * z["front_entries"] + 24 * z["work_entries"] + 4 * z["pivot_entries"]
+    return (8 * z["front_entries"] + 32 * z["work_entries"] + 4 * z["pivot_entries"] +
+            24 * z["nodes"])
 
 
 def memory_estimate(req: IVRequest) -> int:
@@ -93,7 +96,8 @@ def memory_estimate(req: IVRequest) -> int:
     NC = 3 * (d + 1) * nb
     tdf = 3 * nf
     nnz = nfaces * (4 * d - 1) * tdf * tdf
-    solver = _trace_solver_bytes(d, n[:d], tdf) + 8 * nnz
+    # the matrix-entry map: front offset (8 bytes) and entry index (4)
+    solver = _trace_solver_bytes(d, n[:d], tdf) + 12 * nnz
     # the generic kernel's global scratch where its reduced systems do not
     # fit LDS (3D degree 2): 512 slices of R
     maxr = 3 * nb + 2 * d * 3 * nf
