@@ -45,12 +45,22 @@ namespace hdg
     // cells per chunk of a batched work list; a batched workgroup of 4 or
     // 8 waves (the tile the autotuner picks) takes one chunk at a time
     constexpr int BCH = 8;
+    // waves per SIMD the register budget is set for, and persistent
+    // workgroups per CU (two rounds of the resident ones), by degree and
+    // pass (256^2 diode: pass 1 at k = 2 1.88 -> 1.59 ms, k = 1 0.83 ->
+    // 0.64 ms, pass 2 at k = 2 1.20 -> 1.04 ms against two waves and four
+    // workgroups per CU); the batched kernel keeps two waves
 #ifndef WPE
 #define WPE 2
 #endif
-#ifndef WPE2
-#define WPE2 2
-#endif
+    constexpr int waves_per_simd(int deg, bool pass2)
+    {
+      return pass2 ? 3 : deg == 1 ? 4 : 3;
+    }
+    constexpr int groups_per_cu(int deg, bool pass2)
+    {
+      return pass2 ? 6 : deg == 1 ? 8 : 6;
+    }
 
     template <int DEG, int NE>
     struct F2
@@ -76,31 +86,37 @@ namespace hdg
       static constexpr int tPMF   = tPMC + QC * NB;
       static constexpr int tMFI   = tPMF + NFC * QF * NB;
       static constexpr int TAB    = tMFI + 2 * NF * NF;
-      // per-wave scratch (doubles)
-      static constexpr int wXS   = 0;
+      // per-wave scratch (doubles).  The first part lives through the
+      // whole cell; the second (cell state, point data, the residual forms
+      // of the scalar rows, pass 2's solution) is dead once the scalar-test
+      // rows are formed, and step 4's TC' rows and X / Schur exchange reuse
+      // it, so that three workgroups fit one CU's LDS
+      static constexpr int wU0F  = 0;
+      static constexpr int wW0F  = wU0F + 3 * NFC * QF;
+      static constexpr int wWRF  = wW0F + 3 * NFC * QF;
+      static constexpr int wLAMF = wWRF + 3 * NFC * QF;
+      static constexpr int wLUPD = wLAMF + 3 * NFC * QF;
+      static constexpr int wRESF = wLUPD + NFC * 3 * NF;
+      static constexpr int wNEU  = wRESF + NFC * QF * NE;
+      static constexpr int wBLK  = wNEU + NFC * NE * NF; // 4 long long (blkptr of the faces)
+      static constexpr int wFACE = wBLK + NFC;           // 4 long long (face index)
+      static constexpr int wEARLY = wFACE + NFC;
+      static constexpr int wXS   = wEARLY;
       static constexpr int wXT   = wXS + NCD;
       static constexpr int wU0   = wXT + NFC * 3 * NF;
       static constexpr int wW0   = wU0 + 3 * QC;
-      static constexpr int wU0F  = wW0 + 3 * 2 * QC;
-      static constexpr int wW0F  = wU0F + 3 * NFC * QF;
-      static constexpr int wTR0  = wW0F + 3 * NFC * QF;
+      static constexpr int wTR0  = wW0 + 3 * 2 * QC;
       static constexpr int wQD   = wTR0 + 3 * NFC * QF;
       static constexpr int wRHSQ = wQD + 6 * QC;
       static constexpr int wWRC  = wRHSQ + 3 * 2 * NB;
-      static constexpr int wWRF  = wWRC + 3 * 2 * QC;
-      static constexpr int wLAMF = wWRF + 3 * NFC * QF;
-      static constexpr int wLUPD = wLAMF + 3 * NFC * QF;
-      static constexpr int wRESC = wLUPD + NFC * 3 * NF;
-      static constexpr int wRESF = wRESC + QC * NE * 3;
-      static constexpr int wNEU  = wRESF + NFC * QF * NE;
-      static constexpr int wBC   = wNEU + NFC * NE * NF;
-      static constexpr int wBLK  = wBC + 64;      // 4 long long (blkptr of the faces)
-      static constexpr int wFACE = wBLK + NFC;    // 4 long long (face index)
-      static constexpr int wTCL  = wFACE + NFC;   // NS*TDF: TC' rows of one face (pass 1)
-      static constexpr int wXCH  = wTCL + NS * TDF; // X rows / Schur rows to and from the MFMA layouts
+      static constexpr int wRESC = wWRC + 3 * 2 * QC;
+      static constexpr int wBC   = wRESC + QC * NE * 3;
+      static constexpr int wEND  = wBC + 64;
+      static constexpr int wTCL  = wEARLY;           // NS*TDF: TC' rows of one face (step 4)
+      static constexpr int wXCH  = wTCL + NS * TDF;  // X rows / Schur rows to and from the MFMA layouts
       static constexpr int XCH   = (TDF > 4 ? TDF : 4) * (T + 1);
-      static constexpr int WDBL  = wXCH + XCH;    // per-wave doubles, pass 1
-      static constexpr int WDBL2 = wTCL;          // pass 2 needs no TC'
+      static constexpr int WDBL  = wXCH + XCH > wEND ? wXCH + XCH : wEND; // per-wave doubles, pass 1
+      static constexpr int WDBL2 = wEND;             // pass 2 needs no TC'
       // per-wave ints
       static constexpr int iDIR  = 0;             // NFC*3
       static constexpr int iNEU  = iDIR + NFC * 3;
@@ -239,7 +255,12 @@ namespace hdg
       constexpr int NS     = S::NS, TDF = S::TDF, T = S::T, NROW = S::NROW;
       constexpr int K1     = S::K1;
       double       *tab0 = smem;
-      const int     tid  = threadIdx.x;
+      // the thread index behind an empty asm: everything derived from it is
+      // formed again for every cell instead of being hoisted out of the
+      // cell loop and kept live across it (spilled, at three waves per SIMD)
+      int tid_ = threadIdx.x;
+      asm volatile("" : "+v"(tid_));
+      const int     tid  = tid_;
       const int     lane = tid & 63, wv = tid >> 6;
       double       *ws   = smem + S::TAB + wv * wstr;
       int *wi = reinterpret_cast<int *>(smem + S::TAB + nw * wstr) + wv * S::WINT;
@@ -919,7 +940,7 @@ namespace hdg
     // product in FP64
     template <int DEG, int NE, bool THERMO, bool PASS2, bool MIXED = false>
     __global__ void __launch_bounds__(64 * WPB)
-      __attribute__((amdgpu_waves_per_eu(PASS2 ? WPE2 : WPE)))
+      __attribute__((amdgpu_waves_per_eu(waves_per_simd(DEG, PASS2))))
       fast_condense(KParams p, const int *__restrict__ cells, int ncell, int color, int mode)
     {
       using S            = F2<DEG, NE>;
@@ -1041,7 +1062,8 @@ namespace hdg
       if (hipGetDevice(&dev) == hipSuccess)
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
       const int want = (ncell + WPB - 1) / WPB;
-      const int grid = want < 4 * ncu ? want : 4 * ncu;
+      const int gmax = groups_per_cu(DEG, PASS2) * ncu;
+      const int grid = want < gmax ? want : gmax;
       hipLaunchKernelGGL((fast_condense<DEG, NE, THERMO, PASS2, MIXED>), dim3(grid), dim3(64 * WPB),
                          bytes, s, p, cells, ncell, color, mode);
       return hipGetLastError();

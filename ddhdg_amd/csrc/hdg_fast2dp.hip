@@ -170,7 +170,12 @@ namespace hdg
       constexpr int NS     = S::NS, TDF = S::TDF, T = S::T, NE = S::NE;
       constexpr int K1     = S::K1, DEG = S::DEG;
       constexpr bool THERMO = false;
-      const int      tid    = threadIdx.x;
+      // the thread index behind an empty asm: everything derived from it is
+      // formed again for every cell instead of being hoisted out of the
+      // cell loop and kept live (spilled) across it
+      int tid_ = threadIdx.x;
+      asm volatile("" : "+v"(tid_));
+      const int      tid    = tid_;
       // the wave's index is uniform: readfirstlane makes that visible, so
       // branches on half stay scalar
       const int      wv   = __builtin_amdgcn_readfirstlane(tid >> 6);

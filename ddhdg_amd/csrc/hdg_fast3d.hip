@@ -229,7 +229,12 @@ namespace hdg
       // the persistent cell loop (hundreds of live values)
       double *tab = smem;
       asm volatile("" : "+v"(tab));
-      const int     tid = threadIdx.x;
+      // the thread index behind an empty asm as well: everything derived
+      // from it is formed again for every cell instead of being hoisted
+      // out of the cell loop and kept live (spilled) across it
+      int tid_ = threadIdx.x;
+      asm volatile("" : "+v"(tid_));
+      const int     tid = tid_;
       const int     lane = tid & 63, wv = tid >> 6;
       constexpr int WST = S::wdbl(MIXED);
       double       *ws  = smem + S::TAB + wv * WST;
