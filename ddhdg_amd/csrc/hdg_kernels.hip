@@ -111,6 +111,15 @@ namespace hdg
 
     // workgroups of a generic launch whose R and G live in global scratch
     constexpr int GRID_BIG = 512;
+    // the thread index behind an empty asm: what is derived from it is
+    // formed where it is used instead of being hoisted out of the
+    // persistent cell loop and kept live (spilled) across it
+    __device__ __forceinline__ int opaque_tid()
+    {
+      int t = threadIdx.x;
+      asm volatile("" : "+v"(t));
+      return t;
+    }
 
     // base of R and G for this workgroup
     template <int DIM, int DEG>
@@ -135,7 +144,7 @@ namespace hdg
       using S           = Sh<DIM, DEG>;
       constexpr int NB  = S::NB, NF = S::NF, QC = S::QC, QF = S::QF;
       constexpr int NFC = S::NFC, LD = S::LD, SZ = S::SZ;
-      const int     tid = threadIdx.x;
+      const int     tid = opaque_tid();
       double *rg = rg_base<DIM, DEG>(p, sm);
       double *R = rg + S::oR, *G = rg + S::oG, *xs = sm + S::oXS, *xt = sm + S::oXT;
       double *u0 = sm + S::oU0, *W0 = sm + S::oW0, *u0f = sm + S::oU0F;
@@ -633,7 +642,7 @@ namespace hdg
     {
       using S          = Sh<DIM, DEG>;
       constexpr int LD = S::LD;
-      const int     tid = threadIdx.x;
+      const int     tid = opaque_tid();
       double *R = rg + S::oR, *rs = sm + S::oRS, *mul = sm + S::oMUL;
       int    *used = si + S::iUSED, *piv = si + S::iPIV;
       for (int i = tid; i < nrow; i += NT)
@@ -709,7 +718,7 @@ namespace hdg
     {
       using S   = Sh<DIM, DEG>;
       static_assert(2 * S::MAXNS <= NT, "one update lane pair per row");
-      const int tid = threadIdx.x;
+      const int tid = opaque_tid();
       double *rs = sm + S::oRS;
       int    *used = si + S::iUSED, *piv = si + S::iPIV;
       for (int i = tid; i < nr; i += NT)
@@ -899,7 +908,7 @@ namespace hdg
     {
       using S          = Sh<DIM, DEG>;
       constexpr int LD = S::LD;
-      const int     tid = threadIdx.x;
+      const int     tid = opaque_tid();
       for (int i = tid; i < ns; i += NT)
         {
           double m = 0, sum = 0;
@@ -1033,7 +1042,7 @@ namespace hdg
       static_assert(16 * RA * KC + KC * 16 * CB <= MAXNS * AL, "staging fits");
       double    *A   = sm + S::oAW;
       const int *piv = si + S::iPIV;
-      const int  tid = threadIdx.x;
+      const int  tid = opaque_tid();
       const int  W   = T + 1;
       for (int e = tid; e < ns * ns; e += NT)
         {
@@ -1065,7 +1074,7 @@ namespace hdg
       constexpr int KC    = 9;                      // k per staged chunk
       constexpr int RA    = 6;                      // row slots per pass
       constexpr int CB    = (S::MAXT + 1 + 15) / 16; // column slots
-      const int     tid   = threadIdx.x;
+      const int     tid   = opaque_tid();
       const int     W     = T + 1;
       constexpr int WP = 16 * CB;       // staged Y width (zero beyond W)
       double   *Ms = A;                // [16 RA][KC], rows of this pass
@@ -1134,7 +1143,7 @@ namespace hdg
       constexpr int LD = S::LD;
       double       *R   = rg + S::oR;
       const int    *piv = si + S::iPIV;
-      const int     tid = threadIdx.x;
+      const int     tid = opaque_tid();
       int           al;
       float        *A32 = mixed_a32<DIM, DEG>(rg, sm, al);
       double       *rsc = sm + S::oMUL, *csc = rsc + S::MAXNS, *rsum = sm + S::oSOL;
@@ -1187,7 +1196,7 @@ namespace hdg
       double       *del = sm + S::oDEL;
       int          *isdir = si + S::iDIR, *bidv = si + S::iBID;
       long long    *facev = reinterpret_cast<long long *>(si + S::iFACE);
-      const int     tid   = threadIdx.x;
+      const int     tid   = opaque_tid();
       const int     tdf   = p.ne * NF;
       const int     nrow = ns + T, ncol = ns + T + 1;
       // LocalCondenser::condense_ct_matrix (src/local_condenser.cpp:141-160)
@@ -1282,7 +1291,7 @@ namespace hdg
       double       *R = rg + S::oR, *sol = sm + S::oSOL, *rhsq = sm + S::oRHSQ;
       double       *lupd = sm + S::oLUPD;
       int          *piv  = si + S::iPIV;
-      const int     tid  = threadIdx.x;
+      const int     tid  = opaque_tid();
       if constexpr (MIXED)
         {
           // u = A'^-1 c' in FP32 with one FP64 refinement step
