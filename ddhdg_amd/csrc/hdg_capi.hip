@@ -849,12 +849,12 @@ int hdg_batch_condense(hdg_batch *B, int thermodyn, const double *const *x_cell,
       // 8; the pair kernel has one) and persistent grids (workgroups per
       // CU), each timed on this batch over a complete two-colour pass (the
       // outputs are the same for every candidate); the fastest is kept
-      const int tiles4[2] = {4, 8}, per_cu4[2][3] = {{1, 2, 4}, {1, 2, 0}};
+      const int tiles4[2] = {4, 8}, per_cu4[2][5] = {{1, 2, 3, 4, 6}, {1, 2, 0, 0, 0}};
       const int ntile = B->deg3 ? 1 : 2;
       const int mx    = std::max(1, std::max(B->nchunks[0], B->nchunks[1]));
       B->ncand        = 0;
       for (int ti = 0; ti < ntile; ++ti)
-        for (int k = 0; k < 3; ++k)
+        for (int k = 0; k < 5; ++k)
           if (per_cu4[ti][k] > 0)
             {
               B->cand_tile[B->ncand] = B->deg3 ? hdg::batch_pairs_per_group() : tiles4[ti];

@@ -49,10 +49,11 @@ namespace hdg
     // workgroups per CU (two rounds of the resident ones), by degree and
     // pass (256^2 diode: pass 1 at k = 2 1.88 -> 1.59 ms, k = 1 0.83 ->
     // 0.64 ms, pass 2 at k = 2 1.20 -> 1.04 ms against two waves and four
-    // workgroups per CU); the batched kernel keeps two waves
-#ifndef WPE
-#define WPE 2
-#endif
+    // workgroups per CU).  The batched kernel (degree 1 and 2 in one body)
+    // keeps two waves: built for three, its FP32 + refinement form gave
+    // blocks different from the per-mesh kernel's in the batch test (cause
+    // not found)
+    constexpr int BATCH_WPE = 2;
     constexpr int waves_per_simd(int deg, bool pass2)
     {
       return pass2 ? 3 : deg == 1 ? 4 : 3;
@@ -965,7 +966,7 @@ namespace hdg
     // chunk's device changes.  LDS is sized for degree 2; a degree-1 device
     // uses the first part.
     template <int NE, bool THERMO, bool MIXED, int NW>
-    __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(WPE)))
+    __global__ void __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(BATCH_WPE)))
       batch_condense(const KParams *__restrict__ plist, const int *__restrict__ chunk_dev,
                      const int *__restrict__ items, int nchunks, int color)
     {
