@@ -384,8 +384,10 @@ namespace
                  const int *__restrict__ perm, long long pbase, int ps, int nbl, int batch,
                  int yoff)
   {
-    __shared__ double Ts[PW_MAX * PW_MAX]; // left: L row-major; right: U column-major
-    __shared__ int    pm[PW_MAX];
+    // the factor, S x S (left: L row-major; right: U column-major, leading
+    // dimension S): dynamic, so a small panel leaves room for more waves
+    extern __shared__ double Ts[];
+    __shared__ int pm[PW_MAX];
     const int b = blockIdx.y + yoff;
     if (b >= batch)
       return;
@@ -394,9 +396,9 @@ namespace
     const int  r    = (left ? (int)blockIdx.x : (int)blockIdx.x - nbl) * 64 + t;
     double    *Fb   = F + fbase + (long long)b * n * n;
     if (left)
-      stage_block<16>(Fb, n, S, Ts, PW_MAX, 1, t);
+      stage_block<16>(Fb, n, S, Ts, S, 1, t);
     else
-      stage_block<16>(Fb, n, S, Ts, 1, PW_MAX, t);
+      stage_block<16>(Fb, n, S, Ts, 1, S, t);
     if (left && t < S)
       pm[t] = perm[pbase + (long long)b * ps + t];
     __syncthreads();
@@ -419,9 +421,9 @@ namespace
 #pragma unroll
             for (int l = 0; l < j; ++l)
               if (l & 1)
-                a1 -= Ts[j * PW_MAX + l] * y[l];
+                a1 -= Ts[j * S + l] * y[l];
               else
-                a0 -= Ts[j * PW_MAX + l] * y[l];
+                a0 -= Ts[j * S + l] * y[l];
             y[j] = a0 + a1;
           }
         if (act)
@@ -447,10 +449,10 @@ namespace
 #pragma unroll
             for (int l = 0; l < j; ++l)
               if (l & 1)
-                a1 -= y[l] * Ts[l + j * PW_MAX];
+                a1 -= y[l] * Ts[l + j * S];
               else
-                a0 -= y[l] * Ts[l + j * PW_MAX];
-            y[j] = (a0 + a1) / Ts[j + j * PW_MAX];
+                a0 -= y[l] * Ts[l + j * S];
+            y[j] = (a0 + a1) / Ts[j + j * S];
           }
         if (act)
 #pragma unroll
@@ -479,7 +481,7 @@ namespace
   // coalesced through LDS.  SM >= S bounds the blocks; the factor is padded
   // with the identity and the tile with zeros.
   template <int SM>
-  __global__ void __launch_bounds__(256)
+  __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
     k_trsm_mfma(double *F, long long fbase, int n, int S, int Ic, int c0, int Ir,
                 const int *__restrict__ perm, long long pbase, int ps, int nbl, int batch,
                 int yoff)
@@ -1437,7 +1439,7 @@ int hdg_trace_factor(hdg_trace_solver *T, const double *csr_vals, void *stream)
           }
         for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
           hipLaunchKernelGGL(k_panel_trsm, dim3(nbc + nbr, std::min(YCHUNK, G.batch - y0)),
-                             dim3(64), 0, s, T->d_F, fb, G.n, Sp, Ic, c0, Ir, T->d_perm,
+                             dim3(64), (size_t)Sp * Sp * sizeof(double), s, T->d_F, fb, G.n, Sp, Ic, c0, Ir, T->d_perm,
                              G.pbase + q0, G.S, nbc, G.batch, y0);
       };
       // F(r0.., c0..) -= F(r0.., k0..k0+K) F(k0..k0+K, c0..), M x N, every front
