@@ -46,9 +46,9 @@ namespace hdg
     // 8 waves (the tile the autotuner picks) takes one chunk at a time
     constexpr int BCH = 8;
     // waves per SIMD the register budget is set for, and persistent
-    // workgroups per CU (two rounds of the resident ones), by degree and
-    // pass (256^2 diode: pass 1 at k = 2 1.88 -> 1.59 ms, k = 1 0.83 ->
-    // 0.64 ms, pass 2 at k = 2 1.20 -> 1.04 ms against two waves and four
+    // workgroups per CU (four rounds of the resident ones), by degree and
+    // pass (256^2 diode: pass 1 at k = 2 1.88 -> 1.55 ms, k = 1 0.83 ->
+    // 0.62 ms, pass 2 at k = 2 1.20 -> 1.00 ms against two waves and four
     // workgroups per CU).  The batched kernel (degree 1 and 2 in one body)
     // keeps two waves: built for three, its FP32 + refinement form gave
     // blocks different from the per-mesh kernel's in the batch test (cause
@@ -60,7 +60,7 @@ namespace hdg
     }
     constexpr int groups_per_cu(int deg, bool pass2)
     {
-      return pass2 ? 6 : deg == 1 ? 8 : 6;
+      return pass2 ? 12 : deg == 1 ? 16 : 12;
     }
 
     template <int DEG, int NE>

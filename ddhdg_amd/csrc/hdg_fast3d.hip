@@ -1043,7 +1043,9 @@ namespace hdg
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
       constexpr int WPB  = wpb3(MIXED);
       const int     want = (ncell + WPB - 1) / WPB;
-      const int     grid = want < 4 * ncu ? want : 4 * ncu;
+      // eight rounds of the two workgroups a CU holds (48^3 pass 1: 8.53
+      // ms at two rounds, 8.30 ms at eight)
+      const int     grid = want < 16 * ncu ? want : 16 * ncu;
       hipLaunchKernelGGL((fast_condense3<NE, THERMO, MIXED, PASS2>), dim3(grid), dim3(64 * WPB),
                          bytes, s, p, cells, ncell, color, mode);
       return hipGetLastError();
