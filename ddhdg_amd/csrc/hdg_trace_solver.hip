@@ -628,7 +628,40 @@ namespace
   // solved entry reaches the other lanes by readlane), then w_i -= L21 z.
   // perm: row k of the factored block is row perm[k] of the panel; front
   // b's at pbase + b*ps.
-  __global__ void __launch_bounds__(64)
+  // the S x S block at Fb (leading dimension n) into LDS column-major by
+  // the whole workgroup (NT threads), eight loads in flight per thread
+  template <int NT>
+  __device__ __forceinline__ void stage_square(const double *Fb, long long n, int S, double *out,
+                                               int t)
+  {
+    const int SS = S * S;
+#pragma unroll 1
+    for (int e0 = 0; e0 < SS; e0 += 8 * NT)
+      {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          {
+            const int e = e0 + u * NT + t;
+            v[u]        = e < SS ? Fb[(e % S) + (long long)(e / S) * n] : 0.0;
+          }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          {
+            const int e = e0 + u * NT + t;
+            if (e < SS)
+              out[e] = v[u];
+          }
+      }
+  }
+
+  // forward sweep of one panel of a front, four waves: the factor staged by
+  // all of them; wave 0 solves z = L11^-1 P w_s (z in registers, entries by
+  // readlane); then every thread updates rows of the short rest,
+  // w_rest -= L21 z (I rows, one per thread; 0 when a separate kernel does
+  // it)
+  constexpr int SWEEP_T = 256;
+  __global__ void __launch_bounds__(SWEEP_T)
     k_fwd_small(const double *F, double *w, long long fbase, long long wbase, int n, int S,
                 int I, const int *perm, long long pbase, int ps, int batch, int yoff)
   {
@@ -640,31 +673,30 @@ namespace
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
     const int    *pv = perm + pbase + (long long)b * ps;
-    stage_block<32>(Fb, n, S, L, 1, S, t);
-    const int i0 = t, i1 = t + 64;
-    double    z0 = i0 < S ? wb[pv[i0]] : 0.0;
-    double    z1 = i1 < S ? wb[pv[i1]] : 0.0;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int j = 0; j + 1 < S; ++j)
+    stage_square<SWEEP_T>(Fb, n, S, L, t);
+    __syncthreads();
+    if (t < 64)
       {
-        const double y = j < 64 ? lane_value(z0, j) : lane_value(z1, j - 64);
-        if (i0 > j && i0 < S)
-          z0 -= L[i0 + j * S] * y;
-        if (i1 > j && i1 < S)
-          z1 -= L[i1 + j * S] * y;
+        const int i0 = t, i1 = t + 64;
+        double    z0 = i0 < S ? wb[pv[i0]] : 0.0;
+        double    z1 = i1 < S ? wb[pv[i1]] : 0.0;
+        for (int j = 0; j + 1 < S; ++j)
+          {
+            const double y = j < 64 ? lane_value(z0, j) : lane_value(z1, j - 64);
+            if (i0 > j && i0 < S)
+              z0 -= L[i0 + j * S] * y;
+            if (i1 > j && i1 < S)
+              z1 -= L[i1 + j * S] * y;
+          }
+        // every lane has read its pivoted entries of w_s: overwrite them
+        __builtin_amdgcn_wave_barrier();
+        if (i0 < S)
+          wb[i0] = z0;
+        if (i1 < S)
+          wb[i1] = z1;
       }
-    // every lane has read its pivoted entries of w_s: overwrite them
-    __builtin_amdgcn_wave_barrier();
-    if (i0 < S)
-      wb[i0] = z0;
-    if (i1 < S)
-      wb[i1] = z1;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int r = t; r < I; r += 64)
+    __syncthreads();
+    for (int r = t; r < I; r += SWEEP_T)
       {
         double a0 = wb[S + r], a1 = 0, a2 = 0, a3 = 0;
         int    j  = 0;
@@ -681,59 +713,65 @@ namespace
       }
   }
 
-  // backward sweep of one panel of a front, one wave: z = w_s - U12 x_i,
-  // then U11^-1 z with U11 staged in LDS and z in registers
-  __global__ void __launch_bounds__(64)
+  // backward sweep of one panel of a front, four waves: z = w_s - U12 x_i
+  // (wave q sums the columns c = q, q + 4, ... of each row; the four
+  // partial sums are added in order), then wave 0 applies U11^-1 with U11
+  // staged in LDS and z in registers
+  __global__ void __launch_bounds__(SWEEP_T)
     k_bwd_small(const double *F, double *w, long long fbase, long long wbase, int n, int S,
                 int I, int batch, int yoff)
   {
-    extern __shared__ double U[]; // S x S, column-major
+    extern __shared__ double U[]; // S x S column-major, then 4 x S partial sums
     const int b = blockIdx.x + yoff;
     if (b >= batch)
       return;
-    const int     t  = threadIdx.x;
+    const int     t  = threadIdx.x, l = t & 63, q = t >> 6;
     const double *Fb = F + fbase + (long long)b * n * n;
     double       *wb = w + wbase + (long long)b * n;
-    stage_block<32>(Fb, n, S, U, 1, S, t);
-    const int i0 = t, i1 = t + 64;
-    double    z0 = 0.0, z1 = 0.0;
+    double       *part = U + S * S;
+    stage_square<SWEEP_T>(Fb, n, S, U, t);
     for (int h = 0; h < 2; ++h)
       {
-        const int i = h ? i1 : i0;
+        const int i = h ? l + 64 : l;
         if (i >= S)
           continue;
-        double a0 = wb[i], a1 = 0, a2 = 0, a3 = 0;
-        int    c  = 0;
-        for (; c + 3 < I; c += 4)
+        double a0 = q == 0 ? wb[i] : 0.0, a1 = 0;
+        int    c  = q;
+        for (; c + 4 < I; c += 8)
           {
             a0 -= Fb[i + (long long)(S + c) * n] * wb[S + c];
-            a1 -= Fb[i + (long long)(S + c + 1) * n] * wb[S + c + 1];
-            a2 -= Fb[i + (long long)(S + c + 2) * n] * wb[S + c + 2];
-            a3 -= Fb[i + (long long)(S + c + 3) * n] * wb[S + c + 3];
+            a1 -= Fb[i + (long long)(S + c + 4) * n] * wb[S + c + 4];
           }
-        for (; c < I; ++c)
+        for (; c < I; c += 4)
           a0 -= Fb[i + (long long)(S + c) * n] * wb[S + c];
-        (h ? z1 : z0) = (a0 + a1) + (a2 + a3);
+        part[q * S + i] = a0 + a1;
       }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int j = S - 1; j >= 0; --j)
+    __syncthreads();
+    if (t < 64)
       {
-        const double y = (j < 64 ? lane_value(z0, j) : lane_value(z1, j - 64)) / U[j + j * S];
-        if (i0 == j)
-          z0 = y;
-        else if (i0 < j)
-          z0 -= U[i0 + j * S] * y;
-        if (i1 == j)
-          z1 = y;
-        else if (i1 < j)
-          z1 -= U[i1 + j * S] * y;
+        const int i0 = t, i1 = t + 64;
+        double    z0 = 0.0, z1 = 0.0;
+        if (i0 < S)
+          z0 = ((part[i0] + part[S + i0]) + part[2 * S + i0]) + part[3 * S + i0];
+        if (i1 < S)
+          z1 = ((part[i1] + part[S + i1]) + part[2 * S + i1]) + part[3 * S + i1];
+        for (int j = S - 1; j >= 0; --j)
+          {
+            const double y = (j < 64 ? lane_value(z0, j) : lane_value(z1, j - 64)) / U[j + j * S];
+            if (i0 == j)
+              z0 = y;
+            else if (i0 < j)
+              z0 -= U[i0 + j * S] * y;
+            if (i1 == j)
+              z1 = y;
+            else if (i1 < j)
+              z1 -= U[i1 + j * S] * y;
+          }
+        if (i0 < S)
+          wb[i0] = z0;
+        if (i1 < S)
+          wb[i1] = z1;
       }
-    if (i0 < S)
-      wb[i0] = z0;
-    if (i1 < S)
-      wb[i1] = z1;
   }
 
 
@@ -1538,7 +1576,7 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
           const bool      fuse = Ip <= FUSE_I;
           const long long fb   = G.fbase + (long long)o * G.n + o;
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-            hipLaunchKernelGGL(k_fwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
+            hipLaunchKernelGGL(k_fwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(SWEEP_T),
                                (size_t)Sp * Sp * sizeof(double), s, T->d_F, T->d_w, fb,
                                G.wbase + o, G.n, Sp, fuse ? Ip : 0, T->d_perm, G.pbase + o, G.S,
                                G.batch, y0);
@@ -1599,8 +1637,8 @@ int hdg_trace_solve(hdg_trace_solver *T, const double *rhs, double *x, void *str
                                        G.wbase + o, G.n, Sp, nch, T->d_part, G.batch, y0);
                   }
               for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
-                hipLaunchKernelGGL(k_bwd_small, dim3(std::min(YCHUNK, G.batch - y0)), dim3(64),
-                                   (size_t)Sp * Sp * sizeof(double), s, T->d_F, T->d_w, fb,
+                hipLaunchKernelGGL(k_bwd_small, dim3(std::min(YCHUNK, G.batch - y0)),
+                                   dim3(SWEEP_T), (size_t)(Sp * Sp + 4 * Sp) * sizeof(double), s, T->d_F, T->d_w, fb,
                                    G.wbase + o, G.n, Sp, fuse ? Ip : 0, G.batch, y0);
             }
           for (int y0 = 0; y0 < G.batch; y0 += YCHUNK)
