@@ -82,3 +82,36 @@ def test_trace_solver_refactor_and_resolve():
             ts.solve(rhs * scale, x)
             t.cuda.synchronize()
             assert H.rel_fro(x.cpu().numpy(), lu.solve(b * scale)) < 1e-8
+
+
+@pytest.mark.parametrize("dim,n,k", [(2, (24, 24), 2), (3, (4, 4, 4), 1)])
+def test_trace_solver_bitwise_repeatable(dim, n, k):
+    """the same matrix factored again after a different one gives the same
+    solution bit for bit: every front entry is assembled afresh (nothing of
+    the previous factorisation survives in the fronts) and every sum runs
+    in a fixed order"""
+    t = _torch()
+    print(f"case trace solve repeatable dim={dim} n={n} k={k}", flush=True)
+    from ddhdg_amd.np_solver import NPSolver
+    prob, params, adim = configs.diode(dim, n, k)
+    s = NPSolver(prob, params, adim)
+    ctx = s.ctx
+    ts = s.trace_solver()
+    systems = []
+    for seed in (3, 4):
+        s.set_state(*H.perturbed(*H.equilibrium_guess(prob, adim, k), seed=seed))
+        vals = t.empty(ctx.nnz, dtype=t.float64, device="cuda")
+        rhs = t.empty(ctx.ntrace, dtype=t.float64, device="cuda")
+        dirv = t.zeros(ctx.ntrace, dtype=t.float64, device="cuda")
+        ctx.condense(s.x_cell, s.x_trace, False, vals, rhs, dirv)
+        systems.append((vals, rhs))
+    xs = []
+    for i in (0, 1, 0):
+        vals, rhs = systems[i]
+        x = t.empty(ctx.ntrace, dtype=t.float64, device="cuda")
+        ts.factor(vals, check=True)
+        ts.solve(rhs, x)
+        t.cuda.synchronize()
+        xs.append(x)
+    assert not t.equal(xs[0], xs[1])
+    assert t.equal(xs[0], xs[2])
