@@ -150,23 +150,6 @@ namespace hdg
       return cand ? (__float_as_uint(sc) & ~63u) + (unsigned)(63 - lane) + 1u : 0u;
     }
 
-    // a / b for the elimination's multipliers: the hardware reciprocal,
-    // one Newton step on it, the quotient and one residual correction
-    // (within an ulp or two of the IEEE quotient, about half the
-    // instructions of the full division sequence); b = 0 gives inf or nan
-    // as the division does
-    __device__ __forceinline__ double pivot_div(double a, double b)
-    {
-      double r = __builtin_amdgcn_rcp(b);
-      r        = fma(fma(-b, r, 1.0), r, r);
-      double q = a * r;
-      return fma(fma(-b, q, a), r, q);
-    }
-    __device__ __forceinline__ float pivot_div(float a, float b)
-    {
-      return a / b;
-    }
-
     // Gauss-Jordan over the NS scalar rows of the lanes' columns with
     // scaled column pivoting: equation k takes the unpivoted scalar lane
     // with the largest |entry| / column max; that column reaches every lane

@@ -187,7 +187,7 @@ namespace hdg
 #pragma unroll
           for (int r = 0; r < NS; ++r)
             bq[r] = lane_value(R[r], pl);
-          const Real f = R[k] / bq[k];
+          const Real f = pivot_div(R[k], bq[k]);
 #pragma unroll
           for (int r = 0; r < NS; ++r)
             if (r != k)
@@ -204,7 +204,7 @@ namespace hdg
       for (int k = 0; k < NS; ++k)
         {
           const Real *bq = rec + k * NS;
-          const Real  f  = R[k] / bq[k];
+          const Real  f  = pivot_div(R[k], bq[k]);
 #pragma unroll
           for (int r = 0; r < NS; ++r)
             if (r != k)

@@ -55,4 +55,20 @@ namespace hdg
   {
     return (unsigned)(__double_as_longlong(v) >> 32);
   }
+
+  // a / b for the eliminations' multipliers: the hardware reciprocal, one
+  // Newton step on it, the quotient and one residual correction (within an
+  // ulp or two of the IEEE quotient, about half the instructions of the
+  // full division sequence); b = 0 gives inf or nan as the division does
+  __device__ __forceinline__ double pivot_div(double a, double b)
+  {
+    double r = __builtin_amdgcn_rcp(b);
+    r        = fma(fma(-b, r, 1.0), r, r);
+    double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+  }
+  __device__ __forceinline__ float pivot_div(float a, float b)
+  {
+    return a / b;
+  }
 } // namespace hdg
