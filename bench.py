@@ -29,7 +29,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes one pass-1 launch (32,768 cells of the 256x256 degree-2 workload)
 # moves, from rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a
 # plain device copy of known size): profiles/condense_traffic_2d_k2_256x256.md
-TRAFFIC_PER_LAUNCH = {(256, 2): (96148.88 * 2.0 + 421986.73 * 1.0) * 1024}
+TRAFFIC_PER_LAUNCH = {(256, 2): (96310.27 * 2.0 + 422006.80 * 1.0) * 1024}
 
 
 _T0 = time.perf_counter()
